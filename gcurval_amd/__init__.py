@@ -1,0 +1,441 @@
+"""gcurval_amd — MI355X (gfx950) arc-length calculation and arc-length sampling.
+
+Python host mirror of gcurval's engine interface over the C ABI declared in
+``include/gcurval_c.h`` (ctypes; no torch). The class ``ArclengthCalculateGpu``
+follows ``gcurval::ArclengthCalculateGpu`` (gcurval/arclength_calculate.h:10-18,
+gcurval/detail/cuda/arclength_calculate.h:141-237): construct with a curve batch,
+``createArclengthTable()``, then ``getAllCurveArclengths()`` /
+``getArclength(cid)`` and ``arclengthReverse(...)``.
+
+The HIP library is required: importing works without it, but every call that
+would reach the device raises ``RuntimeError`` when ``libgcurval_amd.so`` is
+missing. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "load_library", "GcvError", "Opts", "Context", "DeviceArray",
+    "EllipseBatch", "BSplineBatch", "ArclengthCalculateGpu",
+    "gcurvalDevice", "gcurvalHost", "gcurvalReverseReturnUs", "gcurvalReverseReturnPoints",
+    "GCV_FP32", "GCV_FP64",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgcurval_amd.so")
+
+# GcurvalEnum (gcurval/base/util.h:71-77)
+gcurvalDevice, gcurvalHost, gcurvalReverseReturnUs, gcurvalReverseReturnPoints = 0, 1, 2, 3
+GCV_FP32, GCV_FP64 = 0, 1
+
+_STATUS = {0: "GCV_OK", 1: "GCV_ERR_INVALID", 2: "GCV_ERR_OOM", 3: "GCV_ERR_HIP",
+           4: "GCV_ERR_UNSUPPORTED", 5: "GCV_ERR_STATE"}
+
+
+class GcvError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("gl_order", C.c_int),
+        ("n_subdivided", C.c_int),
+        ("max_depth", C.c_int),
+        ("tolerance", C.c_double),
+        ("normalize", C.c_int),
+        ("newton_iters", C.c_int),
+        ("reserved", C.c_int * 6),
+    ]
+
+
+class _BSplineBatchC(C.Structure):
+    _fields_ = [
+        ("n_curve", C.c_int),
+        ("ctrl_stride", C.c_int),
+        ("degree", C.c_void_p),
+        ("dim", C.c_void_p),
+        ("knot_off", C.c_void_p),
+        ("ctrl_off", C.c_void_p),
+        ("knots", C.c_void_p),
+        ("ctrl", C.c_void_p),
+    ]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_SIGS = [
+    ("gcv_version", C.c_char_p, []),
+    ("gcv_opts_default", None, [C.POINTER(Opts)]),
+    ("gcv_ctx_create", C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("gcv_ctx_destroy", C.c_int, [C.c_void_p]),
+    ("gcv_last_error", C.c_char_p, [C.c_void_p]),
+    ("gcv_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("gcv_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gcv_arena_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("gcv_copy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("gcv_copy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("gcv_sync", C.c_int, [C.c_void_p]),
+    ("gcv_device_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
+    ("gcv_timer_start", C.c_int, [C.c_void_p]),
+    ("gcv_timer_stop", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("gcv_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
+    ("gcv_profile_reset", C.c_int, [C.c_void_p]),
+    ("gcv_profile_query", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    ("gcv_table_create_ellipse", C.c_int,
+     [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(C.c_void_p)]),
+    ("gcv_table_create_bspline", C.c_int,
+     [C.c_void_p, C.POINTER(_BSplineBatchC), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
+    ("gcv_table_build", C.c_int, [C.c_void_p]),
+    ("gcv_table_destroy", C.c_int, [C.c_void_p]),
+    ("gcv_table_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    ("gcv_table_lengths", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gcv_table_device_views", C.c_int,
+     [C.c_void_p] + [C.POINTER(C.c_void_p)] * 5),
+    ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5),
+    ("gcv_sample", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("gcv_host_bspline_span_poly", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+]
+
+
+def load_library(path: str | None = None):
+    """Load libgcurval_amd.so (raises RuntimeError if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"gcurval_amd: HIP library not built ({p}); run python -m gcurval_amd.build")
+    L = C.CDLL(p)
+    for name, res, args in _SIGS:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def lib():
+    return load_library()
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGS]
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class DeviceArray:
+    """A device allocation from the context arena with a numpy dtype."""
+
+    def __init__(self, ctx: "Context", count: int, dtype):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(lib().gcv_alloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, host: np.ndarray) -> "DeviceArray":
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        assert h.size == self.count, (h.size, self.count)
+        self.ctx._check(lib().gcv_copy_h2d(self.ctx.h, self.ptr, _ptr(h), self.nbytes))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h is not None:
+            lib().gcv_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+
+class Context:
+    """One HIP device, one stream, one arena (arena_bytes=0: hipMalloc per allocation)."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.gcv_ctx_create(device, arena_bytes, C.byref(h))
+        if rc != 0:
+            raise GcvError(rc, f"gcv_ctx_create(device={device}) failed")
+        self.h = h.value
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise GcvError(rc, lib().gcv_last_error(self.h).decode())
+
+    def array(self, host: np.ndarray, dtype=None) -> DeviceArray:
+        a = np.ascontiguousarray(host, dtype=dtype or np.asarray(host).dtype)
+        return DeviceArray(self, a.size, a.dtype).upload(a)
+
+    def empty(self, count: int, dtype) -> DeviceArray:
+        return DeviceArray(self, count, dtype)
+
+    def sync(self):
+        self._check(lib().gcv_sync(self.h))
+
+    def device_info(self):
+        n = C.c_int()
+        buf = C.create_string_buffer(256)
+        self._check(lib().gcv_device_info(self.h, C.byref(n), buf, 256))
+        return n.value, buf.value.decode()
+
+    def timer_start(self):
+        self._check(lib().gcv_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        self._check(lib().gcv_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def profile(self, on: bool):
+        self._check(lib().gcv_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        self._check(lib().gcv_profile_reset(self.h))
+
+    def profile_query(self, prefix: str):
+        t = C.c_double()
+        n = C.c_int64()
+        self._check(lib().gcv_profile_query(self.h, prefix.encode(), C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def arena_stats(self):
+        u, c = C.c_size_t(), C.c_size_t()
+        self._check(lib().gcv_arena_stats(self.h, C.byref(u), C.byref(c)))
+        return u.value, c.value
+
+    def close(self):
+        if self.h is not None:
+            lib().gcv_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def make_opts(precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0, tolerance=0.0,
+              normalize=False, newton_iters=0) -> Opts:
+    o = Opts()
+    lib().gcv_opts_default(C.byref(o))
+    o.precision = precision
+    o.gl_order = gl_order
+    o.n_subdivided = n_subdivided
+    o.max_depth = max_depth
+    o.tolerance = tolerance
+    o.normalize = 1 if normalize else 0
+    o.newton_iters = newton_iters
+    return o
+
+
+class EllipseBatch:
+    """Ellipses x = a cos t, y = b sin t (samples/sample1_ellipse/ellipse.h:17-34)."""
+
+    kind = "ellipse"
+
+    def __init__(self, a, b):
+        self.a = np.ascontiguousarray(a, dtype=np.float64)
+        self.b = np.ascontiguousarray(b, dtype=np.float64)
+        assert self.a.shape == self.b.shape and self.a.ndim == 1
+        self.n_curve = self.a.size
+
+    @property
+    def n_intervals(self):
+        return self.n_curve
+
+
+class BSplineBatch:
+    """Clamped B-splines in CSR layout (samples/sample2_bspline/structs.h:6-25).
+
+    knots[knot_off[c]:knot_off[c+1]] and ctrl[ctrl_off[c]:ctrl_off[c+1], :dim[c]]
+    for curve c; ctrl has ctrl_stride (2 or 3) columns.
+    """
+
+    kind = "bspline"
+
+    def __init__(self, degree, dim, knot_off, ctrl_off, knots, ctrl):
+        self.degree = np.ascontiguousarray(degree, dtype=np.int32)
+        self.dim = np.ascontiguousarray(dim, dtype=np.int32)
+        self.knot_off = np.ascontiguousarray(knot_off, dtype=np.int64)
+        self.ctrl_off = np.ascontiguousarray(ctrl_off, dtype=np.int64)
+        self.knots = np.ascontiguousarray(knots, dtype=np.float64)
+        ctrl = np.ascontiguousarray(ctrl, dtype=np.float64)
+        assert ctrl.ndim == 2 and ctrl.shape[1] in (2, 3)
+        self.ctrl = ctrl
+        self.ctrl_stride = ctrl.shape[1]
+        self.n_curve = self.degree.size
+
+    @property
+    def n_intervals(self):
+        nk = np.diff(self.knot_off)
+        return int(np.sum(nk - 2 * self.degree - 1))
+
+    def intervals_per_curve(self):
+        return np.diff(self.knot_off) - 2 * self.degree.astype(np.int64) - 1
+
+
+class ArclengthCalculateGpu:
+    """Arc-length table + sampler over one curve batch on one GPU.
+
+    Mirrors gcurval::ArclengthCalculateGpu (gcurval/detail/cuda/arclength_calculate.h:141-237):
+    the constructor uploads the batch, createArclengthTable() builds, then the
+    getters and arclengthReverse() use the table.
+    """
+
+    def __init__(self, ctx: Context, batch, precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0,
+                 tolerance=0.0, normalize=False, newton_iters=0):
+        self.ctx = ctx
+        self.batch = batch
+        self.opts = make_opts(precision, gl_order, n_subdivided, max_depth, tolerance, normalize, newton_iters)
+        self._bufs = []
+        h = C.c_void_p()
+        if batch.kind == "ellipse":
+            da, db = ctx.array(batch.a), ctx.array(batch.b)
+            self._bufs += [da, db]
+            ctx._check(lib().gcv_table_create_ellipse(ctx.h, batch.n_curve, da.ptr, db.ptr, C.byref(self.opts),
+                                                      C.byref(h)))
+        else:
+            bufs = [ctx.array(batch.degree), ctx.array(batch.dim), ctx.array(batch.knot_off),
+                    ctx.array(batch.ctrl_off), ctx.array(batch.knots), ctx.array(batch.ctrl.reshape(-1))]
+            self._bufs += bufs
+            bc = _BSplineBatchC(batch.n_curve, batch.ctrl_stride, *[b.ptr for b in bufs])
+            ctx._check(lib().gcv_table_create_bspline(ctx.h, C.byref(bc), C.byref(self.opts), C.byref(h)))
+        self.h = h.value
+        nc, ni, ns = C.c_int(), C.c_int64(), C.c_int()
+        ctx._check(lib().gcv_table_info(self.h, C.byref(nc), C.byref(ni), C.byref(ns)))
+        self.n_curve, self.n_intervals, self.n_subdivided = nc.value, ni.value, ns.value
+        self._lengths = None
+
+    # -- table build (createArclengthTable) --
+    def createArclengthTable(self):
+        self.ctx._check(lib().gcv_table_build(self.h))
+        self._lengths = None
+
+    def getAllCurveArclengths(self) -> np.ndarray:
+        out = np.empty(self.n_curve, dtype=np.float64)
+        self.ctx._check(lib().gcv_table_lengths(self.h, _ptr(out)))
+        self._lengths = out
+        return out.copy()
+
+    def getArclength(self, cid: int) -> float:
+        if self._lengths is None:
+            self.getAllCurveArclengths()
+        return float(self._lengths[cid])
+
+    def clearArclength(self):
+        self._lengths = None
+
+    def stats(self):
+        v = [C.c_uint64() for _ in range(5)]
+        self.ctx._check(lib().gcv_table_stats(self.h, *[C.byref(x) for x in v]))
+        keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks")
+        return {k: x.value for k, x in zip(keys, v)}
+
+    def device_views(self):
+        v = [C.c_void_p() for _ in range(5)]
+        self.ctx._check(lib().gcv_table_device_views(self.h, *[C.byref(x) for x in v]))
+        return [x.value for x in v]
+
+    def tables(self):
+        """Host copies: (int_cum[n_int], root_prefix[n_int*n_sub], deep[n_int], curve_len, int_off)."""
+        ic, rp, dp, cl, io = self.device_views()
+        L = lib()
+        n_int, ns = self.n_intervals, self.n_subdivided
+        out = (np.empty(n_int, np.float64), np.empty(n_int * ns, np.float64), np.empty(n_int, np.uint64),
+               np.empty(self.n_curve, np.float64), np.empty(self.n_curve + 1, np.int64))
+        for arr, p in zip(out, (ic, rp, dp, cl, io)):
+            if arr.nbytes:
+                self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(arr), p, arr.nbytes))
+        return out
+
+    # -- sampling (arclengthReverse) --
+    def sample_device(self, d_off: DeviceArray, n_total: int, d_pos: DeviceArray | None, is_relative: bool,
+                      d_out: DeviceArray, d_seg: DeviceArray | None = None, uniform_closed: bool = True):
+        """Enqueue sampling on device buffers (no host sync)."""
+        self.ctx._check(lib().gcv_sample(self.h, d_off.ptr, n_total, d_pos.ptr if d_pos is not None else None,
+                                         1 if uniform_closed else 0, 1 if is_relative else 0, d_out.ptr,
+                                         d_seg.ptr if d_seg is not None else None))
+
+    def arclengthReverse(self, n_ptrs_for_curves, sample_position=None, is_relative=True,
+                         where=gcurvalHost, what=gcurvalReverseReturnUs, uniform_closed=True,
+                         return_segments=False):
+        """Sample parameters for per-curve sample counts.
+
+        sample_position: host array of positions concatenated per curve, or
+        None for uniform positions j/(N_c-1) (uniform_closed) or j/N_c.
+        where=gcurvalHost returns numpy arrays; gcurvalDevice returns the
+        DeviceArray (owned by the caller, free with .free()).
+        """
+        if what != gcurvalReverseReturnUs:
+            raise GcvError(4, "gcurvalReverseReturnPoints is not built")
+        n = np.ascontiguousarray(n_ptrs_for_curves, dtype=np.int64)
+        assert n.size == self.n_curve
+        off = np.zeros(self.n_curve + 1, dtype=np.int64)
+        np.cumsum(n, out=off[1:])
+        total = int(off[-1])
+        d_off = self.ctx.array(off)
+        d_pos = self.ctx.array(np.asarray(sample_position, dtype=np.float64)) if sample_position is not None else None
+        d_out = self.ctx.empty(max(total, 1), np.float64)
+        d_seg = self.ctx.empty(max(total, 1), np.int32) if return_segments else None
+        try:
+            self.sample_device(d_off, total, d_pos, is_relative, d_out, d_seg, uniform_closed)
+            if where == gcurvalDevice:
+                return (d_out, d_seg) if return_segments else d_out
+            u = d_out.download()[:total]
+            seg = d_seg.download()[:total] if d_seg is not None else None
+            d_out.free()
+            if d_seg is not None:
+                d_seg.free()
+            return (u, seg) if return_segments else u
+        finally:
+            d_off.free()
+            if d_pos is not None:
+                d_pos.free()
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            lib().gcv_table_destroy(self.h)
+            self.h = None
+        for b in self._bufs:
+            b.free()
+        self._bufs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def host_bspline_span_poly(p, D, s, knots, ctrl, scale_inv=1.0):
+    """Power-basis coefficients of dC/dx on span s, computed by the library's
+    host build of the setup routine (no GPU needed)."""
+    knots = np.ascontiguousarray(knots, np.float64)
+    ctrl = np.ascontiguousarray(ctrl, np.float64)
+    out = np.zeros(p * D, np.float64)
+    u0, h = C.c_double(), C.c_double()
+    rc = lib().gcv_host_bspline_span_poly(p, D, s, _ptr(knots), _ptr(ctrl), ctrl.shape[1], scale_inv, _ptr(out),
+                                          C.byref(u0), C.byref(h))
+    if rc != 0:
+        raise GcvError(rc, "bspline_span_poly")
+    return out.reshape(D, p), u0.value, h.value

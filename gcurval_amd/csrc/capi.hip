@@ -1,0 +1,750 @@
+// C ABI of the arc-length engine: context + hipMalloc arena, table objects,
+// per-kernel event timing, and the orchestration of the gfx950 kernels.
+//
+// Host orchestration replaces ArclengthCalculateCuda
+// (gcurval/detail/cuda/arclength_calculate.h:93-507): no per-level host
+// round trips, no free+malloc growth of task buffers, no device syncs inside a
+// build; every allocation happens when a table is created.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gcurval_c.h"
+#include "launch.h"
+
+using namespace gcurval;
+using namespace gcurval::rt;
+
+// ===========================================================================
+// Context
+// ===========================================================================
+struct ProfRec {
+    std::string name;
+    hipEvent_t s, e;
+};
+
+struct gcv_ctx {
+    int device = 0;
+    int n_cu = 0;
+    char dev_name[256] = {0};
+    hipStream_t stream = nullptr;
+    std::string err;
+    // arena: one hipMalloc block, first-fit with coalescing (256-byte aligned)
+    char* arena = nullptr;
+    size_t cap = 0, used = 0;
+    std::map<size_t, size_t> free_blocks;           // offset -> size
+    std::unordered_map<void*, size_t> live;         // pointer -> size
+    // timing
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool profiling = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> prof;
+    ProfRec pending;
+    bool pending_on = false;
+};
+
+static int set_err(gcv_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                            \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            return set_err((ctx), GCV_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                           __FILE__, __LINE__);                                                      \
+    } while (0)
+
+static hipEvent_t take_event(gcv_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+namespace gcurval {
+namespace rt {
+void prof_begin(void* owner, const char* name) {
+    gcv_ctx* c = (gcv_ctx*)owner;
+    if (!c || !c->profiling) return;
+    c->pending.name = name;
+    c->pending.s = take_event(c);
+    c->pending.e = take_event(c);
+    if (c->pending.s) hipEventRecord(c->pending.s, c->stream);
+    c->pending_on = true;
+}
+void prof_end(void* owner) {
+    gcv_ctx* c = (gcv_ctx*)owner;
+    if (!c || !c->pending_on) return;
+    if (c->pending.e) hipEventRecord(c->pending.e, c->stream);
+    c->recs.push_back(c->pending);
+    c->pending_on = false;
+}
+}  // namespace rt
+}  // namespace gcurval
+
+static void harvest(gcv_ctx* c) {
+    if (c->recs.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (ProfRec& r : c->recs) {
+        float ms = 0.f;
+        if (r.s && r.e && hipEventElapsedTime(&ms, r.s, r.e) == hipSuccess) {
+            auto& slot = c->prof[r.name];
+            slot.first += ms;
+            slot.second += 1;
+        }
+        if (r.s) c->event_pool.push_back(r.s);
+        if (r.e) c->event_pool.push_back(r.e);
+    }
+    c->recs.clear();
+}
+
+static LaunchCfg cfg_of(gcv_ctx* c) { return LaunchCfg{c->stream, c->n_cu, c}; }
+
+extern "C" {
+
+const char* gcv_version(void) { return "gcurval-amd 0.1 (gfx950)"; }
+
+void gcv_opts_default(gcv_opts* o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->precision = GCV_FP32;
+    o->gl_order = 3;
+    o->n_subdivided = 64;
+    o->max_depth = 0;
+    o->tolerance = 0.0;
+    o->normalize = 0;
+    o->newton_iters = 0;
+}
+
+int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
+    if (!out) return GCV_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GCV_ERR_HIP;
+    if (device < 0 || device >= n) return GCV_ERR_INVALID;
+    gcv_ctx* c = new gcv_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return GCV_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->n_cu = prop.multiProcessorCount;
+        snprintf(c->dev_name, sizeof c->dev_name, "%s", prop.gcnArchName);
+    }
+    if (c->n_cu <= 0) c->n_cu = 256;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GCV_ERR_HIP; }
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return GCV_ERR_HIP;
+    }
+    if (arena_bytes > 0) {
+        arena_bytes = (arena_bytes + 255) & ~(size_t)255;
+        if (hipMalloc((void**)&c->arena, arena_bytes) != hipSuccess) {
+            hipEventDestroy(c->t0);
+            hipEventDestroy(c->t1);
+            hipStreamDestroy(c->stream);
+            delete c;
+            return GCV_ERR_OOM;
+        }
+        c->cap = arena_bytes;
+        c->free_blocks[0] = arena_bytes;
+    }
+    *out = c;
+    return GCV_OK;
+}
+
+int gcv_ctx_destroy(gcv_ctx* c) {
+    if (!c) return GCV_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    harvest(c);
+    for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+    if (c->arena) hipFree(c->arena);
+    else
+        for (auto& kv : c->live) hipFree(kv.first);
+    hipEventDestroy(c->t0);
+    hipEventDestroy(c->t1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return GCV_OK;
+}
+
+const char* gcv_last_error(const gcv_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gcv_alloc(gcv_ctx* c, size_t bytes, void** dptr) {
+    if (!c || !dptr) return GCV_ERR_INVALID;
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (!c->arena) {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess)
+            return set_err(c, GCV_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+        c->live[p] = bytes;
+        c->used += bytes;
+        *dptr = p;
+        return GCV_OK;
+    }
+    for (auto it = c->free_blocks.begin(); it != c->free_blocks.end(); ++it) {
+        if (it->second >= bytes) {
+            size_t off = it->first, sz = it->second;
+            c->free_blocks.erase(it);
+            if (sz > bytes) c->free_blocks[off + bytes] = sz - bytes;
+            void* p = c->arena + off;
+            c->live[p] = bytes;
+            c->used += bytes;
+            *dptr = p;
+            return GCV_OK;
+        }
+    }
+    return set_err(c, GCV_ERR_OOM, "arena exhausted: %zu bytes requested, %zu of %zu in use", bytes, c->used,
+                   c->cap);
+}
+
+int gcv_free(gcv_ctx* c, void* p) {
+    if (!c) return GCV_ERR_INVALID;
+    if (!p) return GCV_OK;
+    auto it = c->live.find(p);
+    if (it == c->live.end()) return set_err(c, GCV_ERR_INVALID, "gcv_free of unknown pointer %p", p);
+    size_t sz = it->second;
+    c->live.erase(it);
+    c->used -= sz;
+    if (!c->arena) {
+        hipStreamSynchronize(c->stream);
+        HIPCHK(c, hipFree(p));
+        return GCV_OK;
+    }
+    // stream-ordered reuse: later work on the same stream may take the block
+    size_t off = (size_t)((char*)p - c->arena);
+    auto nx = c->free_blocks.lower_bound(off);
+    if (nx != c->free_blocks.end() && nx->first == off + sz) {
+        sz += nx->second;
+        nx = c->free_blocks.erase(nx);
+    }
+    if (nx != c->free_blocks.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += sz;
+            return GCV_OK;
+        }
+    }
+    c->free_blocks[off] = sz;
+    return GCV_OK;
+}
+
+int gcv_arena_stats(const gcv_ctx* c, size_t* used, size_t* capacity) {
+    if (!c) return GCV_ERR_INVALID;
+    if (used) *used = c->used;
+    if (capacity) *capacity = c->cap;
+    return GCV_OK;
+}
+
+int gcv_copy_h2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
+    if (!bytes) return GCV_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GCV_OK;
+}
+
+int gcv_copy_d2h(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
+    if (!bytes) return GCV_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GCV_OK;
+}
+
+int gcv_sync(gcv_ctx* c) {
+    if (!c) return GCV_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GCV_OK;
+}
+
+int gcv_device_info(gcv_ctx* c, int* n_cu, char* name, int name_len) {
+    if (!c) return GCV_ERR_INVALID;
+    if (n_cu) *n_cu = c->n_cu;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->dev_name);
+    return GCV_OK;
+}
+
+int gcv_timer_start(gcv_ctx* c) {
+    if (!c) return GCV_ERR_INVALID;
+    HIPCHK(c, hipEventRecord(c->t0, c->stream));
+    return GCV_OK;
+}
+
+int gcv_timer_stop(gcv_ctx* c, double* ms) {
+    if (!c || !ms) return GCV_ERR_INVALID;
+    HIPCHK(c, hipEventRecord(c->t1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->t1));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->t0, c->t1));
+    *ms = f;
+    return GCV_OK;
+}
+
+int gcv_profile_enable(gcv_ctx* c, int on) {
+    if (!c) return GCV_ERR_INVALID;
+    c->profiling = on != 0;
+    return GCV_OK;
+}
+
+int gcv_profile_reset(gcv_ctx* c) {
+    if (!c) return GCV_ERR_INVALID;
+    harvest(c);
+    c->prof.clear();
+    return GCV_OK;
+}
+
+int gcv_profile_query(gcv_ctx* c, const char* prefix, double* total_ms, int64_t* count) {
+    if (!c || !prefix) return GCV_ERR_INVALID;
+    harvest(c);
+    double t = 0;
+    int64_t n = 0;
+    const size_t lp = strlen(prefix);
+    for (auto& kv : c->prof)
+        if (kv.first.compare(0, lp, prefix) == 0) {
+            t += kv.second.first;
+            n += kv.second.second;
+        }
+    if (total_ms) *total_ms = t;
+    if (count) *count = n;
+    return GCV_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Tables
+// ===========================================================================
+enum { KIND_ELLIPSE = 0, KIND_BSPLINE = 1 };
+
+struct ClassRange {
+    int p, D;
+    int64_t offset, count;  // into class_list
+};
+
+struct gcv_table {
+    gcv_ctx* ctx = nullptr;
+    int kind = KIND_ELLIPSE;
+    gcv_opts o{};
+    int n_curve = 0;
+    int64_t n_int = 0;
+    int log2_nsub = 6;
+    double eps0 = 0;  // per-root level-0 tolerance, exact value of the build's Dt
+    int stack_cap = 0;
+    bool built = false;
+    // curve data (caller-owned, device)
+    const double* a = nullptr;
+    const double* b = nullptr;
+    gcv_bspline_batch bat{};
+    std::vector<ClassRange> classes;
+    int uniform_class = 0;
+    // owned device arrays
+    int64_t* ioff = nullptr;
+    double* int_cum = nullptr;
+    double* root_prefix = nullptr;
+    uint64_t* deep = nullptr;
+    double* curve_len = nullptr;
+    double* curve_scale = nullptr;
+    double* int_scale = nullptr;
+    int32_t* int_curve = nullptr;
+    double* coef64 = nullptr;
+    float* coef32 = nullptr;
+    double* u0 = nullptr;
+    double* h = nullptr;
+    int32_t* class_list = nullptr;
+    int32_t* class_of_curve = nullptr;
+    unsigned long long* stats = nullptr;
+    std::vector<void*> owned;
+};
+
+template <typename T>
+static int talloc(gcv_table* t, T** p, size_t count) {
+    void* v = nullptr;
+    int rc = gcv_alloc(t->ctx, sizeof(T) * (count ? count : 1), &v);
+    if (rc) return rc;
+    t->owned.push_back(v);
+    *p = (T*)v;
+    return GCV_OK;
+}
+
+static void table_free(gcv_table* t) {
+    if (!t) return;
+    if (t->ctx) {
+        hipStreamSynchronize(t->ctx->stream);
+        for (void* p : t->owned) gcv_free(t->ctx, p);
+    }
+    delete t;
+}
+
+static int resolve_opts(gcv_ctx* c, const gcv_opts* in, gcv_opts* o) {
+    gcv_opts_default(o);
+    if (in) *o = *in;
+    if (o->precision != GCV_FP32 && o->precision != GCV_FP64)
+        return set_err(c, GCV_ERR_INVALID, "precision must be GCV_FP32 or GCV_FP64");
+    if (o->gl_order != 3 && o->gl_order != 5)
+        return set_err(c, GCV_ERR_UNSUPPORTED, "gl_order %d not built (3 or 5)", o->gl_order);
+    if (o->n_subdivided < 1 || o->n_subdivided > 64 || (o->n_subdivided & (o->n_subdivided - 1)))
+        return set_err(c, GCV_ERR_INVALID, "n_subdivided must be a power of two in [1, 64]");
+    if (o->max_depth == 0) o->max_depth = o->precision == GCV_FP32 ? 14 : 23;
+    const int maxd = o->precision == GCV_FP32 ? 18 : 31;
+    if (o->max_depth < 0 || o->max_depth > maxd)
+        return set_err(c, GCV_ERR_INVALID, "max_depth must be in [0, %d]", maxd);
+    if (o->tolerance == 0.0) o->tolerance = o->precision == GCV_FP32 ? 1e-6 : 1e-14;
+    if (!(o->tolerance > 0.0)) return set_err(c, GCV_ERR_INVALID, "tolerance must be positive");
+    if (o->newton_iters == 0) o->newton_iters = o->precision == GCV_FP32 ? 3 : 4;
+    if (o->newton_iters < 0 || o->newton_iters > 32) return set_err(c, GCV_ERR_INVALID, "newton_iters out of range");
+    return GCV_OK;
+}
+
+static int log2i(int v) {
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+// Allocation shared by both families.
+static int table_alloc_common(gcv_table* t) {
+    int rc;
+    const int64_t nroots = t->n_int << t->log2_nsub;
+    if ((rc = talloc(t, &t->ioff, t->n_curve + 1))) return rc;
+    if ((rc = talloc(t, &t->int_cum, t->n_int))) return rc;
+    if ((rc = talloc(t, &t->root_prefix, nroots))) return rc;
+    if ((rc = talloc(t, &t->deep, t->n_int))) return rc;
+    if ((rc = talloc(t, &t->curve_len, t->n_curve))) return rc;
+    if ((rc = talloc(t, &t->stats, dev::kStatCount))) return rc;
+    // depth bound of the wave-local stack: 64 (max_depth + 1), plus one block of slack
+    t->stack_cap = 64 * (t->o.max_depth + 2);
+    const int nsub = 1 << t->log2_nsub;
+    if (t->o.precision == GCV_FP32) t->eps0 = (double)((float)t->o.tolerance / (float)nsub);
+    else t->eps0 = t->o.tolerance / (double)nsub;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_create_ellipse(gcv_ctx* c, int n_curve, const double* d_a, const double* d_b,
+                                        const gcv_opts* opts, gcv_table** out) {
+    if (!c || !out) return GCV_ERR_INVALID;
+    *out = nullptr;
+    if (n_curve < 0 || (n_curve > 0 && (!d_a || !d_b))) return set_err(c, GCV_ERR_INVALID, "bad ellipse batch");
+    gcv_table* t = new gcv_table;
+    t->ctx = c;
+    t->kind = KIND_ELLIPSE;
+    int rc = resolve_opts(c, opts, &t->o);
+    if (rc) { delete t; return rc; }
+    t->n_curve = n_curve;
+    t->n_int = n_curve;
+    t->log2_nsub = log2i(t->o.n_subdivided);
+    t->a = d_a;
+    t->b = d_b;
+    if ((rc = table_alloc_common(t))) { table_free(t); return rc; }
+    std::vector<int64_t> ioff(n_curve + 1);
+    for (int i = 0; i <= n_curve; i++) ioff[i] = i;
+    if ((rc = gcv_copy_h2d(c, t->ioff, ioff.data(), sizeof(int64_t) * (n_curve + 1)))) { table_free(t); return rc; }
+    *out = t;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, const gcv_opts* opts,
+                                        gcv_table** out) {
+    if (!c || !B || !out) return GCV_ERR_INVALID;
+    *out = nullptr;
+    const int n = B->n_curve;
+    if (n < 0) return set_err(c, GCV_ERR_INVALID, "n_curve < 0");
+    if (n > 0 && (!B->degree || !B->dim || !B->knot_off || !B->ctrl_off || !B->knots || !B->ctrl))
+        return set_err(c, GCV_ERR_INVALID, "null array in B-spline batch");
+    if (B->ctrl_stride != 2 && B->ctrl_stride != 3) return set_err(c, GCV_ERR_INVALID, "ctrl_stride must be 2 or 3");
+    gcv_table* t = new gcv_table;
+    t->ctx = c;
+    t->kind = KIND_BSPLINE;
+    int rc = resolve_opts(c, opts, &t->o);
+    if (rc) { delete t; return rc; }
+    t->bat = *B;
+    t->n_curve = n;
+    t->log2_nsub = log2i(t->o.n_subdivided);
+    // per-curve metadata to host for validation and planning
+    std::vector<int32_t> deg(n), dim(n);
+    std::vector<int64_t> ko(n + 1), co(n + 1);
+    if (n > 0) {
+        if ((rc = gcv_copy_d2h(c, deg.data(), B->degree, sizeof(int32_t) * n)) ||
+            (rc = gcv_copy_d2h(c, dim.data(), B->dim, sizeof(int32_t) * n)) ||
+            (rc = gcv_copy_d2h(c, ko.data(), B->knot_off, sizeof(int64_t) * (n + 1))) ||
+            (rc = gcv_copy_d2h(c, co.data(), B->ctrl_off, sizeof(int64_t) * (n + 1)))) {
+            delete t;
+            return rc;
+        }
+    } else {
+        ko[0] = co[0] = 0;
+    }
+    std::vector<int64_t> ioff(n + 1, 0);
+    std::map<int, std::vector<int>> by_class;
+    for (int i = 0; i < n; i++) {
+        const int p = deg[i], D = dim[i];
+        const int64_t nk = ko[i + 1] - ko[i], np = co[i + 1] - co[i];
+        if (p < 1 || p > 5) { delete t; return set_err(c, GCV_ERR_UNSUPPORTED, "curve %d: degree %d not in [1, 5]", i, p); }
+        if (D < 2 || D > 3 || D > B->ctrl_stride) { delete t; return set_err(c, GCV_ERR_INVALID, "curve %d: bad dim %d", i, D); }
+        if (nk < 2 * p + 2) { delete t; return set_err(c, GCV_ERR_INVALID, "curve %d: %lld knots < 2p+2", i, (long long)nk); }
+        if (np != nk - p - 1) { delete t; return set_err(c, GCV_ERR_INVALID, "curve %d: %lld control points != n_knots-p-1", i, (long long)np); }
+        ioff[i + 1] = ioff[i] + (nk - 2 * p - 1);
+        by_class[bspline_class(p, D)].push_back(i);
+    }
+    t->n_int = ioff[n];
+    if (t->n_int > INT32_MAX) { delete t; return set_err(c, GCV_ERR_UNSUPPORTED, "more than 2^31-1 intervals"); }
+    if ((rc = table_alloc_common(t))) { table_free(t); return rc; }
+    if ((rc = gcv_copy_h2d(c, t->ioff, ioff.data(), sizeof(int64_t) * (n + 1)))) { table_free(t); return rc; }
+    if ((rc = talloc(t, &t->curve_scale, n)) || (rc = talloc(t, &t->int_scale, t->n_int)) ||
+        (rc = talloc(t, &t->int_curve, t->n_int)) || (rc = talloc(t, &t->coef64, t->n_int * dev::kCoefStride)) ||
+        (rc = talloc(t, &t->u0, t->n_int)) || (rc = talloc(t, &t->h, t->n_int))) {
+        table_free(t);
+        return rc;
+    }
+    if (t->o.precision == GCV_FP32 && (rc = talloc(t, &t->coef32, t->n_int * dev::kCoefStride))) {
+        table_free(t);
+        return rc;
+    }
+    // class plan: a single class needs no interval list
+    if (by_class.size() <= 1) {
+        if (!by_class.empty()) {
+            const int cls = by_class.begin()->first;
+            t->classes.push_back(ClassRange{cls / 10, cls % 10, 0, t->n_int});
+            t->uniform_class = cls;
+        }
+    } else {
+        std::vector<int32_t> list;
+        list.reserve(t->n_int);
+        std::vector<int32_t> coc(n);
+        for (auto& kv : by_class) {
+            ClassRange r{kv.first / 10, kv.first % 10, (int64_t)list.size(), 0};
+            for (int cv : kv.second) {
+                coc[cv] = kv.first;
+                for (int64_t i = ioff[cv]; i < ioff[cv + 1]; i++) list.push_back((int32_t)i);
+            }
+            r.count = (int64_t)list.size() - r.offset;
+            t->classes.push_back(r);
+        }
+        if ((rc = talloc(t, &t->class_list, list.size())) ||
+            (rc = gcv_copy_h2d(c, t->class_list, list.data(), sizeof(int32_t) * list.size())) ||
+            (rc = talloc(t, &t->class_of_curve, n)) ||
+            (rc = gcv_copy_h2d(c, t->class_of_curve, coc.data(), sizeof(int32_t) * n))) {
+            table_free(t);
+            return rc;
+        }
+    }
+    *out = t;
+    return GCV_OK;
+}
+
+template <typename Dt>
+static dev::SubdivArgs<Dt> subdiv_args(gcv_table* t) {
+    dev::SubdivArgs<Dt> A{};
+    A.n_slots = t->n_int;
+    A.int_list = nullptr;
+    A.log2_nsub = t->log2_nsub;
+    A.eps0 = (Dt)t->eps0;
+    A.max_depth = t->o.max_depth;
+    A.root_prefix = t->root_prefix;
+    A.int_len = t->int_cum;
+    A.curve_len = t->kind == KIND_ELLIPSE ? t->curve_len : nullptr;
+    A.deep = t->deep;
+    A.stack_cap = t->stack_cap;
+    A.stats = t->stats;
+    return A;
+}
+
+static BSplineTableDev bspline_dev(gcv_table* t) {
+    return BSplineTableDev{t->coef64, t->coef32, t->int_scale, t->u0, t->h};
+}
+
+template <typename Dt, int NGL>
+static hipError_t build_impl(gcv_table* t) {
+    gcv_ctx* c = t->ctx;
+    const LaunchCfg L = cfg_of(c);
+    hipError_t e;
+    prof_begin(c, "k_fill_stats");
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, t->stats, (int)dev::kStatCount, 0ull);
+    e = hipGetLastError();
+    prof_end(c);
+    if (e != hipSuccess) return e;
+    dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
+    if (t->kind == KIND_ELLIPSE) {
+        return Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
+    }
+    // B-spline: scale + interval map, coefficients, subdivision per class, scan
+    dev::BSplineDev B{t->n_curve, t->bat.ctrl_stride, t->bat.degree, t->bat.dim, t->bat.knot_off,
+                      t->bat.ctrl_off, t->bat.knots, t->bat.ctrl};
+    if (t->n_curve > 0) {
+        const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 8);
+        prof_begin(c, "k_bspline_curve_setup");
+        hipLaunchKernelGGL(dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0, c->stream, B, t->ioff,
+                           t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
+        e = hipGetLastError();
+        prof_end(c);
+        if (e != hipSuccess) return e;
+    }
+    if (t->n_int > 0) {
+        const int64_t blocks = (t->n_int + 255) / 256;
+        prof_begin(c, "k_bspline_coef");
+        hipLaunchKernelGGL(dev::k_bspline_coef, dim3((unsigned)blocks), dim3(256), 0, c->stream, B, t->ioff, t->n_int,
+                           t->int_curve, t->int_scale, t->coef64, t->coef32, t->u0, t->h);
+        e = hipGetLastError();
+        prof_end(c);
+        if (e != hipSuccess) return e;
+    }
+    const BSplineTableDev T = bspline_dev(t);
+    for (const ClassRange& r : t->classes) {
+        dev::SubdivArgs<Dt> Ar = A;
+        if (t->class_list) {
+            Ar.int_list = t->class_list + r.offset;
+            Ar.n_slots = r.count;
+        }
+        e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, Ar);
+        if (e != hipSuccess) return e;
+    }
+    if (t->n_curve > 0) {
+        const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 16);
+        prof_begin(c, "k_curve_scan");
+        hipLaunchKernelGGL(dev::k_curve_scan, dim3((unsigned)blocks), dim3(256), 0, c->stream, t->n_curve, t->ioff,
+                           t->int_cum, t->curve_len);
+        e = hipGetLastError();
+        prof_end(c);
+    }
+    return e;
+}
+
+extern "C" int gcv_table_build(gcv_table* t) {
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    hipError_t e;
+    if (t->o.precision == GCV_FP32) e = t->o.gl_order == 3 ? build_impl<float, 3>(t) : build_impl<float, 5>(t);
+    else e = t->o.gl_order == 3 ? build_impl<double, 3>(t) : build_impl<double, 5>(t);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "table build launch failed: %s", hipGetErrorString(e));
+    t->built = true;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_destroy(gcv_table* t) {
+    table_free(t);
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_intervals, int* n_subdivided) {
+    if (!t) return GCV_ERR_INVALID;
+    if (n_curve) *n_curve = t->n_curve;
+    if (n_intervals) *n_intervals = t->n_int;
+    if (n_subdivided) *n_subdivided = 1 << t->log2_nsub;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_lengths(gcv_table* t, double* host_out) {
+    if (!t || (!host_out && t->n_curve > 0)) return GCV_ERR_INVALID;
+    if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
+    return gcv_copy_d2h(t->ctx, host_out, t->curve_len, sizeof(double) * t->n_curve);
+}
+
+extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** root_prefix,
+                                      const uint64_t** deep_mask, const double** curve_len,
+                                      const int64_t** int_off) {
+    if (!t) return GCV_ERR_INVALID;
+    if (int_cum) *int_cum = t->int_cum;
+    if (root_prefix) *root_prefix = t->root_prefix;
+    if (deep_mask) *deep_mask = t->deep;
+    if (curve_len) *curve_len = t->curve_len;
+    if (int_off) *int_off = t->ioff;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
+                               uint64_t* overflow, uint64_t* tasks) {
+    if (!t) return GCV_ERR_INVALID;
+    if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
+    unsigned long long s[dev::kStatCount];
+    int rc = gcv_copy_d2h(t->ctx, s, t->stats, sizeof s);
+    if (rc) return rc;
+    if (quadratures) *quadratures = s[dev::kStatQuad];
+    if (deep_roots) *deep_roots = s[dev::kStatDeepRoots];
+    if (max_stack) *max_stack = s[dev::kStatMaxStack];
+    if (overflow) *overflow = s[dev::kStatOverflow];
+    if (tasks) *tasks = s[dev::kStatTasks];
+    return GCV_OK;
+}
+
+extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
+                          int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg) {
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (!t->built) return set_err(c, GCV_ERR_STATE, "arclengthReverse before createArclengthTable");
+    if (n_total < 0) return set_err(c, GCV_ERR_INVALID, "n_total < 0");
+    if (n_total == 0) return GCV_OK;
+    if (!d_soff || !d_out_u) return set_err(c, GCV_ERR_INVALID, "null sample offsets or output");
+    if (t->n_curve == 0) return set_err(c, GCV_ERR_INVALID, "samples requested from an empty batch");
+    dev::SampleArgs A{};
+    A.n_curve = t->n_curve;
+    A.n_total = n_total;
+    A.soff = d_soff;
+    A.pos = d_pos;
+    A.uniform_closed = uniform_closed;
+    A.is_relative = is_relative;
+    A.ioff = t->ioff;
+    A.int_cum = t->int_cum;
+    A.curve_len = t->curve_len;
+    A.root_prefix = t->root_prefix;
+    A.deep = t->deep;
+    A.log2_nsub = t->log2_nsub;
+    A.eps0 = t->eps0;
+    A.max_depth = t->o.max_depth;
+    A.newton_iters = t->o.newton_iters;
+    A.out_u = d_out_u;
+    A.out_seg = d_out_seg;
+    A.class_of_curve = t->class_of_curve;
+    const LaunchCfg L = cfg_of(c);
+    hipError_t e;
+    const bool f32 = t->o.precision == GCV_FP32;
+    const bool g3 = t->o.gl_order == 3;
+    if (t->kind == KIND_ELLIPSE) {
+        if (f32) e = g3 ? Launch<float, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A)
+                        : Launch<float, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A);
+        else e = g3 ? Launch<double, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A)
+                    : Launch<double, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A);
+    } else {
+        const BSplineTableDev T = bspline_dev(t);
+        if (f32) e = g3 ? Launch<float, 3>::sample_bspline(L, t->uniform_class, T, A)
+                        : Launch<float, 5>::sample_bspline(L, t->uniform_class, T, A);
+        else e = g3 ? Launch<double, 3>::sample_bspline(L, t->uniform_class, T, A)
+                    : Launch<double, 5>::sample_bspline(L, t->uniform_class, T, A);
+    }
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "sample launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
+extern "C" int gcv_host_bspline_span_poly(int p, int D, int s, const double* knots, const double* ctrl,
+                                          int ctrl_stride, double scale_inv, double* out, double* u0, double* h) {
+    if (p < 1 || p > 5 || D < 2 || D > 3 || !knots || !ctrl || !out || ctrl_stride < D) return GCV_ERR_INVALID;
+    double tmp[dev::kCoefStride];
+    for (int i = 0; i < dev::kCoefStride; i++) tmp[i] = 0.0;
+    int rc = dev::bspline_span_poly(p, D, s, knots, ctrl, ctrl_stride, scale_inv, tmp, u0, h);
+    for (int i = 0; i < p * D; i++) out[i] = tmp[i];
+    return rc ? GCV_ERR_INVALID : GCV_OK;
+}

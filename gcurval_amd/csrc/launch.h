@@ -1,0 +1,56 @@
+// Internal launcher interface between the C ABI (capi.hip) and the kernel
+// instantiations (inst_<precision>_gl<order>.hip). Each instantiation unit
+// compiles the kernels of one (arithmetic type, Gauss-Legendre order) pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gcurval/detail/kernels.h"
+
+namespace gcurval {
+namespace rt {
+
+// Launch context: stream, device size, and profiling hooks of the owner.
+struct LaunchCfg {
+    hipStream_t stream;
+    int n_cu;
+    void* owner;  // gcv_ctx*
+};
+
+// Profiling hooks implemented by the context (events around a launch).
+void prof_begin(void* owner, const char* name);
+void prof_end(void* owner);
+
+// Device arrays of a B-spline table needed by the evaluators.
+struct BSplineTableDev {
+    const double* coef64;
+    const float* coef32;
+    const double* int_scale;
+    const double* u0;
+    const double* h;
+};
+
+// Class of a B-spline interval: 10 * degree + dim.
+inline int bspline_class(int p, int D) { return 10 * p + D; }
+
+template <typename Dt, int NGL>
+struct Launch {
+    // Subdivision of ellipses (one interval per curve).
+    static hipError_t subdivide_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
+                                        dev::SubdivArgs<Dt> A);
+    // Subdivision of one B-spline class (degree p, dimension D).
+    static hipError_t subdivide_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
+                                        dev::SubdivArgs<Dt> A);
+    static hipError_t sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
+                                     const dev::SampleArgs& A);
+    // Sampling of B-splines; class per curve from A.class_of_curve, or
+    // uniform_class when that is null.
+    static hipError_t sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
+                                     const dev::SampleArgs& A);
+    // LDS per block and waves per block of the subdivision kernel.
+    static void subdivide_shape(int stack_cap, int* waves_per_block, size_t* lds_bytes);
+};
+
+}  // namespace rt
+}  // namespace gcurval

@@ -1,0 +1,155 @@
+// Definitions of the launchers; included once per instantiation unit.
+#pragma once
+
+#include "launch.h"
+
+namespace gcurval {
+namespace rt {
+
+using namespace gcurval::dev;
+
+template <typename Dt, int NGL>
+void Launch<Dt, NGL>::subdivide_shape(int stack_cap, int* waves_per_block, size_t* lds_bytes) {
+    const int per_wave = subdiv_lds_per_wave<Dt>(stack_cap);
+    int w = 4;
+    while (w > 1 && w * per_wave > 64 * 1024) w >>= 1;
+    *waves_per_block = w;
+    *lds_bytes = (size_t)w * per_wave;
+}
+
+template <typename Dt, int NGL, class Ev>
+static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name) {
+    if (A.n_slots <= 0) return hipSuccess;
+    int wpb;
+    size_t lds;
+    Launch<Dt, NGL>::subdivide_shape(A.stack_cap, &wpb, &lds);
+    const int threads = 64 * wpb;
+    const int ipw = 64 >> A.log2_nsub;
+    const int64_t n_tiles = (A.n_slots + ipw - 1) / ipw;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subdivide<Dt, NGL, Ev>, threads, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    int64_t blocks = (int64_t)c.n_cu * per_cu;
+    const int64_t need = (n_tiles + wpb - 1) / wpb;
+    if (blocks > need) blocks = need;
+    prof_begin(c.owner, name);
+    hipLaunchKernelGGL((k_subdivide<Dt, NGL, Ev>), dim3((unsigned)blocks), dim3(threads), lds, c.stream, ev, A);
+    e = hipGetLastError();
+    prof_end(c.owner);
+    return e;
+}
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::subdivide_ellipse(const LaunchCfg& c, const double* a, const double* b,
+                                              int normalize, SubdivArgs<Dt> A) {
+    EllipseEval<Dt> ev{a, b, normalize};
+    return run_subdivide<Dt, NGL>(c, ev, A, "k_subdivide_ellipse");
+}
+
+template <typename Dt, int P, int D>
+struct MakeEv;
+template <int P, int D>
+struct MakeEv<float, P, D> {
+    __host__ __device__ static BSplineEval<float, P, D> get(const BSplineTableDev& T) {
+        return BSplineEval<float, P, D>{T.coef32, T.int_scale, T.u0, T.h};
+    }
+};
+template <int P, int D>
+struct MakeEv<double, P, D> {
+    __host__ __device__ static BSplineEval<double, P, D> get(const BSplineTableDev& T) {
+        return BSplineEval<double, P, D>{T.coef64, T.int_scale, T.u0, T.h};
+    }
+};
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::subdivide_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
+                                              SubdivArgs<Dt> A) {
+#define GCV_SUBDIV_CASE(PP, DD) \
+    case 10 * PP + DD:          \
+        return run_subdivide<Dt, NGL>(c, MakeEv<Dt, PP, DD>::get(T), A, "k_subdivide_bspline");
+    switch (bspline_class(p, D)) {
+        GCV_SUBDIV_CASE(1, 2)
+        GCV_SUBDIV_CASE(1, 3)
+        GCV_SUBDIV_CASE(2, 2)
+        GCV_SUBDIV_CASE(2, 3)
+        GCV_SUBDIV_CASE(3, 2)
+        GCV_SUBDIV_CASE(3, 3)
+        GCV_SUBDIV_CASE(4, 2)
+        GCV_SUBDIV_CASE(4, 3)
+        GCV_SUBDIV_CASE(5, 2)
+        GCV_SUBDIV_CASE(5, 3)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef GCV_SUBDIV_CASE
+}
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
+                                           const SampleArgs& A) {
+    if (A.n_total <= 0) return hipSuccess;
+    EllipseEval<Dt> ev{a, b, normalize};
+    EllipseEval<double> ev64{a, b, normalize};
+    const int64_t blocks = (A.n_total + 255) / 256;
+    prof_begin(c.owner, "k_sample_ellipse");
+    hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>>), dim3((unsigned)blocks),
+                       dim3(256), 0, c.stream, ev, ev64, A);
+    hipError_t e = hipGetLastError();
+    prof_end(c.owner);
+    return e;
+}
+
+// Mixed-class B-spline sampler: the class is uniform within a curve, so a
+// wave diverges only where its samples straddle curves of different classes.
+template <typename Dt, int NGL>
+__global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gw0 = g - lane_id();
+    const int64_t gw1 = (gw0 + 63 < A.n_total - 1) ? gw0 + 63 : A.n_total - 1;
+    const int64_t c_lo = upper_bound_i64(A.soff, 0, A.n_curve + 1, gw0) - 1;
+    const int64_t c_hi = upper_bound_i64(A.soff, c_lo, A.n_curve + 1, gw1);
+    if (g >= A.n_total) return;
+    const int c = (int)(upper_bound_i64(A.soff, c_lo, c_hi, g) - 1);
+    const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
+    int32_t seg = 0;
+    double u = 0.0;
+#define GCV_SAMPLE_CASE(PP, DD)                                                                     \
+    case 10 * PP + DD:                                                                             \
+        u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv<double, PP, DD>::get(T), A, c, g, \
+                                &seg);                                                             \
+        break;
+    switch (cls) {
+        GCV_SAMPLE_CASE(1, 2)
+        GCV_SAMPLE_CASE(1, 3)
+        GCV_SAMPLE_CASE(2, 2)
+        GCV_SAMPLE_CASE(2, 3)
+        GCV_SAMPLE_CASE(3, 2)
+        GCV_SAMPLE_CASE(3, 3)
+        GCV_SAMPLE_CASE(4, 2)
+        GCV_SAMPLE_CASE(4, 3)
+        GCV_SAMPLE_CASE(5, 2)
+        GCV_SAMPLE_CASE(5, 3)
+        default:
+            break;
+    }
+#undef GCV_SAMPLE_CASE
+    A.out_u[g] = u;
+    if (A.out_seg) A.out_seg[g] = seg;
+}
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
+                                           const SampleArgs& A) {
+    if (A.n_total <= 0) return hipSuccess;
+    const int64_t blocks = (A.n_total + 255) / 256;
+    prof_begin(c.owner, "k_sample_bspline");
+    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL>), dim3((unsigned)blocks), dim3(256), 0, c.stream, T,
+                       uniform_class, A);
+    hipError_t e = hipGetLastError();
+    prof_end(c.owner);
+    return e;
+}
+
+}  // namespace rt
+}  // namespace gcurval

@@ -1,0 +1,167 @@
+// Built-in curve families, evaluated in interval-local parameter x in [0, 1]:
+//   * ellipse  C(t) = (a cos t, b sin t), t = 2 pi x, one knot interval per
+//     curve (samples/sample1_ellipse/ellipse.h:5-97);
+//   * clamped B-splines of degree 2..5 in 2-D or 3-D
+//     (samples/sample2_bspline/BSpline.h:26-96, structs.h:6-185), whose
+//     derivative on each knot span is converted once per interval into a
+//     power-basis polynomial of degree p-1 in x, so a quadrature node costs a
+//     Horner evaluation, a norm and a square root, with no divisions.
+// Speeds are |dC/dx| = h |C'(u)| for u = u0 + h x, so integrals over x are arc
+// lengths in the curve's (optionally normalised) units.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#ifndef GCURVAL_HD
+#if defined(__HIPCC__)
+#define GCURVAL_HD __host__ __device__
+#else
+#define GCURVAL_HD
+#endif
+#endif
+
+namespace gcurval {
+namespace dev {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Coefficient block stride per interval (p*D <= 15 used).
+constexpr int kCoefStride = 16;
+
+// ---------------------------------------------------------------------------
+// B-spline span -> power-basis derivative polynomial (double precision).
+//
+// For span s (knot T[s] <= u < T[s+1], h = T[s+1]-T[s]) the derivative curve is
+// a degree q = p-1 B-spline with control points
+//   Q_m = p (P_{m+1} - P_m) / (T[m+p+1] - T[m+1]),   m = s-p .. s-1,
+// evaluated by de Boor's algorithm on the knot vector without its first and
+// last knot. Running de Boor's recurrence on polynomials in x (alpha is linear
+// in x) yields dC/du on the span as sum_j d_j x^j; multiplying by h gives
+// dC/dx. Output layout: out[d*p + j] = coefficient of x^j of component d.
+// Returns 0 on success, 1 if the span is degenerate (h <= 0; coefficients 0).
+// ---------------------------------------------------------------------------
+GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, const double* P, int pstride,
+                                        double scale_inv, double* out, double* u0_out, double* h_out) {
+    const int q = p - 1;
+    const double ts = T[s], h = T[s + 1] - T[s];
+    if (u0_out) *u0_out = ts;
+    if (h_out) *h_out = h;
+    for (int i = 0; i < p * D; i++) out[i] = 0.0;
+    if (!(h > 0.0)) return 1;
+    // d[j][deg][comp]
+    double d[5][5][3];
+    for (int j = 0; j <= q; j++) {
+        const int m = s - p + j;
+        const double den = T[m + p + 1] - T[m + 1];
+        for (int c = 0; c < D; c++) {
+            for (int e = 0; e <= q; e++) d[j][e][c] = 0.0;
+            d[j][0][c] = den > 0.0 ? p * (P[(m + 1) * pstride + c] - P[m * pstride + c]) / den : 0.0;
+        }
+    }
+    for (int r = 1; r <= q; r++) {
+        for (int j = q; j >= r; j--) {
+            const int i = s - q + j;
+            const double den = T[i + q - r + 1] - T[i];
+            const double a0 = den > 0.0 ? (ts - T[i]) / den : 0.0;
+            const double a1 = den > 0.0 ? h / den : 0.0;
+            // d_j <- (1 - a0 - a1 x) d_{j-1} + (a0 + a1 x) d_j
+            for (int c = 0; c < D; c++) {
+                double nw[5];
+                for (int e = 0; e <= q; e++) nw[e] = (1.0 - a0) * d[j - 1][e][c] + a0 * d[j][e][c];
+                for (int e = 0; e < q; e++) nw[e + 1] += a1 * (d[j][e][c] - d[j - 1][e][c]);
+                for (int e = 0; e <= q; e++) d[j][e][c] = nw[e];
+            }
+        }
+    }
+    const double f = h * scale_inv;
+    for (int c = 0; c < D; c++)
+        for (int e = 0; e <= q; e++) out[c * p + e] = d[q][e][c] * f;
+    return 0;
+}
+
+#if defined(__HIPCC__)
+
+// ---------------------------------------------------------------------------
+// Ellipse evaluator. Ctx holds 2 pi a' and 2 pi b' with a', b' the optionally
+// normalised semi-axes, so speed(x) = sqrt((A sin 2pi x)^2 + (B cos 2pi x)^2).
+// ---------------------------------------------------------------------------
+template <typename Dt>
+struct EllipseEval {
+    const double* a;
+    const double* b;
+    int normalize;
+
+    struct Ctx {
+        Dt A, B;
+    };
+
+    __device__ __forceinline__ double scale(int i) const {
+        if (!normalize) return 1.0;
+        double s = fmax(fabs(a[i]), fabs(b[i]));
+        return s > 0.0 ? s : 1.0;
+    }
+    __device__ __forceinline__ Ctx load(int i) const {
+        double s = 1.0 / scale(i);
+        return Ctx{Dt(kTwoPi * a[i] * s), Dt(kTwoPi * b[i] * s)};
+    }
+    __device__ __forceinline__ float speed(const Ctx& c, float x) const {
+        float sn, cs;
+        sincospif(2.0f * x, &sn, &cs);
+        float u = c.A * sn, v = c.B * cs;
+        return __builtin_amdgcn_sqrtf(u * u + v * v);
+    }
+    __device__ __forceinline__ double speed(const Ctx& c, double x) const {
+        double sn, cs;
+        sincospi(2.0 * x, &sn, &cs);
+        double u = c.A * sn, v = c.B * cs;
+        return sqrt(u * u + v * v);
+    }
+    // interval-local x -> curve parameter t
+    __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }
+};
+
+// ---------------------------------------------------------------------------
+// B-spline evaluator for one (degree P, dimension D) class; coefficient
+// blocks are precomputed per interval (kCoefStride values each).
+// ---------------------------------------------------------------------------
+template <typename Dt, int P, int D>
+struct BSplineEval {
+    const Dt* coef;           // [n_int * kCoefStride]
+    const double* int_scale;  // per interval normalisation scale (may be null -> 1)
+    const double* u0;         // per interval knot T[s]
+    const double* h;          // per interval span width
+
+    struct Ctx {
+        Dt c[P * D];
+    };
+
+    __device__ __forceinline__ double scale(int i) const { return int_scale ? int_scale[i] : 1.0; }
+    __device__ __forceinline__ Ctx load(int i) const {
+        Ctx k;
+        const Dt* src = coef + (int64_t)i * kCoefStride;
+#pragma unroll
+        for (int j = 0; j < P * D; j++) k.c[j] = src[j];
+        return k;
+    }
+    template <typename T>
+    __device__ __forceinline__ T speed(const Ctx& k, T x) const {
+        T s = T(0);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T v = T(k.c[d * P + P - 1]);
+#pragma unroll
+            for (int j = P - 2; j >= 0; j--) v = v * x + T(k.c[d * P + j]);
+            s += v * v;
+        }
+        return sq(s);
+    }
+    __device__ __forceinline__ static float sq(float v) { return __builtin_amdgcn_sqrtf(v); }
+    __device__ __forceinline__ static double sq(double v) { return sqrt(v); }
+    __device__ __forceinline__ double to_u(int i, double x) const { return u0[i] + h[i] * x; }
+};
+
+#endif  // __HIPCC__
+
+}  // namespace dev
+}  // namespace gcurval

@@ -1,0 +1,107 @@
+// Device-side building blocks shared by every gcurval kernel on gfx950:
+// Gauss-Legendre rules, wave64 primitives, and the task-key encoding of the
+// wave-local refinement stack.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gcurval {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre rules of order 1..5 on [-1, 1] in the node order of
+// gcurval/base/constants.inl.h:74-92 (textbook values). The order of the
+// node sum is kept so float rounding follows the reference's evaluation.
+// ---------------------------------------------------------------------------
+template <int N> struct GL;
+template <> struct GL<1> {
+    static constexpr double x[1] = {0.0};
+    static constexpr double w[1] = {2.0};
+};
+template <> struct GL<2> {
+    static constexpr double x[2] = {-0.5773502691896258, 0.5773502691896258};
+    static constexpr double w[2] = {1.0, 1.0};
+};
+template <> struct GL<3> {
+    static constexpr double x[3] = {0.0, -0.7745966692414834, 0.7745966692414834};
+    static constexpr double w[3] = {0.8888888888888889, 0.5555555555555556, 0.5555555555555556};
+};
+template <> struct GL<4> {
+    static constexpr double x[4] = {-0.33998104358485626, 0.33998104358485626, -0.8611363115940526,
+                                    0.8611363115940526};
+    static constexpr double w[4] = {0.6521451548625462, 0.6521451548625462, 0.34785484513745385,
+                                    0.34785484513745385};
+};
+template <> struct GL<5> {
+    static constexpr double x[5] = {0.0, -0.5384693101056831, 0.5384693101056831, -0.906179845938664,
+                                    0.906179845938664};
+    static constexpr double w[5] = {0.5688888888888889, 0.47862867049936647, 0.47862867049936647,
+                                    0.23692688505618908, 0.23692688505618908};
+};
+
+// n-point quadrature of the evaluator's speed over [x0, x1] (interval-local
+// parameter), in the arithmetic of Dt.
+template <int N, class Ev, typename Dt>
+__device__ __forceinline__ Dt quad(const Ev& ev, const typename Ev::Ctx& c, Dt x0, Dt x1) {
+    const Dt k = (x1 - x0) * Dt(0.5);
+    const Dt m = (x1 + x0) * Dt(0.5);
+    Dt acc = Dt(0);
+#pragma unroll
+    for (int j = 0; j < N; j++) acc += Dt(GL<N>::w[j]) * ev.speed(c, k * Dt(GL<N>::x[j]) + m);
+    return acc * k;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 primitives
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+
+__device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+
+// Number of set bits of m in lanes below this lane.
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ double shfl_up_d(double v, int d) { return __shfl_up(v, (unsigned)d, 64); }
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+
+// Inclusive scan over aligned segments of 2^log2seg lanes, in double.
+__device__ __forceinline__ double seg_scan_incl(double v, int log2seg) {
+    const int k = lane_id() & ((1 << log2seg) - 1);
+    for (int d = 1; d < (1 << log2seg); d <<= 1) {
+        double t = shfl_up_d(v, d);
+        if (k >= d) v += t;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Refinement-task key: root lane (6 bits) | level (6 bits) | index (32 bits).
+// A task at `level` with index `idx` below root k spans, in root-local units,
+// [idx 2^-level, (idx+1) 2^-level]; in interval-local x it is
+// [(k 2^level + idx) 2^-(level+log2nsub), ...], exact in float for
+// log2nsub + level <= 24 and in double for <= 53.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t task_key(int root, int level, uint32_t idx) {
+    return ((uint64_t)root << 38) | ((uint64_t)level << 32) | (uint64_t)idx;
+}
+__device__ __forceinline__ int key_root(uint64_t k) { return (int)(k >> 38) & 63; }
+__device__ __forceinline__ int key_level(uint64_t k) { return (int)(k >> 32) & 63; }
+__device__ __forceinline__ uint32_t key_idx(uint64_t k) { return (uint32_t)k; }
+
+// 2^-e, exact (v_ldexp).
+__device__ __forceinline__ float pow2m(float, int e) { return ldexpf(1.0f, -e); }
+__device__ __forceinline__ double pow2m(double, int e) { return ldexp(1.0, -e); }
+
+// num * 2^-shift, exact while num fits the mantissa.
+template <typename Dt>
+__device__ __forceinline__ Dt dyadic(uint64_t num, int shift) {
+    return Dt(num) * pow2m(Dt(0), shift);
+}
+
+}  // namespace dev
+}  // namespace gcurval

@@ -1,0 +1,478 @@
+// gfx950 kernels of the arc-length engine (templates over the curve
+// evaluator, the arithmetic type Dt and the Gauss-Legendre order).
+//
+//   k_subdivide   adaptive subdivision of every knot interval: one wave64 tile
+//                 holds 64 root sub-intervals (n_sub per interval), tests
+//                 |L + R - P| > eps(level) for each, refines failing roots
+//                 cooperatively through a wave-local task stack with ballot /
+//                 mbcnt compaction, and writes each root's inclusive in-interval
+//                 prefix (FP64) plus the interval length.
+//                 Replaces kn_curveLengthTableInit, kn_curveArclengthForward,
+//                 kn_curveArclengthBackward_shfl and kn_mergeSubdividedIntoSeg
+//                 (kernel/generic.inl.cuh:10-115, kernel/bfs.inl.cuh:9-125).
+//   k_curve_scan  FP64 cumulative interval lengths per curve plus the curve
+//                 totals. Replaces kn_mergeSeg and kn_getAllArclength
+//                 (kernel/generic.inl.cuh:117-149).
+//   k_sample      inverse arc-length map per sample: bracket the interval
+//                 (FP64 cumulative) and root (FP64 prefix), re-walk the
+//                 adaptive tree for refined roots, then Newton on
+//                 int_{x0}^{x} |C'| = tau in Dt with a final FP64 step.
+//                 Replaces kn_initSearchTask and kn_arclengthReverseDfsFast_Paper
+//                 (kernel/reverse.inl.cuh:32-99, 216-341).
+#pragma once
+
+#include "device.h"
+#include "curves.h"
+
+namespace gcurval {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Subdivision
+// ---------------------------------------------------------------------------
+enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverflow = 3, kStatTasks = 4,
+                kStatCount = 8 };
+
+template <typename Dt>
+struct SubdivArgs {
+    int64_t n_slots;          // interval slots (n_int, or int_list length)
+    const int32_t* int_list;  // optional slot -> interval map
+    int log2_nsub;            // roots per interval = 2^log2_nsub (<= 64)
+    Dt eps0;                  // level-0 tolerance per root (tol / n_sub)
+    int max_depth;
+    double* root_prefix;      // [n_int << log2_nsub] inclusive prefix in interval (normalised units)
+    double* int_len;          // [n_int] interval length (curve units)
+    double* curve_len;        // optional: one interval per curve -> also curve length
+    uint64_t* deep;           // [n_int] bit k set if root k was refined
+    int stack_cap;            // task-stack entries per wave (LDS)
+    unsigned long long* stats;
+};
+
+// Task-stack key: float builds (depth <= 20) fit root | level | index in 32
+// bits; double builds use 64 bits.
+template <typename Dt> struct KeyOf { typedef uint64_t type; };
+template <> struct KeyOf<float> { typedef uint32_t type; };
+
+template <typename K> __device__ __forceinline__ K make_key(int root, int level, uint32_t idx);
+template <> __device__ __forceinline__ uint32_t make_key<uint32_t>(int root, int level, uint32_t idx) {
+    return ((uint32_t)root << 26) | ((uint32_t)level << 21) | idx;
+}
+template <> __device__ __forceinline__ uint64_t make_key<uint64_t>(int root, int level, uint32_t idx) {
+    return task_key(root, level, idx);
+}
+__device__ __forceinline__ int key_root_of(uint32_t k) { return (int)(k >> 26) & 63; }
+__device__ __forceinline__ int key_level_of(uint32_t k) { return (int)(k >> 21) & 31; }
+__device__ __forceinline__ uint32_t key_idx_of(uint32_t k) { return k & 0x1FFFFFu; }
+__device__ __forceinline__ int key_root_of(uint64_t k) { return key_root(k); }
+__device__ __forceinline__ int key_level_of(uint64_t k) { return key_level(k); }
+__device__ __forceinline__ uint32_t key_idx_of(uint64_t k) { return key_idx(k); }
+
+// Bytes of dynamic LDS per wave for a stack of `cap` entries.
+template <typename Dt>
+__host__ __device__ constexpr int subdiv_lds_per_wave(int cap) {
+    return 64 * (int)sizeof(double) + cap * (int)(sizeof(typename KeyOf<Dt>::type) + sizeof(Dt));
+}
+
+// Bounds of task (root kr, level l, index idx) in interval-local x; exact.
+template <typename Dt>
+__device__ __forceinline__ void task_bounds(int kr, int l, uint32_t idx, int ls, Dt& x0, Dt& xm, Dt& x1) {
+    const uint64_t base = ((uint64_t)kr << l) + idx;
+    x0 = dyadic<Dt>(base, ls + l);
+    x1 = dyadic<Dt>(base + 1, ls + l);
+    xm = dyadic<Dt>(2 * base + 1, ls + l + 1);
+}
+
+template <typename Dt>
+__device__ __forceinline__ Dt dabs(Dt v) { return v < Dt(0) ? -v : v; }
+
+// The wave-local stack holds at most 64 (max_depth + 1) tasks: each pop takes
+// the top 64 entries and pushes their children on top, so the blocks left
+// below the top hold <= 64 entries each and have strictly increasing minimum
+// levels (1 .. max_depth), and the top block holds <= 128.
+template <typename Dt, int NGL, class Ev>
+__global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
+    typedef typename KeyOf<Dt>::type Key;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = lane_id();
+    const int wib = (int)(threadIdx.x >> 6);
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    unsigned char* wbase = smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap);
+    double* s_acc = (double*)wbase;
+    Key* tkey = (Key*)(wbase + 64 * sizeof(double));
+    Dt* tpar = (Dt*)(wbase + 64 * sizeof(double) + A.stack_cap * sizeof(Key));
+    const int ls = A.log2_nsub;
+    const int nsub = 1 << ls;
+    const int ipw_log2 = 6 - ls;  // interval slots per wave tile (log2)
+    const int jslot = lane >> ls;
+    const int k = lane & (nsub - 1);
+    const int64_t n_tiles = (A.n_slots + (1 << ipw_log2) - 1) >> ipw_log2;
+
+    unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0;
+    int max_stack = 0, overflow = 0;
+
+    for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
+        const int64_t slot = (tile << ipw_log2) + jslot;
+        const bool valid = slot < A.n_slots;
+        const int i = valid ? (A.int_list ? A.int_list[slot] : (int)slot) : (A.int_list ? A.int_list[0] : 0);
+        const typename Ev::Ctx ctx = ev.load(i);
+
+        Dt x0, xm, x1;
+        task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
+        const Dt P = quad<NGL>(ev, ctx, x0, x1);
+        const Dt L = quad<NGL>(ev, ctx, x0, xm);
+        const Dt R = quad<NGL>(ev, ctx, xm, x1);
+        Dt len = L + R;
+        const bool fail = valid && A.max_depth > 0 && dabs(len - P) > A.eps0;
+        n_quad += 3ull * popc64(ballot(valid));
+        const uint64_t fmask = ballot(fail);
+
+        if (fmask) {
+            n_deep += popc64(fmask);
+            s_acc[lane] = 0.0;
+            const int pos = 2 * mbcnt(fmask);
+            if (fail) {
+                tkey[pos] = make_key<Key>(lane, 1, 0u);
+                tpar[pos] = L;
+                tkey[pos + 1] = make_key<Key>(lane, 1, 1u);
+                tpar[pos + 1] = R;
+            }
+            int size = 2 * popc64(fmask);
+            if (size > max_stack) max_stack = size;
+            while (size > 0) {
+                const int n = size < 64 ? size : 64;
+                const bool take = lane < n;
+                Key kk = 0;
+                Dt par = Dt(0);
+                if (take) {
+                    kk = tkey[size - 1 - lane];
+                    par = tpar[size - 1 - lane];
+                }
+                size -= n;
+                const int r = key_root_of(kk);
+                const int l = key_level_of(kk);
+                const uint32_t idx = key_idx_of(kk);
+                const int ir = __shfl(i, r, 64);
+                const typename Ev::Ctx cr = (ipw_log2 == 0) ? ctx : ev.load(ir);
+                Dt a0, am, a1;
+                task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
+                const Dt L2 = quad<NGL>(ev, cr, a0, am);
+                const Dt R2 = quad<NGL>(ev, cr, am, a1);
+                const Dt lr = L2 + R2;
+                bool f2 = take && l < A.max_depth && dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
+                const uint64_t tmask = ballot(take);
+                n_quad += 2ull * popc64(tmask);
+                n_tasks += popc64(tmask);
+                uint64_t m2 = ballot(f2);
+                if (size + 2 * popc64(m2) > A.stack_cap) {
+                    // Unreachable while stack_cap >= 64 (max_depth + 1); a bad
+                    // capacity ends as flagged leaves, never as an out-of-bounds write.
+                    overflow = 1;
+                    f2 = false;
+                    m2 = 0;
+                }
+                if (f2) {
+                    const int p2 = size + 2 * mbcnt(m2);
+                    tkey[p2] = make_key<Key>(r, l + 1, 2u * idx);
+                    tpar[p2] = L2;
+                    tkey[p2 + 1] = make_key<Key>(r, l + 1, 2u * idx + 1u);
+                    tpar[p2 + 1] = R2;
+                } else if (take) {
+                    atomicAdd(&s_acc[r], (double)lr);
+                }
+                size += 2 * popc64(m2);
+                if (size > max_stack) max_stack = size;
+            }
+            if (fail) len = (Dt)s_acc[lane];
+        }
+
+        // in-interval inclusive prefix over the n_sub roots (FP64)
+        const double pre = seg_scan_incl(valid ? (double)len : 0.0, ls);
+        if (valid) {
+            A.root_prefix[((int64_t)i << ls) + k] = pre;
+            if (k == nsub - 1) {
+                const double L_int = pre * ev.scale(i);
+                A.int_len[i] = L_int;
+                if (A.curve_len) A.curve_len[i] = L_int;
+                const uint64_t bits = (ls == 6) ? fmask : ((fmask >> (jslot << ls)) & ((1ull << nsub) - 1ull));
+                A.deep[i] = bits;
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&A.stats[kStatQuad], n_quad);
+        atomicAdd(&A.stats[kStatDeepRoots], n_deep);
+        atomicAdd(&A.stats[kStatTasks], n_tasks);
+        atomicMax(&A.stats[kStatMaxStack], (unsigned long long)max_stack);
+        if (overflow) atomicMax(&A.stats[kStatOverflow], 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-curve FP64 cumulative scan (wave per curve), in place on int_cum.
+// ---------------------------------------------------------------------------
+static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const int64_t* ioff, double* int_cum,
+                                                    double* curve_len) {
+    const int lane = lane_id();
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t c = gwave; c < n_curve; c += nwaves) {
+        const int64_t b = ioff[c], e = ioff[c + 1];
+        double carry = 0.0;
+        for (int64_t base = b; base < e; base += 64) {
+            const int64_t j = base + lane;
+            double v = j < e ? int_cum[j] : 0.0;
+            v = seg_scan_incl(v, 6) + carry;
+            if (j < e) int_cum[j] = v;
+            carry = shfl_d(v, 63);
+        }
+        if (lane == 0) curve_len[c] = carry;
+    }
+}
+
+static __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// B-spline setup: per-curve scale and interval -> curve map (wave per curve),
+// then per-interval derivative polynomials (thread per interval).
+// ---------------------------------------------------------------------------
+struct BSplineDev {
+    int n_curve;
+    int ctrl_stride;
+    const int32_t* degree;
+    const int32_t* dim;
+    const int64_t* knot_off;
+    const int64_t* ctrl_off;
+    const double* knots;
+    const double* ctrl;
+};
+
+static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B, const int64_t* ioff, int normalize,
+                                                             double* curve_scale, double* int_scale,
+                                                             int32_t* int_curve) {
+    const int lane = lane_id();
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t c = gwave; c < B.n_curve; c += nwaves) {
+        double sc = 1.0;
+        if (normalize) {
+            const int D = B.dim[c];
+            const int64_t p0 = B.ctrl_off[c], p1 = B.ctrl_off[c + 1];
+            double cen[3] = {0, 0, 0};
+            for (int64_t j = p0 + lane; j < p1; j += 64)
+                for (int d = 0; d < D; d++) cen[d] += B.ctrl[j * B.ctrl_stride + d];
+            for (int d = 0; d < D; d++) {
+                double v = cen[d];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                cen[d] = v / (double)(p1 - p0);
+            }
+            double m = 0.0;
+            for (int64_t j = p0 + lane; j < p1; j += 64)
+                for (int d = 0; d < D; d++) m = fmax(m, fabs(B.ctrl[j * B.ctrl_stride + d] - cen[d]));
+            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+            sc = m > 0.0 ? m : 1.0;
+        }
+        if (lane == 0) curve_scale[c] = sc;
+        for (int64_t i = ioff[c] + lane; i < ioff[c + 1]; i += 64) {
+            int_scale[i] = sc;
+            int_curve[i] = (int32_t)c;
+        }
+    }
+}
+
+static __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int64_t* ioff, int64_t n_int,
+                                                      const int32_t* int_curve, const double* int_scale,
+                                                      double* coef64, float* coef32, double* u0, double* h) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_int) return;
+    const int c = int_curve[i];
+    const int p = B.degree[c], D = B.dim[c];
+    const int segid = (int)(i - ioff[c]);
+    double out[kCoefStride];
+    for (int j = 0; j < kCoefStride; j++) out[j] = 0.0;
+    bspline_span_poly(p, D, segid + p, B.knots + B.knot_off[c], B.ctrl + B.ctrl_off[c] * B.ctrl_stride,
+                      B.ctrl_stride, 1.0 / int_scale[i], out, &u0[i], &h[i]);
+    for (int j = 0; j < kCoefStride; j++) {
+        coef64[i * kCoefStride + j] = out[j];
+        if (coef32) coef32[i * kCoefStride + j] = (float)out[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sampling
+// ---------------------------------------------------------------------------
+struct SampleArgs {
+    int n_curve;
+    int64_t n_total;
+    const int64_t* soff;        // [n_curve+1] sample offsets
+    const double* pos;          // optional positions; null -> uniform
+    int uniform_closed;         // uniform: 1 -> j/(N-1), 0 -> j/N
+    int is_relative;
+    const int64_t* ioff;        // [n_curve+1] interval offsets
+    const double* int_cum;      // [n_int] cumulative interval lengths (curve units)
+    const double* curve_len;    // [n_curve]
+    const double* root_prefix;  // [n_int << log2_nsub]
+    const uint64_t* deep;       // [n_int]
+    int log2_nsub;
+    double eps0;                // level-0 tolerance per root (same value as the build)
+    int max_depth;
+    int newton_iters;
+    double* out_u;              // [n_total]
+    int32_t* out_seg;           // optional local interval index per sample
+    const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
+};
+
+// first index in [lo, hi) with a[idx] > v (hi if none)
+__device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t lo, int64_t hi, int64_t v) {
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        if (a[m] > v) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+// first index in [lo, hi) with a[idx] >= v (hi if none)
+__device__ __forceinline__ int64_t lower_bound_f64(const double* a, int64_t lo, int64_t hi, double v) {
+    int64_t cnt = hi - lo;
+    while (cnt > 0) {
+        const int64_t h = cnt >> 1;
+        if (a[lo + h] < v) { lo += h + 1; cnt -= h + 1; } else cnt = h;
+    }
+    return lo;
+}
+
+// Safeguarded Newton on F(x) = int_{a}^{x} speed - tau over [a, b].
+template <int NGL, class Ev, typename Dt>
+__device__ __forceinline__ Dt newton_in(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
+                                        int iters) {
+    Dt lo = a, hi = b;
+    Dt f = len > Dt(0) ? tau / len : Dt(0);
+    f = f < Dt(0) ? Dt(0) : (f > Dt(1) ? Dt(1) : f);
+    Dt x = a + (b - a) * f;
+    for (int it = 0; it < iters; it++) {
+        const Dt F = quad<NGL>(ev, c, a, x) - tau;
+        if (F > Dt(0)) hi = x; else lo = x;
+        const Dt d = ev.speed(c, x);
+        Dt xn = d > Dt(0) ? x - F / d : Dt(0.5) * (lo + hi);
+        if (!(xn >= lo && xn <= hi)) xn = Dt(0.5) * (lo + hi);
+        x = xn;
+    }
+    return x;
+}
+
+// Re-walk the adaptive tree of a refined root in the build's arithmetic and
+// return the build leaf [a, b] holding tau (root-local target), with tau
+// reduced to a leaf-local target and the leaf's length.
+template <typename Dt, int NGL, class Ev>
+__device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int ls, Dt eps0, int max_depth,
+                             double tau, Dt& a, Dt& b, double& tau_leaf, Dt& leaf_len) {
+    Dt stackR[64];
+    int l = 0;
+    uint32_t idx = 0;
+    Dt x0, xm, x1;
+    task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
+    Dt P = quad<NGL>(ev, c, x0, x1);
+    double acc = 0.0;
+    for (int guard = 0; guard < (1 << 26); guard++) {
+        task_bounds<Dt>(k, l, idx, ls, x0, xm, x1);
+        const Dt L = quad<NGL>(ev, c, x0, xm);
+        const Dt R = quad<NGL>(ev, c, xm, x1);
+        const Dt lr = L + R;
+        if (l < max_depth && dabs(lr - P) > eps0 * pow2m(Dt(0), l)) {
+            stackR[l + 1] = R;
+            l++;
+            idx = 2u * idx;
+            P = L;
+            continue;
+        }
+        // build leaf
+        const double v = (double)lr;
+        if (tau <= acc + v) {
+            a = x0; b = x1; tau_leaf = tau - acc; leaf_len = lr;
+            return;
+        }
+        acc += v;
+        while (l > 0 && (idx & 1u)) { idx >>= 1; l--; }
+        if (l == 0) {  // target beyond the root's last leaf (rounding): clamp to its end
+            a = x0; b = x1; tau_leaf = v; leaf_len = lr;
+            return;
+        }
+        idx += 1u;
+        P = stackR[l];
+    }
+    a = x0; b = x1; tau_leaf = 0.0; leaf_len = Dt(0);
+}
+
+template <typename Dt, int NGL, class Ev, class Ev64>
+__device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
+                                             int64_t g, int32_t* seg_out) {
+    double pos;
+    const int64_t s0 = A.soff[c];
+    if (A.pos) {
+        pos = A.pos[g];
+    } else {
+        const int64_t N = A.soff[c + 1] - s0, j = g - s0;
+        pos = A.uniform_closed ? (N > 1 ? (double)j * (1.0 / (double)(N - 1)) : 0.0) : (double)j * (1.0 / (double)N);
+    }
+    const double s = A.is_relative ? pos * A.curve_len[c] : pos;
+    const int64_t ib = A.ioff[c], ie = A.ioff[c + 1];
+    int64_t i = lower_bound_f64(A.int_cum, ib, ie, s);
+    if (i >= ie) i = ie - 1;
+    if (seg_out) *seg_out = (int32_t)(i - ib);
+    const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
+    const double sn = sl / ev.scale((int)i);
+    const int ls = A.log2_nsub, nsub = 1 << ls;
+    const int64_t ro = (int64_t)i << ls;
+    int64_t kk = lower_bound_f64(A.root_prefix, ro, ro + nsub, sn) - ro;
+    if (kk >= nsub) kk = nsub - 1;
+    const int k = (int)kk;
+    const double pre = k > 0 ? A.root_prefix[ro + k - 1] : 0.0;
+    const double rlen = A.root_prefix[ro + k] - pre;
+    double tau = sn - pre;
+    tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
+
+    const typename Ev::Ctx ctx = ev.load((int)i);
+    Dt a, b, xm;
+    task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
+    Dt len = (Dt)rlen;
+    if ((A.deep[i] >> k) & 1ull) {
+        double tl;
+        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, (Dt)A.eps0, A.max_depth, tau, a, b, tl, len);
+        tau = tl;
+    }
+    Dt x = newton_in<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters);
+    double xd = (double)x;
+    if (sizeof(Dt) < sizeof(double)) {
+        // one FP64 Newton step on the same leaf
+        const typename Ev64::Ctx c64 = ev64.load((int)i);
+        const double ad = (double)a, bd = (double)b;
+        const double F = quad<NGL>(ev64, c64, ad, xd) - tau;
+        const double d = ev64.speed(c64, xd);
+        if (d > 0.0) {
+            const double xn = xd - F / d;
+            xd = xn < ad ? ad : (xn > bd ? bd : xn);
+        }
+    }
+    return ev.to_u((int)i, xd);
+}
+
+template <typename Dt, int NGL, class Ev, class Ev64>
+__global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // curve range of this wave's samples, then per-lane search inside it
+    const int64_t gw0 = g - lane_id();
+    const int64_t gw1 = (gw0 + 63 < A.n_total - 1) ? gw0 + 63 : A.n_total - 1;
+    const int64_t c_lo = upper_bound_i64(A.soff, 0, A.n_curve + 1, gw0) - 1;
+    const int64_t c_hi = upper_bound_i64(A.soff, c_lo, A.n_curve + 1, gw1);
+    if (g >= A.n_total) return;
+    const int c = (int)(upper_bound_i64(A.soff, c_lo, c_hi, g) - 1);
+    int32_t seg;
+    const double u = sample_one<Dt, NGL>(ev, ev64, A, c, g, &seg);
+    A.out_u[g] = u;
+    if (A.out_seg) A.out_seg[g] = seg;
+}
+
+}  // namespace dev
+}  // namespace gcurval

@@ -1,0 +1,160 @@
+/*
+ * gcurval_c.h — C ABI of the MI355X (gfx950) arc-length engine.
+ *
+ * Plain pointers and sizes only. Every call returns a gcv_status (0 = OK);
+ * gcv_last_error() gives the message of the last failure on a context. One
+ * context per host thread; all work is stream-ordered on the context's HIP
+ * stream, and results handed to the host synchronise that stream.
+ *
+ * Each entry point names the reference interface it replaces. Paths are
+ * relative to the HolyChen/gcurval tree.
+ */
+#ifndef GCURVAL_C_H
+#define GCURVAL_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gcv_status {
+    GCV_OK = 0,
+    GCV_ERR_INVALID = 1,     /* bad argument or batch layout */
+    GCV_ERR_OOM = 2,         /* arena exhausted */
+    GCV_ERR_HIP = 3,         /* HIP runtime error */
+    GCV_ERR_UNSUPPORTED = 4, /* option combination not built */
+    GCV_ERR_STATE = 5        /* call order (e.g. sampling before a build) */
+} gcv_status;
+
+/* Mirrors GcurvalEnum (gcurval/base/util.h:71-77). */
+typedef enum gcv_where { GCV_DEVICE = 0, GCV_HOST = 1 } gcv_where;
+typedef enum gcv_what { GCV_RETURN_US = 2, GCV_RETURN_POINTS = 3 } gcv_what;
+
+typedef enum gcv_precision { GCV_FP32 = 0, GCV_FP64 = 1 } gcv_precision;
+
+typedef struct gcv_ctx gcv_ctx;
+typedef struct gcv_table gcv_table;
+
+/*
+ * Table-build options. Defaults (gcv_opts_default) follow gcurval/config.h:17-43
+ * and gcurval/base/functors.inl.h:42: FP32, 3-point Gauss-Legendre, 64 root
+ * sub-intervals per knot interval, depth 14 (FP32) / 23 (FP64), tolerance
+ * 1e-6 (FP32) / 1e-14 (FP64), absolute tolerance on raw coordinates.
+ */
+typedef struct gcv_opts {
+    int precision;     /* gcv_precision */
+    int gl_order;      /* 3 or 5 */
+    int n_subdivided;  /* roots per knot interval: power of two, 1..64 */
+    int max_depth;     /* 0 = default; <= 18 (FP32), <= 31 (FP64) */
+    double tolerance;  /* 0 = default; per-interval absolute tolerance */
+    int normalize;     /* 1: test each curve at unit max-abs coordinate (lengths rescaled) */
+    int newton_iters;  /* 0 = default (3 FP32 + 1 FP64 step; 4 in FP64 mode) */
+    int reserved[6];
+} gcv_opts;
+
+void gcv_opts_default(gcv_opts* o);
+
+/* ---- context, arena, copies (replaces gMalloc/gFree/gFree_s over RMM or
+ * cudaMalloc, gcurval/detail/cuda/util.h:41-71, and the cc()/cet() checks
+ * that exit(-1), util.h:14-39) ---- */
+int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out);
+int gcv_ctx_destroy(gcv_ctx* ctx);
+const char* gcv_last_error(const gcv_ctx* ctx);
+int gcv_alloc(gcv_ctx* ctx, size_t bytes, void** dptr);
+int gcv_free(gcv_ctx* ctx, void* dptr);
+int gcv_arena_stats(const gcv_ctx* ctx, size_t* used, size_t* capacity);
+int gcv_copy_h2d(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gcv_copy_d2h(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gcv_sync(gcv_ctx* ctx);
+int gcv_device_info(gcv_ctx* ctx, int* n_cu, char* name, int name_len);
+
+/* ---- timing: HIP events on the context stream ---- */
+int gcv_timer_start(gcv_ctx* ctx);
+int gcv_timer_stop(gcv_ctx* ctx, double* ms); /* synchronises */
+/* Per-kernel launch timing (events around every launch while enabled). */
+int gcv_profile_enable(gcv_ctx* ctx, int on);
+int gcv_profile_reset(gcv_ctx* ctx);
+/* Sum of durations and launch count of kernels whose name starts with prefix. */
+int gcv_profile_query(gcv_ctx* ctx, const char* prefix, double* total_ms, int64_t* count);
+
+/* ---- curve batches (device pointers, stay owned by the caller and must
+ * outlive the table) ---- */
+
+/* Ellipses x = a cos t, y = b sin t, t in [0, 2 pi], one knot interval each
+ * (samples/sample1_ellipse/ellipse.h:17-65). a, b: device double[n_curve]. */
+int gcv_table_create_ellipse(gcv_ctx* ctx, int n_curve, const double* d_a, const double* d_b,
+                             const gcv_opts* opts, gcv_table** out);
+
+/* Clamped B-splines of degree 1..5 in 2-D/3-D, mixed degrees allowed
+ * (samples/sample2_bspline/structs.h:6-25, 126-163). Curve c has knots
+ * knots[knot_off[c] .. knot_off[c+1]) and control points
+ * ctrl[(ctrl_off[c] + j) * ctrl_stride + d]; its knot intervals are the spans
+ * [t_{s}, t_{s+1}], s = p .. n_knots-p-2, i.e. n_knots - 2p - 1 intervals.
+ * All arrays are device pointers. */
+typedef struct gcv_bspline_batch {
+    int n_curve;
+    int ctrl_stride;          /* 2 or 3 doubles per control point */
+    const int32_t* degree;    /* [n_curve] */
+    const int32_t* dim;       /* [n_curve], 2 or 3 (<= ctrl_stride) */
+    const int64_t* knot_off;  /* [n_curve+1] */
+    const int64_t* ctrl_off;  /* [n_curve+1] */
+    const double* knots;
+    const double* ctrl;
+} gcv_bspline_batch;
+
+int gcv_table_create_bspline(gcv_ctx* ctx, const gcv_bspline_batch* batch, const gcv_opts* opts,
+                             gcv_table** out);
+
+/* Enqueue the table build: ArclengthCalculateCuda::createArclengthTable
+ * (gcurval/detail/cuda/arclength_calculate.h:185-210). No host sync, no
+ * allocation; may be called repeatedly. */
+int gcv_table_build(gcv_table* t);
+
+int gcv_table_destroy(gcv_table* t);
+int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_intervals, int* n_subdivided);
+
+/* Per-curve lengths to host: getAllCurveArclengths / getArclength
+ * (gcurval/detail/cuda/arclength_calculate.h:212-222, 372-406). Synchronises. */
+int gcv_table_lengths(gcv_table* t, double* host_out);
+
+/* Device views of the table (valid until the next build or destroy):
+ * per-interval FP64 cumulative lengths within each curve, per-root inclusive
+ * FP64 prefix within each interval (normalised units), per-interval refined
+ * root masks, per-curve lengths, interval offsets [n_curve+1]. */
+int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** root_prefix,
+                           const uint64_t** deep_mask, const double** curve_len, const int64_t** int_off);
+
+/* Build statistics of the last build: quadratures evaluated, refined roots,
+ * maximum task-stack depth, stack overflow flag (must be 0), refinement tasks. */
+int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
+                    uint64_t* overflow, uint64_t* tasks);
+
+/* Arc-length sampling: arclengthReverse / kn_initSearchTask +
+ * kn_arclengthReverseDfsFast_Paper (gcurval/detail/cuda/arclength_calculate.h:224-231,
+ * 408-505; kernel/reverse.inl.cuh:32-99, 216-341).
+ *   d_sample_off: device int64[n_curve+1], samples of curve c are
+ *                 [off[c], off[c+1]) (the partial sum of n_ptrs_for_curves).
+ *   d_pos:        device double[n_total] positions, or NULL for uniform
+ *                 positions j/(N_c-1) (uniform_closed=1) or j/N_c (0).
+ *   is_relative:  positions are fractions of the curve length (else arc lengths).
+ *   d_out_u:      device double[n_total] curve parameters u (t for ellipses).
+ *   d_out_seg:    optional device int32[n_total] local knot-interval index.
+ * Enqueued on the context stream; no host sync. */
+int gcv_sample(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_pos,
+               int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg);
+
+/* Host-side helpers that run without a GPU (used by the CPU test suite). */
+/* Power-basis derivative coefficients of one B-spline span, as computed by
+ * the setup kernel (out[d*p + j] multiplies x^j, x in [0,1] across the span). */
+int gcv_host_bspline_span_poly(int p, int D, int s, const double* knots, const double* ctrl, int ctrl_stride,
+                               double scale_inv, double* out, double* u0, double* h);
+/* Library version string. */
+const char* gcv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCURVAL_C_H */

@@ -1,0 +1,262 @@
+"""GPU parity of the gfx950 table build and sampler against the oracle.
+
+Bars (stated per test): arc length <= 1e-6 relative in FP32 mode and <= 1e-10
+in FP64 mode against the long-double truth and against the serial
+restatement of the reference's GPU path; per-interval sample counts equal to
+the truth's except inside the tie band |s - S_i| <= 2 tol L_c; sample
+parameters within the stated absolute bound of the truth. Every case runs
+through the C ABI (gcurval_amd -> libgcurval_amd.so).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gcurval_amd as G
+from gcurval_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def build(ctx, batch, **kw):
+    eng = G.ArclengthCalculateGpu(ctx, batch, **kw)
+    eng.createArclengthTable()
+    return eng
+
+
+def count_mismatches(batch, counts, pos, seg_gpu, seg_truth, L_truth, cum_truth, tol):
+    """Samples whose interval differs from the truth's, outside the tie band."""
+    off = O.offsets(counts)
+    bad = 0
+    ties = 0
+    ipc = np.ones(batch.n_curve, np.int64) if batch.kind == "ellipse" else batch.intervals_per_curve()
+    ioff = np.zeros(batch.n_curve + 1, np.int64)
+    np.cumsum(ipc, out=ioff[1:])
+    for c in range(batch.n_curve):
+        s = pos[off[c]:off[c + 1]] * L_truth[c]
+        bounds = cum_truth[ioff[c]:ioff[c + 1]]
+        near = np.min(np.abs(s[:, None] - bounds[None, :]), axis=1) <= 2 * tol * L_truth[c]
+        diff = seg_gpu[off[c]:off[c + 1]] != seg_truth[off[c]:off[c + 1]]
+        bad += int(np.sum(diff & ~near))
+        ties += int(np.sum(diff & near))
+    return bad, ties
+
+
+# ---------------------------------------------------------------------------
+# C0: one ellipse a=3, b=1, FP64, tolerance 1e-8 (as float), GL-5 and GL-3
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("gl", [5, 3])
+def test_c0_ellipse_fp64(ctx, gl):
+    tol = float(np.float32(1e-8))
+    b = W.ellipse_c0()
+    with build(ctx, b, precision=G.GCV_FP64, gl_order=gl, tolerance=tol) as eng:
+        L = eng.getAllCurveArclengths()[0]
+        assert abs(L - W.ELLIPSE_3_1_LENGTH) / W.ELLIPSE_3_1_LENGTH <= 1e-10
+        ref = O.RefGpuPlain(b, O.opts(1, gl, tol=tol))
+        ref.build()
+        assert abs(L - ref.tables()[0][0]) / L <= 1e-10
+        n = 1000
+        for closed in (False, True):
+            pos = O.uniform_positions([n], closed=closed)
+            u = eng.arclengthReverse(np.array([n]), None, True, uniform_closed=closed)
+            ut, _ = O.Truth(b).reverse([n], pos)
+            assert np.max(np.abs(u - ut)) <= 1e-9
+        assert eng.stats()["overflow"] == 0
+
+
+def test_unit_circle_known_answer(ctx):
+    b = G.EllipseBatch([1.0], [1.0])
+    n = 1000
+    for prec, atol in ((G.GCV_FP64, 1e-12), (G.GCV_FP32, 1e-6)):
+        with build(ctx, b, precision=prec) as eng:
+            assert abs(eng.getAllCurveArclengths()[0] - 2 * math.pi) <= (1e-12 if prec else 1e-6)
+            u = eng.arclengthReverse(np.array([n]), None, True, uniform_closed=False)
+            np.testing.assert_allclose(u, 2 * math.pi * np.arange(n) / n, rtol=0, atol=atol)
+
+
+# ---------------------------------------------------------------------------
+# C1 distribution (small): FP32 quadrature, FP64 scan, normalised curves
+# ---------------------------------------------------------------------------
+def test_ellipse_batch_fp32(ctx):
+    b = W.ellipse_batch(400, seed=21)
+    counts = W.ellipse_sample_counts(400, seed=22, lo=16, hi=256)
+    tr = O.Truth(b)
+    Lt, cumt = tr.lengths()
+    with build(ctx, b, normalize=True) as eng:
+        Lg = eng.getAllCurveArclengths()
+        assert np.max(np.abs(Lg - Lt) / Lt) <= 1e-6
+        ref = O.RefGpuPlain(b, O.opts(0, 3, normalize=True))
+        ref.build()
+        assert np.max(np.abs(Lg - ref.tables()[0]) / Lt) <= 1e-6
+        pos = O.uniform_positions(counts)
+        u = eng.arclengthReverse(counts, None, True)
+        ut, _ = tr.reverse(counts, pos)
+        # t-error bound: arc-length error (1e-6 L) over the slowest speed b
+        off = O.offsets(counts)
+        bound = np.repeat(np.maximum(1e-6, 2e-7 * Lt / b.b), counts)
+        assert np.all(np.abs(u - ut) <= bound)
+        assert eng.stats()["overflow"] == 0
+
+
+def test_reference_sample1_dataset_lengths(ctx):
+    b = W.reference_sample1_ellipses(1000)
+    ref = O.RefGpuPlain(b, O.opts(0, 3))
+    ref.build()
+    with build(ctx, b) as eng:
+        Lg = eng.getAllCurveArclengths()
+        Lt, _ = O.Truth(b).lengths()
+        assert np.max(np.abs(Lg - Lt) / Lt) <= 1e-6
+        assert np.max(np.abs(Lg - ref.tables()[0]) / Lt) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# B-splines
+# ---------------------------------------------------------------------------
+def bspline_case(ctx, batch, prec=G.GCV_FP32, gl=3, per_interval=32, len_bar=1e-6, u_bar=1e-6, tol=1e-6):
+    tr = O.Truth(batch)
+    Lt, cumt = tr.lengths()
+    counts = per_interval * batch.intervals_per_curve()
+    with build(ctx, batch, precision=prec, gl_order=gl, normalize=True) as eng:
+        Lg = eng.getAllCurveArclengths()
+        rel = np.max(np.abs(Lg - Lt) / Lt)
+        assert rel <= len_bar, rel
+        u, seg = eng.arclengthReverse(counts, None, True, return_segments=True)
+        pos = O.uniform_positions(counts)
+        ut, segt = tr.reverse(counts, pos)
+        bad, ties = count_mismatches(batch, counts, pos, seg, segt, Lt, cumt, tol)
+        assert bad == 0, (bad, ties)
+        err = np.abs(u - ut)
+        assert np.max(err) <= u_bar, np.max(err)
+        assert eng.stats()["overflow"] == 0
+        return rel, float(np.max(err)), ties
+
+
+def test_bspline_cubic3d_fp32(ctx):
+    b = W.bspline_batch(60, seed=31, degrees=(3,), dims=(3,), n_ctrl_range=(8, 200))
+    bspline_case(ctx, b)
+
+
+def test_bspline_mixed_degrees_fp32(ctx):
+    b = W.bspline_batch(60, seed=32, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(16, 120), hairpin_frac=0.2)
+    bspline_case(ctx, b)
+
+
+@pytest.mark.parametrize("gl", [3, 5])
+def test_bspline_fp64(ctx, gl):
+    b = W.bspline_batch(20, seed=33, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(8, 60), hairpin_frac=0.3)
+    bspline_case(ctx, b, prec=G.GCV_FP64, gl=gl, per_interval=8, len_bar=1e-10, u_bar=1e-9, tol=1e-14)
+
+
+def test_bspline_vs_reference_gpu_plain(ctx):
+    b = W.bspline_batch(40, seed=34, degrees=(3,), dims=(2,), n_ctrl_range=(8, 100))
+    ref = O.RefGpuPlain(b, O.opts(0, 3, normalize=True))
+    ref.build()
+    Lr, cumr, _, _ = ref.tables()
+    with build(ctx, b, normalize=True) as eng:
+        int_cum, _, _, Lg, _ = eng.tables()
+        assert np.max(np.abs(Lg - Lr) / Lr) <= 1e-6
+        ioff = O.offsets(b.intervals_per_curve())
+        scale = np.repeat(Lr, np.diff(ioff))
+        assert np.max(np.abs(int_cum - cumr) / scale) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Options, edge cases and invariants
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nsub", [1, 8, 64])
+def test_n_subdivided_variants(ctx, nsub):
+    b = W.ellipse_batch(50, seed=41)
+    Lt, _ = O.Truth(b).lengths()
+    with build(ctx, b, normalize=True, n_subdivided=nsub) as eng:
+        assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
+        counts = np.full(50, 64)
+        u = eng.arclengthReverse(counts, None, True)
+        assert np.all(np.diff(u.reshape(50, 64), axis=1) > 0)
+
+
+def test_determinism_and_rebuild(ctx):
+    b = W.bspline_batch(30, seed=42, degrees=(3,), dims=(3,), n_ctrl_range=(8, 80))
+    with build(ctx, b, normalize=True) as eng:
+        t1 = eng.tables()
+        eng.createArclengthTable()
+        t2 = eng.tables()
+        for x, y in zip(t1, t2):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_empty_batch_and_zero_samples(ctx):
+    b = G.EllipseBatch(np.zeros(0), np.zeros(0))
+    with build(ctx, b) as eng:
+        assert eng.getAllCurveArclengths().size == 0
+    b2 = W.ellipse_batch(5, seed=3)
+    with build(ctx, b2) as eng:
+        u = eng.arclengthReverse(np.array([0, 3, 0, 0, 2]), None, True)
+        assert u.size == 5 and np.all(np.isfinite(u))
+
+
+def test_explicit_positions_absolute_and_clamped(ctx):
+    b = W.ellipse_batch(10, seed=5)
+    Lt, _ = O.Truth(b).lengths()
+    rng = np.random.default_rng(6)
+    counts = np.full(10, 20)
+    rel = rng.uniform(0, 1, 200)
+    rel[::37] = 1.0
+    rel[5] = 0.0
+    with build(ctx, b, normalize=True) as eng:
+        u_rel = eng.arclengthReverse(counts, rel, True)
+        u_abs = eng.arclengthReverse(counts, rel * np.repeat(Lt, 20), False)
+        ut, _ = O.Truth(b).reverse(counts, rel)
+        bound = np.repeat(np.maximum(1e-6, 2e-7 * Lt / b.b), counts)
+        assert np.all(np.abs(u_rel - ut) <= bound)
+        assert np.all(np.abs(u_abs - ut) <= 2 * bound)
+        # positions beyond the curve end clamp to the end of the last interval
+        u_over = eng.arclengthReverse(np.array([1] + [0] * 9), np.array([1.5]), True)
+        assert abs(u_over[0] - 2 * math.pi) <= 1e-6
+
+
+def test_degenerate_intervals_have_zero_length(ctx):
+    # repeated interior knot -> a zero-width knot interval
+    knots = np.array([0, 0, 0, 0, 0.4, 0.4, 0.7, 1, 1, 1, 1.0])
+    ctrl = np.random.default_rng(9).normal(size=(7, 3))
+    b = G.BSplineBatch([3], [3], [0, knots.size], [0, 7], knots, ctrl)
+    Lt, cumt = O.Truth(b).lengths()
+    with build(ctx, b) as eng:
+        int_cum, _, _, Lg, _ = eng.tables()
+        assert abs(Lg[0] - Lt[0]) / Lt[0] <= 1e-6
+        assert int_cum[1] == int_cum[0]  # interval 1 is [0.4, 0.4]
+        u = eng.arclengthReverse(np.array([100]), None, True)
+        assert np.all(np.isfinite(u)) and np.all(np.diff(u) >= 0)
+
+
+def test_tolerance_below_float_floor_saturates_depth(ctx):
+    b = W.ellipse_batch(20, seed=8)
+    with build(ctx, b, tolerance=1e-13, max_depth=10) as eng:
+        st = eng.stats()
+        assert st["overflow"] == 0
+        assert st["max_stack"] <= 64 * (10 + 1)
+        Lt, _ = O.Truth(b).lengths()
+        assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
+
+
+def test_max_depth_zero_is_one_level(ctx):
+    b = W.ellipse_batch(20, seed=8)
+    with build(ctx, b, max_depth=0, normalize=True) as eng:
+        st = eng.stats()
+        assert st["deep_roots"] == 0 and st["tasks"] == 0
+        assert st["quadratures"] == 3 * 64 * 20
+
+
+def test_arena_context(ctx):
+    with G.Context(0, arena_bytes=64 << 20) as c2:
+        b = W.bspline_batch(10, seed=4, n_ctrl_range=(8, 40))
+        with build(c2, b, normalize=True) as eng:
+            Lt, _ = O.Truth(b).lengths()
+            assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
+        used, cap = c2.arena_stats()
+        assert cap == 64 << 20
+    with G.Context(0, arena_bytes=1 << 20) as c3:
+        big = W.ellipse_batch(100000, seed=1)
+        with pytest.raises(G.GcvError) as ei:
+            G.ArclengthCalculateGpu(c3, big)
+        assert "OOM" in str(ei.value) or ei.value.code == 2
