@@ -380,6 +380,9 @@ struct gcv_table {
     int32_t* class_of_curve = nullptr;
     unsigned long long* stats = nullptr;
     std::vector<void*> owned;
+    // sampling scratch: curve of the first sample of each sample block
+    int32_t* blk_curve = nullptr;
+    int64_t blk_cap = 0;
 };
 
 template <typename T>
@@ -397,6 +400,7 @@ static void table_free(gcv_table* t) {
     if (t->ctx) {
         hipStreamSynchronize(t->ctx->stream);
         for (void* p : t->owned) gcv_free(t->ctx, p);
+        if (t->blk_curve) gcv_free(t->ctx, t->blk_curve);
     }
     delete t;
 }
@@ -721,19 +725,41 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     A.class_of_curve = t->class_of_curve;
     const LaunchCfg L = cfg_of(c);
     hipError_t e;
+    // block -> curve table (grown on demand; reused while large enough)
+    const int64_t n_blk = (n_total + dev::kSampleBlock - 1) / dev::kSampleBlock;
+    if (n_blk > t->blk_cap) {
+        if (t->blk_curve) {
+            hipStreamSynchronize(c->stream);
+            gcv_free(c, t->blk_curve);
+            t->blk_curve = nullptr;
+            t->blk_cap = 0;
+        }
+        void* p = nullptr;
+        int rc = gcv_alloc(c, sizeof(int32_t) * (size_t)n_blk, &p);
+        if (rc) return rc;
+        t->blk_curve = (int32_t*)p;
+        t->blk_cap = n_blk;
+    }
+    prof_begin(c, "k_block_curve");
+    hipLaunchKernelGGL(dev::k_block_curve, dim3((unsigned)((t->n_curve + 255) / 256)), dim3(256), 0, c->stream,
+                       t->n_curve, d_soff, t->blk_curve);
+    e = hipGetLastError();
+    prof_end(c);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "block-curve launch failed: %s", hipGetErrorString(e));
     const bool f32 = t->o.precision == GCV_FP32;
     const bool g3 = t->o.gl_order == 3;
+    const int32_t* bc = t->blk_curve;
     if (t->kind == KIND_ELLIPSE) {
-        if (f32) e = g3 ? Launch<float, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A)
-                        : Launch<float, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A);
-        else e = g3 ? Launch<double, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A)
-                    : Launch<double, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A);
+        if (f32) e = g3 ? Launch<float, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
+                        : Launch<float, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
+        else e = g3 ? Launch<double, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
+                    : Launch<double, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
     } else {
         const BSplineTableDev T = bspline_dev(t);
-        if (f32) e = g3 ? Launch<float, 3>::sample_bspline(L, t->uniform_class, T, A)
-                        : Launch<float, 5>::sample_bspline(L, t->uniform_class, T, A);
-        else e = g3 ? Launch<double, 3>::sample_bspline(L, t->uniform_class, T, A)
-                    : Launch<double, 5>::sample_bspline(L, t->uniform_class, T, A);
+        if (f32) e = g3 ? Launch<float, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
+                        : Launch<float, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
+        else e = g3 ? Launch<double, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
+                    : Launch<double, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
     }
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "sample launch failed: %s", hipGetErrorString(e));
     return GCV_OK;

@@ -42,12 +42,13 @@ struct Launch {
     // Subdivision of one B-spline class (degree p, dimension D).
     static hipError_t subdivide_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
                                         dev::SubdivArgs<Dt> A);
+    // blk_curve: curve of the first sample of every kSampleBlock samples (k_block_curve).
     static hipError_t sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
-                                     const dev::SampleArgs& A);
+                                     const dev::SampleArgs& A, const int32_t* blk_curve);
     // Sampling of B-splines; class per curve from A.class_of_curve, or
     // uniform_class when that is null.
     static hipError_t sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
-                                     const dev::SampleArgs& A);
+                                     const dev::SampleArgs& A, const int32_t* blk_curve);
     // LDS per block and waves per block of the subdivision kernel.
     static void subdivide_shape(int stack_cap, int* waves_per_block, size_t* lds_bytes);
 };

@@ -87,14 +87,14 @@ hipError_t Launch<Dt, NGL>::subdivide_bspline(const LaunchCfg& c, int p, int D, 
 
 template <typename Dt, int NGL>
 hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
-                                           const SampleArgs& A) {
+                                           const SampleArgs& A, const int32_t* blk_curve) {
     if (A.n_total <= 0) return hipSuccess;
     EllipseEval<Dt> ev{a, b, normalize};
     EllipseEval<double> ev64{a, b, normalize};
-    const int64_t blocks = (A.n_total + 255) / 256;
+    const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_ellipse");
     hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>>), dim3((unsigned)blocks),
-                       dim3(256), 0, c.stream, ev, ev64, A);
+                       dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve);
     hipError_t e = hipGetLastError();
     prof_end(c.owner);
     return e;
@@ -103,14 +103,11 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
 // Mixed-class B-spline sampler: the class is uniform within a curve, so a
 // wave diverges only where its samples straddle curves of different classes.
 template <typename Dt, int NGL>
-__global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A) {
+__global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A,
+                                                        const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t gw0 = g - lane_id();
-    const int64_t gw1 = (gw0 + 63 < A.n_total - 1) ? gw0 + 63 : A.n_total - 1;
-    const int64_t c_lo = upper_bound_i64(A.soff, 0, A.n_curve + 1, gw0) - 1;
-    const int64_t c_hi = upper_bound_i64(A.soff, c_lo, A.n_curve + 1, gw1);
     if (g >= A.n_total) return;
-    const int c = (int)(upper_bound_i64(A.soff, c_lo, c_hi, g) - 1);
+    const int c = sample_curve(A, blk_curve, blockIdx.x, g);
     const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
     int32_t seg = 0;
     double u = 0.0;
@@ -140,12 +137,12 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
 
 template <typename Dt, int NGL>
 hipError_t Launch<Dt, NGL>::sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
-                                           const SampleArgs& A) {
+                                           const SampleArgs& A, const int32_t* blk_curve) {
     if (A.n_total <= 0) return hipSuccess;
-    const int64_t blocks = (A.n_total + 255) / 256;
+    const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_bspline");
-    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL>), dim3((unsigned)blocks), dim3(256), 0, c.stream, T,
-                       uniform_class, A);
+    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL>), dim3((unsigned)blocks), dim3(kSampleBlock), 0, c.stream, T,
+                       uniform_class, A, blk_curve);
     hipError_t e = hipGetLastError();
     prof_end(c.owner);
     return e;

@@ -458,16 +458,33 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     return ev.to_u((int)i, xd);
 }
 
+// Curve of the first sample of every block of kSampleBlock samples: thread per
+// curve writes the blocks whose first sample falls in its range, so the
+// sampler finds a lane's curve with one load and a search over the few
+// curves one block spans.
+constexpr int kSampleBlock = 256;
+
+static __global__ __launch_bounds__(256) void k_block_curve(int n_curve, const int64_t* soff, int32_t* blk_curve) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_curve) return;
+    const int64_t b0 = (soff[c] + kSampleBlock - 1) / kSampleBlock;
+    const int64_t b1 = (soff[c + 1] + kSampleBlock - 1) / kSampleBlock;
+    for (int64_t b = b0; b < b1; b++) blk_curve[b] = (int32_t)c;
+}
+
+// Curve of sample g of block `blk` (blockDim.x == kSampleBlock).
+__device__ __forceinline__ int sample_curve(const SampleArgs& A, const int32_t* blk_curve, int64_t blk, int64_t g) {
+    const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
+    const int64_t c0 = blk_curve[blk];
+    const int64_t c1 = blk + 1 < n_blk ? blk_curve[blk + 1] : A.n_curve - 1;
+    return (int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1);
+}
+
 template <typename Dt, int NGL, class Ev, class Ev64>
-__global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A) {
+__global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // curve range of this wave's samples, then per-lane search inside it
-    const int64_t gw0 = g - lane_id();
-    const int64_t gw1 = (gw0 + 63 < A.n_total - 1) ? gw0 + 63 : A.n_total - 1;
-    const int64_t c_lo = upper_bound_i64(A.soff, 0, A.n_curve + 1, gw0) - 1;
-    const int64_t c_hi = upper_bound_i64(A.soff, c_lo, A.n_curve + 1, gw1);
     if (g >= A.n_total) return;
-    const int c = (int)(upper_bound_i64(A.soff, c_lo, c_hi, g) - 1);
+    const int c = sample_curve(A, blk_curve, blockIdx.x, g);
     int32_t seg;
     const double u = sample_one<Dt, NGL>(ev, ev64, A, c, g, &seg);
     A.out_u[g] = u;

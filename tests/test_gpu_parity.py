@@ -239,12 +239,15 @@ def test_tolerance_below_float_floor_saturates_depth(ctx):
         assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
 
 
-def test_max_depth_zero_is_one_level(ctx):
+def test_max_depth_one_refines_once(ctx):
+    # depth 1: a failing root is split once and its halves are leaves
     b = W.ellipse_batch(20, seed=8)
-    with build(ctx, b, max_depth=0, normalize=True) as eng:
+    with build(ctx, b, max_depth=1, tolerance=1e-9) as eng:
         st = eng.stats()
-        assert st["deep_roots"] == 0 and st["tasks"] == 0
-        assert st["quadratures"] == 3 * 64 * 20
+        assert st["deep_roots"] > 0
+        assert st["tasks"] == 2 * st["deep_roots"]
+        assert st["quadratures"] == 3 * 64 * 20 + 2 * st["tasks"]
+        assert st["max_stack"] <= 128
 
 
 def test_arena_context(ctx):
