@@ -94,7 +94,7 @@ def cpu_baseline(batch, counts, threads, budget_s=20.0):
         pos = O.uniform_positions(sc, closed=True)
         _, t_rev = eng.reverse(sc, pos, True, threads)
         t = t_build + t_rev
-        if t > budget_s / 8 or k >= batch.n_curve:
+        if t >= budget_s / 2.5 or k >= batch.n_curve:
             break
         k = min(batch.n_curve, int(k * max(2.0, min(8.0, (budget_s / 2) / max(t, 1e-3)))))
     n_int = ob.n_int
@@ -106,6 +106,28 @@ def cpu_baseline(batch, counts, threads, budget_s=20.0):
     }
 
 
+def copy_calibration(G, mib, steps):
+    """HBM ceiling of a plain streaming copy (read + write) and the byte
+    count rocprof's FETCH_SIZE/WRITE_SIZE must report for it."""
+    nbytes = mib << 20
+    with G.Context(0) as ctx:
+        src = ctx.empty(nbytes // 8, np.float64)
+        dst = ctx.empty(nbytes // 8, np.float64)
+        for _ in range(2):
+            ctx.copy_d2d(dst, src, nbytes)
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            ctx.copy_d2d(dst, src, nbytes)
+        ms = ctx.timer_stop()
+        t, n = ctx.profile_query("k_copy16")
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "read_kib": nbytes / 1024, "write_kib": nbytes / 1024,
+                          "avg_launch_ms": t / n, "gbs_read_plus_write": 2 * nbytes / (t / n * 1e-3) / 1e9,
+                          "timed_ms": ms, "steps": steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,13 +136,19 @@ def main():
     ap.add_argument("--config", default=os.environ.get("GCV_BENCH_CONFIG", "c1"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only time the library's streaming copy kernel over MIB mebibytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (curves shard independently; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (curves are independent and would "
+              "shard across GPUs by predicted work). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import gcurval_amd as G
+
+    if args.copy_calibration:
+        copy_calibration(G, args.copy_calibration, args.steps)
+        return
 
     t_gen = time.time()
     batch, counts, cfg = workload(args.config)
@@ -143,8 +171,7 @@ def main():
     for _ in range(args.warmup):
         step()
     ctx.sync()
-    st = eng.stats()
-    log(f"[bench] warmup done; build stats {st}")
+    log(f"[bench] warmup done ({args.warmup} steps)")
 
     # timed region: K steps between two stream synchronisations, per-kernel events on
     ctx.profile(True)
@@ -162,6 +189,8 @@ def main():
         if n:
             kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n}
     ctx.profile(False)
+    st = eng.stats()
+    log(f"[bench] timed {args.steps} steps in {ms:.3f} ms; build stats {st}")
     ms_per_step = ms / args.steps
     build_ms = sum(v["avg_ms"] * v["launches"] for k, v in kernels.items() if k != "k_sample") / args.steps
     sample_ms = kernels.get("k_sample", {"total_ms": 0})["total_ms"] / args.steps
@@ -187,8 +216,17 @@ def main():
         dom, flop, dur = "k_subdivide", quad_flop, sub["avg_ms"]
     achieved = flop / (dur * 1e-3) / 1e12 if dur > 0 else 0.0
     # FP32 vector-ALU bound; the peak is MI355X's published FP32 figure.
+    # traffic: HBM bytes per launch of the same kernel on the same workload from
+    # the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a
+    # streaming copy), when they exist.
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        rec = json.load(open(tf)).get(cfg["workload"], {}).get("kernels", {}).get(dom)
+        if rec:
+            traffic = rec["read_bytes"] + rec["write_bytes"]
     roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                 "flop_per_launch": flop, "avg_launch_ms": dur}
     hbm_out_gbs = (n_total * 8) / (smp["avg_ms"] * 1e-3) / 1e9 if smp["avg_ms"] > 0 else 0.0
 

@@ -82,6 +82,7 @@ _SIGS = [
     ("gcv_arena_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("gcv_copy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("gcv_copy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("gcv_copy_d2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("gcv_sync", C.c_int, [C.c_void_p]),
     ("gcv_device_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
     ("gcv_timer_start", C.c_int, [C.c_void_p]),
@@ -190,6 +191,9 @@ class Context:
 
     def sync(self):
         self._check(lib().gcv_sync(self.h))
+
+    def copy_d2d(self, dst: DeviceArray, src: DeviceArray, nbytes: int):
+        self._check(lib().gcv_copy_d2d(self.h, dst.ptr, src.ptr, nbytes))
 
     def device_info(self):
         n = C.c_int()

@@ -277,6 +277,22 @@ int gcv_copy_d2h(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
     return GCV_OK;
 }
 
+int gcv_copy_d2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return set_err(c, GCV_ERR_INVALID, "gcv_copy_d2d needs 16-byte aligned pointers and sizes");
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (!n16) return GCV_OK;
+    const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, (int64_t)c->n_cu * 32);
+    prof_begin(c, "k_copy16");
+    hipLaunchKernelGGL(dev::k_copy16, dim3((unsigned)blocks), dim3(256), 0, c->stream, (uint4*)dst,
+                       (const uint4*)src, n16);
+    hipError_t e = hipGetLastError();
+    prof_end(c);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
 int gcv_sync(gcv_ctx* c) {
     if (!c) return GCV_ERR_INVALID;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -420,7 +436,7 @@ static int resolve_opts(gcv_ctx* c, const gcv_opts* in, gcv_opts* o) {
         return set_err(c, GCV_ERR_INVALID, "max_depth must be in [0, %d]", maxd);
     if (o->tolerance == 0.0) o->tolerance = o->precision == GCV_FP32 ? 1e-6 : 1e-14;
     if (!(o->tolerance > 0.0)) return set_err(c, GCV_ERR_INVALID, "tolerance must be positive");
-    if (o->newton_iters == 0) o->newton_iters = o->precision == GCV_FP32 ? 3 : 4;
+    if (o->newton_iters == 0) o->newton_iters = o->precision == GCV_FP32 ? 2 : 3;
     if (o->newton_iters < 0 || o->newton_iters > 32) return set_err(c, GCV_ERR_INVALID, "newton_iters out of range");
     return GCV_OK;
 }

@@ -83,6 +83,59 @@ GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, co
 #if defined(__HIPCC__)
 
 // ---------------------------------------------------------------------------
+// |sin 2 pi x| and |cos 2 pi x| for x in [0, 1]: r = 4x and its nearest
+// integer q are exact, f = r - q lies in [-1/2, 1/2], and
+//   sin(pi f / 2) = f S(f^2),  cos(pi f / 2) = C(f^2),
+// swapped when q is odd (signs are not needed: only squares are used).
+// FP32: S, C are weighted least-squares fits on f^2 in [0, 1/4]
+// (gcurval_amd/fit_sincos.py; max abs error 8.6e-8 over x in [0, 1]).
+// FP64: S, C are the Taylor series of sin(pi f/2)/f and cos(pi f/2) to
+// f^16 / f^18 (max abs error 1.1e-16).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void abs_sincos_2pi(float x, float& as, float& ac) {
+    const float r = 4.0f * x;
+    const float q = rintf(r);
+    const float f = r - q;
+    const float g = f * f;
+    const float sp = f * fmaf(fmaf(fmaf(-0.004601657390594482f, g, 0.07968003302812576f), g, -0.6459634900093079f), g,
+                              1.5707963705062866f);
+    const float cp = fmaf(fmaf(fmaf(fmaf(0.0009033629903569818f, g, -0.020859936252236366f), g, 0.2536691725254059f), g,
+                               -1.2337005138397217f), g, 1.0f);
+    const bool odd = ((int)q) & 1;
+    as = odd ? cp : sp;
+    ac = odd ? sp : cp;
+}
+__device__ __forceinline__ void abs_sincos_2pi(double x, double& as, double& ac) {
+    const double r = 4.0 * x;
+    const double q = rint(r);
+    const double f = r - q;
+    const double g = f * f;
+    double sp = 6.0669357311061955e-12;
+    sp = fma(sp, g, -6.688035109811468e-10);
+    sp = fma(sp, g, 5.692172921967927e-08);
+    sp = fma(sp, g, -3.598843235212085e-06);
+    sp = fma(sp, g, 0.00016044118478735983);
+    sp = fma(sp, g, -0.004681754135318688);
+    sp = fma(sp, g, 0.07969262624616705);
+    sp = fma(sp, g, -0.6459640975062463);
+    sp = fma(sp, g, 1.5707963267948966);
+    sp *= f;
+    double cp = -5.294400200734623e-13;
+    cp = fma(cp, g, 6.565963114979473e-11);
+    cp = fma(cp, g, -6.386603083791852e-09);
+    cp = fma(cp, g, 4.710874778818172e-07);
+    cp = fma(cp, g, -2.5202042373060607e-05);
+    cp = fma(cp, g, 0.0009192602748394266);
+    cp = fma(cp, g, -0.02086348076335296);
+    cp = fma(cp, g, 0.25366950790104803);
+    cp = fma(cp, g, -1.2337005501361697);
+    cp = fma(cp, g, 1.0);
+    const bool odd = ((long long)q) & 1;
+    as = odd ? cp : sp;
+    ac = odd ? sp : cp;
+}
+
+// ---------------------------------------------------------------------------
 // Ellipse evaluator. Ctx holds 2 pi a' and 2 pi b' with a', b' the optionally
 // normalised semi-axes, so speed(x) = sqrt((A sin 2pi x)^2 + (B cos 2pi x)^2).
 // ---------------------------------------------------------------------------
@@ -107,15 +160,15 @@ struct EllipseEval {
     }
     __device__ __forceinline__ float speed(const Ctx& c, float x) const {
         float sn, cs;
-        sincospif(2.0f * x, &sn, &cs);
-        float u = c.A * sn, v = c.B * cs;
-        return __builtin_amdgcn_sqrtf(u * u + v * v);
+        abs_sincos_2pi(x, sn, cs);
+        const float u = float(c.A) * sn, v = float(c.B) * cs;
+        return __builtin_amdgcn_sqrtf(fmaf(u, u, v * v));
     }
     __device__ __forceinline__ double speed(const Ctx& c, double x) const {
         double sn, cs;
-        sincospi(2.0 * x, &sn, &cs);
-        double u = c.A * sn, v = c.B * cs;
-        return sqrt(u * u + v * v);
+        abs_sincos_2pi(x, sn, cs);
+        const double u = double(c.A) * sn, v = double(c.B) * cs;
+        return sqrt(fma(u, u, v * v));
     }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }

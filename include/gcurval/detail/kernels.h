@@ -230,6 +230,13 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
     }
 }
 
+// Streaming copy in 16-byte vectors (HBM ceiling / counter calibration).
+static __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                       int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 static __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -343,23 +350,53 @@ __device__ __forceinline__ int64_t lower_bound_f64(const double* a, int64_t lo, 
     return lo;
 }
 
-// Safeguarded Newton on F(x) = int_{a}^{x} speed - tau over [a, b].
+// Quadrature of speed over [a, a + d], with the width d carried exactly.
+template <int N, class Ev, typename Dt>
+__device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt d) {
+    const Dt k = d * Dt(0.5);
+    const Dt m = a + k;
+    Dt acc = Dt(0);
+#pragma unroll
+    for (int j = 0; j < N; j++) acc += Dt(GL<N>::w[j]) * ev.speed(c, m + k * Dt(GL<N>::x[j]));
+    return acc * k;
+}
+
+// Inverse arc length on one leaf [a, b] of the adaptive tree: find the offset
+// d in [0, b - a] with int_a^{a+d} speed = tau. The start is the cubic Hermite
+// interpolant of the inverse map x(s) (x' = 1 / speed at both ends); the first
+// Newton step evaluates the integral by quadrature, later steps update it by
+// the trapezoid rule over the (quadratically shrinking) step.
 template <int NGL, class Ev, typename Dt>
-__device__ __forceinline__ Dt newton_in(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
-                                        int iters) {
-    Dt lo = a, hi = b;
-    Dt f = len > Dt(0) ? tau / len : Dt(0);
-    f = f < Dt(0) ? Dt(0) : (f > Dt(1) ? Dt(1) : f);
-    Dt x = a + (b - a) * f;
-    for (int it = 0; it < iters; it++) {
-        const Dt F = quad<NGL>(ev, c, a, x) - tau;
-        if (F > Dt(0)) hi = x; else lo = x;
-        const Dt d = ev.speed(c, x);
-        Dt xn = d > Dt(0) ? x - F / d : Dt(0.5) * (lo + hi);
-        if (!(xn >= lo && xn <= hi)) xn = Dt(0.5) * (lo + hi);
-        x = xn;
+__device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
+                                          int iters) {
+    const Dt w = b - a;
+    Dt t = len > Dt(0) ? tau / len : Dt(0);
+    t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
+    const Dt va = ev.speed(c, a), vb = ev.speed(c, b);
+    Dt d;
+    if (va > Dt(0) && vb > Dt(0) && len > Dt(0)) {
+        const Dt t2 = t * t, t3 = t2 * t;
+        const Dt h10 = t3 - Dt(2) * t2 + t, h01 = Dt(3) * t2 - Dt(2) * t3, h11 = t3 - t2;
+        d = w * h01 + len * (h10 / va + h11 / vb);
+    } else {
+        d = w * t;
     }
-    return x;
+    d = d < Dt(0) ? Dt(0) : (d > w ? w : d);
+    if (iters <= 0) return d;
+    Dt lo = Dt(0), hi = w;
+    Dt F = quad_from<NGL>(ev, c, a, d) - tau;
+    Dt v = ev.speed(c, a + d);
+    for (int it = 0; it < iters; it++) {
+        if (F > Dt(0)) hi = d; else lo = d;
+        Dt dn = v > Dt(0) ? d - F / v : Dt(0.5) * (lo + hi);
+        if (!(dn >= lo && dn <= hi)) dn = Dt(0.5) * (lo + hi);
+        if (it + 1 == iters) return dn;
+        const Dt vn = ev.speed(c, a + dn);
+        F += (dn - d) * (v + vn) * Dt(0.5);
+        d = dn;
+        v = vn;
+    }
+    return d;
 }
 
 // Re-walk the adaptive tree of a refined root in the build's arithmetic and
@@ -442,20 +479,20 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, (Dt)A.eps0, A.max_depth, tau, a, b, tl, len);
         tau = tl;
     }
-    Dt x = newton_in<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters);
-    double xd = (double)x;
+    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters);
+    double dd = (double)d;
     if (sizeof(Dt) < sizeof(double)) {
         // one FP64 Newton step on the same leaf
         const typename Ev64::Ctx c64 = ev64.load((int)i);
-        const double ad = (double)a, bd = (double)b;
-        const double F = quad<NGL>(ev64, c64, ad, xd) - tau;
-        const double d = ev64.speed(c64, xd);
-        if (d > 0.0) {
-            const double xn = xd - F / d;
-            xd = xn < ad ? ad : (xn > bd ? bd : xn);
+        const double ad = (double)a, wd = (double)b - (double)a;
+        const double F = quad_from<NGL>(ev64, c64, ad, dd) - tau;
+        const double v = ev64.speed(c64, ad + dd);
+        if (v > 0.0) {
+            const double dn = dd - F / v;
+            dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
         }
     }
-    return ev.to_u((int)i, xd);
+    return ev.to_u((int)i, (double)a + dd);
 }
 
 // Curve of the first sample of every block of kSampleBlock samples: thread per

@@ -50,7 +50,7 @@ typedef struct gcv_opts {
     int max_depth;     /* 0 = default; <= 18 (FP32), <= 31 (FP64) */
     double tolerance;  /* 0 = default; per-interval absolute tolerance */
     int normalize;     /* 1: test each curve at unit max-abs coordinate (lengths rescaled) */
-    int newton_iters;  /* 0 = default (3 FP32 + 1 FP64 step; 4 in FP64 mode) */
+    int newton_iters;  /* 0 = default: Newton steps after the Hermite start (2 FP32 + 1 FP64; 3 in FP64 mode) */
     int reserved[6];
 } gcv_opts;
 
@@ -67,6 +67,9 @@ int gcv_free(gcv_ctx* ctx, void* dptr);
 int gcv_arena_stats(const gcv_ctx* ctx, size_t* used, size_t* capacity);
 int gcv_copy_h2d(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gcv_copy_d2h(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by the library's streaming kernel (16-byte aligned
+ * pointers and size); enqueued, no sync. Used to measure the HBM ceiling. */
+int gcv_copy_d2d(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gcv_sync(gcv_ctx* ctx);
 int gcv_device_info(gcv_ctx* ctx, int* n_cu, char* name, int name_len);
 
