@@ -204,12 +204,13 @@ def main():
         ds = np.repeat(batch.dim, batch.intervals_per_curve())
         fnode = float(np.mean([f_node_bspline(int(p), int(d)) for p, d in zip(ps[:100000], ds[:100000])]))
     ngl = 3
-    iters = 3
+    iters = 2
     sub = kernels.get("k_subdivide", {"avg_ms": 0.0})
     smp = kernels.get("k_sample", {"avg_ms": 0.0})
     quad_flop = (st["quadratures"] * (ngl * fnode + 5))
-    # sampler per sample: iters x (GL quadrature + one speed) in FP32, then one FP64 step of the same
-    smp_flop = n_total * (iters + 1) * ((ngl + 1) * fnode + 5)
+    # sampler per sample: 2 speeds (Hermite start) + one NGL quadrature and its
+    # speed + (iters - 1) trapezoid speeds in FP32, then one NGL quadrature in FP64
+    smp_flop = n_total * ((2 + ngl + iters + ngl) * fnode + 2 * 5)
     if smp["avg_ms"] >= sub["avg_ms"]:
         dom, flop, dur = "k_sample", smp_flop, smp["avg_ms"]
     else:

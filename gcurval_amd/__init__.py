@@ -51,7 +51,8 @@ class Opts(C.Structure):
         ("tolerance", C.c_double),
         ("normalize", C.c_int),
         ("newton_iters", C.c_int),
-        ("reserved", C.c_int * 6),
+        ("skip_fp64_step", C.c_int),
+        ("reserved", C.c_int * 5),
     ]
 
 
@@ -239,7 +240,7 @@ class Context:
 
 
 def make_opts(precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0, tolerance=0.0,
-              normalize=False, newton_iters=0) -> Opts:
+              normalize=False, newton_iters=0, fp64_step=True) -> Opts:
     o = Opts()
     lib().gcv_opts_default(C.byref(o))
     o.precision = precision
@@ -249,6 +250,7 @@ def make_opts(precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0, tole
     o.tolerance = tolerance
     o.normalize = 1 if normalize else 0
     o.newton_iters = newton_iters
+    o.skip_fp64_step = 0 if fp64_step else 1
     return o
 
 
@@ -307,10 +309,11 @@ class ArclengthCalculateGpu:
     """
 
     def __init__(self, ctx: Context, batch, precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0,
-                 tolerance=0.0, normalize=False, newton_iters=0):
+                 tolerance=0.0, normalize=False, newton_iters=0, fp64_step=True):
         self.ctx = ctx
         self.batch = batch
-        self.opts = make_opts(precision, gl_order, n_subdivided, max_depth, tolerance, normalize, newton_iters)
+        self.opts = make_opts(precision, gl_order, n_subdivided, max_depth, tolerance, normalize, newton_iters,
+                              fp64_step)
         self._bufs = []
         h = C.c_void_p()
         if batch.kind == "ellipse":

@@ -736,6 +736,7 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     A.eps0 = t->eps0;
     A.max_depth = t->o.max_depth;
     A.newton_iters = t->o.newton_iters;
+    A.fp64_step = t->o.skip_fp64_step ? 0 : 1;
     A.out_u = d_out_u;
     A.out_seg = d_out_seg;
     A.class_of_curve = t->class_of_curve;

@@ -41,43 +41,90 @@ constexpr int kCoefStride = 16;
 // dC/dx. Output layout: out[d*p + j] = coefficient of x^j of component d.
 // Returns 0 on success, 1 if the span is degenerate (h <= 0; coefficients 0).
 // ---------------------------------------------------------------------------
-GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, const double* P, int pstride,
-                                        double scale_inv, double* out, double* u0_out, double* h_out) {
-    const int q = p - 1;
+template <int P, int D>
+GCURVAL_HD inline int span_poly(int s, const double* T, const double* Pc, int pstride, double scale_inv,
+                                double* out, double* u0_out, double* h_out) {
+    constexpr int q = P - 1;
     const double ts = T[s], h = T[s + 1] - T[s];
     if (u0_out) *u0_out = ts;
     if (h_out) *h_out = h;
-    for (int i = 0; i < p * D; i++) out[i] = 0.0;
-    if (!(h > 0.0)) return 1;
-    // d[j][deg][comp]
-    double d[5][5][3];
-    for (int j = 0; j <= q; j++) {
-        const int m = s - p + j;
-        const double den = T[m + p + 1] - T[m + 1];
-        for (int c = 0; c < D; c++) {
-            for (int e = 0; e <= q; e++) d[j][e][c] = 0.0;
-            d[j][0][c] = den > 0.0 ? p * (P[(m + 1) * pstride + c] - P[m * pstride + c]) / den : 0.0;
-        }
+    if (!(h > 0.0)) {
+#pragma unroll
+        for (int i = 0; i < P * D; i++) out[i] = 0.0;
+        return 1;
     }
+    // knot-only quantities: derivative-point factors and the linear alphas
+    double fq[q + 1];
+#pragma unroll
+    for (int j = 0; j <= q; j++) {
+        const int m = s - P + j;
+        const double den = T[m + P + 1] - T[m + 1];
+        fq[j] = den > 0.0 ? double(P) / den : 0.0;
+    }
+    double a0[q + 1][q + 1], a1[q + 1][q + 1];
+#pragma unroll
     for (int r = 1; r <= q; r++) {
+#pragma unroll
         for (int j = q; j >= r; j--) {
             const int i = s - q + j;
             const double den = T[i + q - r + 1] - T[i];
-            const double a0 = den > 0.0 ? (ts - T[i]) / den : 0.0;
-            const double a1 = den > 0.0 ? h / den : 0.0;
-            // d_j <- (1 - a0 - a1 x) d_{j-1} + (a0 + a1 x) d_j
-            for (int c = 0; c < D; c++) {
-                double nw[5];
-                for (int e = 0; e <= q; e++) nw[e] = (1.0 - a0) * d[j - 1][e][c] + a0 * d[j][e][c];
-                for (int e = 0; e < q; e++) nw[e + 1] += a1 * (d[j][e][c] - d[j - 1][e][c]);
-                for (int e = 0; e <= q; e++) d[j][e][c] = nw[e];
-            }
+            a0[r][j] = den > 0.0 ? (ts - T[i]) / den : 0.0;
+            a1[r][j] = den > 0.0 ? h / den : 0.0;
         }
     }
     const double f = h * scale_inv;
-    for (int c = 0; c < D; c++)
-        for (int e = 0; e <= q; e++) out[c * p + e] = d[q][e][c] * f;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+        // d[j][e]: coefficient of x^e of the j-th de Boor point (component c)
+        double d[q + 1][q + 1];
+#pragma unroll
+        for (int j = 0; j <= q; j++) {
+            const int m = s - P + j;
+#pragma unroll
+            for (int e = 0; e <= q; e++) d[j][e] = 0.0;
+            d[j][0] = fq[j] * (Pc[(m + 1) * pstride + c] - Pc[m * pstride + c]);
+        }
+#pragma unroll
+        for (int r = 1; r <= q; r++) {
+#pragma unroll
+            for (int j = q; j >= r; j--) {
+                // d_j <- (1 - a0 - a1 x) d_{j-1} + (a0 + a1 x) d_j
+                double nw[q + 1];
+#pragma unroll
+                for (int e = 0; e <= q; e++) nw[e] = (1.0 - a0[r][j]) * d[j - 1][e] + a0[r][j] * d[j][e];
+#pragma unroll
+                for (int e = 0; e < q; e++) nw[e + 1] += a1[r][j] * (d[j][e] - d[j - 1][e]);
+#pragma unroll
+                for (int e = 0; e <= q; e++) d[j][e] = nw[e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e <= q; e++) out[c * P + e] = d[q][e] * f;
+    }
     return 0;
+}
+
+// Runtime (degree, dimension) dispatch onto the fully unrolled span_poly.
+GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, const double* P, int pstride,
+                                        double scale_inv, double* out, double* u0_out, double* h_out) {
+#define GCV_SPAN_CASE(PP, DD) \
+    case 10 * PP + DD:        \
+        return span_poly<PP, DD>(s, T, P, pstride, scale_inv, out, u0_out, h_out);
+    switch (10 * p + D) {
+        GCV_SPAN_CASE(1, 2)
+        GCV_SPAN_CASE(1, 3)
+        GCV_SPAN_CASE(2, 2)
+        GCV_SPAN_CASE(2, 3)
+        GCV_SPAN_CASE(3, 2)
+        GCV_SPAN_CASE(3, 3)
+        GCV_SPAN_CASE(4, 2)
+        GCV_SPAN_CASE(4, 3)
+        GCV_SPAN_CASE(5, 2)
+        GCV_SPAN_CASE(5, 3)
+        default:
+            return 2;
+    }
+#undef GCV_SPAN_CASE
 }
 
 #if defined(__HIPCC__)
@@ -154,9 +201,10 @@ struct EllipseEval {
         double s = fmax(fabs(a[i]), fabs(b[i]));
         return s > 0.0 ? s : 1.0;
     }
-    __device__ __forceinline__ Ctx load(int i) const {
-        double s = 1.0 / scale(i);
-        return Ctx{Dt(kTwoPi * a[i] * s), Dt(kTwoPi * b[i] * s)};
+    __device__ __forceinline__ Ctx load(int i) const { return load(i, 1.0 / scale(i)); }
+    // with the reciprocal of scale(i) already at hand
+    __device__ __forceinline__ Ctx load(int i, double inv_scale) const {
+        return Ctx{Dt(kTwoPi * a[i] * inv_scale), Dt(kTwoPi * b[i] * inv_scale)};
     }
     __device__ __forceinline__ float speed(const Ctx& c, float x) const {
         float sn, cs;
@@ -197,6 +245,8 @@ struct BSplineEval {
         for (int j = 0; j < P * D; j++) k.c[j] = src[j];
         return k;
     }
+    // coefficients are stored pre-normalised
+    __device__ __forceinline__ Ctx load(int i, double) const { return load(i); }
     template <typename T>
     __device__ __forceinline__ T speed(const Ctx& k, T x) const {
         T s = T(0);

@@ -290,6 +290,23 @@ static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B
     }
 }
 
+// One interval's coefficient block; p * D used slots (the rest of the
+// 16-slot block is never read).
+template <int P, int D>
+__device__ __forceinline__ void coef_one(int s, const double* T, const double* Pc, int pstride, double isc,
+                                         double* c64, float* c32, double* u0, double* h) {
+    double out[P * D];
+    double uu, hh;
+    span_poly<P, D>(s, T, Pc, pstride, isc, out, &uu, &hh);
+    *u0 = uu;
+    *h = hh;
+#pragma unroll
+    for (int j = 0; j < P * D; j++) {
+        c64[j] = out[j];
+        if (c32) c32[j] = (float)out[j];
+    }
+}
+
 static __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int64_t* ioff, int64_t n_int,
                                                       const int32_t* int_curve, const double* int_scale,
                                                       double* coef64, float* coef32, double* u0, double* h) {
@@ -298,14 +315,29 @@ static __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const
     const int c = int_curve[i];
     const int p = B.degree[c], D = B.dim[c];
     const int segid = (int)(i - ioff[c]);
-    double out[kCoefStride];
-    for (int j = 0; j < kCoefStride; j++) out[j] = 0.0;
-    bspline_span_poly(p, D, segid + p, B.knots + B.knot_off[c], B.ctrl + B.ctrl_off[c] * B.ctrl_stride,
-                      B.ctrl_stride, 1.0 / int_scale[i], out, &u0[i], &h[i]);
-    for (int j = 0; j < kCoefStride; j++) {
-        coef64[i * kCoefStride + j] = out[j];
-        if (coef32) coef32[i * kCoefStride + j] = (float)out[j];
+    const double* T = B.knots + B.knot_off[c];
+    const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
+    const double isc = 1.0 / int_scale[i];
+#define GCV_COEF_CASE(PP, DD)                                                                   \
+    case 10 * PP + DD:                                                                          \
+        coef_one<PP, DD>(segid + PP, T, Pc, B.ctrl_stride, isc, coef64 + i * kCoefStride,     \
+                         coef32 ? coef32 + i * kCoefStride : nullptr, &u0[i], &h[i]);          \
+        break;
+    switch (10 * p + D) {
+        GCV_COEF_CASE(1, 2)
+        GCV_COEF_CASE(1, 3)
+        GCV_COEF_CASE(2, 2)
+        GCV_COEF_CASE(2, 3)
+        GCV_COEF_CASE(3, 2)
+        GCV_COEF_CASE(3, 3)
+        GCV_COEF_CASE(4, 2)
+        GCV_COEF_CASE(4, 3)
+        GCV_COEF_CASE(5, 2)
+        GCV_COEF_CASE(5, 3)
+        default:
+            break;
     }
+#undef GCV_COEF_CASE
 }
 
 // ---------------------------------------------------------------------------
@@ -327,6 +359,7 @@ struct SampleArgs {
     double eps0;                // level-0 tolerance per root (same value as the build)
     int max_depth;
     int newton_iters;
+    int fp64_step;              // FP32 mode: finish with one FP64 Newton step
     double* out_u;              // [n_total]
     int32_t* out_seg;           // optional local interval index per sample
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
@@ -350,6 +383,11 @@ __device__ __forceinline__ int64_t lower_bound_f64(const double* a, int64_t lo, 
     return lo;
 }
 
+// Reciprocal for Newton updates and start values: one v_rcp (1 ulp) in float,
+// exact division in double.
+__device__ __forceinline__ float nrcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double nrcp(double v) { return 1.0 / v; }
+
 // Quadrature of speed over [a, a + d], with the width d carried exactly.
 template <int N, class Ev, typename Dt>
 __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt d) {
@@ -366,35 +404,40 @@ __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c,
 // interpolant of the inverse map x(s) (x' = 1 / speed at both ends); the first
 // Newton step evaluates the integral by quadrature, later steps update it by
 // the trapezoid rule over the (quadratically shrinking) step.
+// Returns d; *v_out receives the speed at the last evaluated point (a usable
+// Newton derivative for a final step at d).
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
-                                          int iters) {
+                                          int iters, Dt* v_out) {
     const Dt w = b - a;
-    Dt t = len > Dt(0) ? tau / len : Dt(0);
+    Dt t = len > Dt(0) ? tau * nrcp(len) : Dt(0);
     t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
     const Dt va = ev.speed(c, a), vb = ev.speed(c, b);
     Dt d;
     if (va > Dt(0) && vb > Dt(0) && len > Dt(0)) {
         const Dt t2 = t * t, t3 = t2 * t;
         const Dt h10 = t3 - Dt(2) * t2 + t, h01 = Dt(3) * t2 - Dt(2) * t3, h11 = t3 - t2;
-        d = w * h01 + len * (h10 / va + h11 / vb);
+        d = w * h01 + len * (h10 * nrcp(va) + h11 * nrcp(vb));
     } else {
         d = w * t;
     }
     d = d < Dt(0) ? Dt(0) : (d > w ? w : d);
+    *v_out = va;
     if (iters <= 0) return d;
     Dt lo = Dt(0), hi = w;
     Dt F = quad_from<NGL>(ev, c, a, d) - tau;
     Dt v = ev.speed(c, a + d);
+    *v_out = v;
     for (int it = 0; it < iters; it++) {
         if (F > Dt(0)) hi = d; else lo = d;
-        Dt dn = v > Dt(0) ? d - F / v : Dt(0.5) * (lo + hi);
+        Dt dn = v > Dt(0) ? d - F * nrcp(v) : Dt(0.5) * (lo + hi);
         if (!(dn >= lo && dn <= hi)) dn = Dt(0.5) * (lo + hi);
         if (it + 1 == iters) return dn;
         const Dt vn = ev.speed(c, a + dn);
         F += (dn - d) * (v + vn) * Dt(0.5);
         d = dn;
         v = vn;
+        *v_out = v;
     }
     return d;
 }
@@ -459,7 +502,8 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     if (i >= ie) i = ie - 1;
     if (seg_out) *seg_out = (int32_t)(i - ib);
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
-    const double sn = sl / ev.scale((int)i);
+    const double inv_sc = 1.0 / ev.scale((int)i);
+    const double sn = sl * inv_sc;
     const int ls = A.log2_nsub, nsub = 1 << ls;
     const int64_t ro = (int64_t)i << ls;
     int64_t kk = lower_bound_f64(A.root_prefix, ro, ro + nsub, sn) - ro;
@@ -470,7 +514,7 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
 
-    const typename Ev::Ctx ctx = ev.load((int)i);
+    const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
     Dt a, b, xm;
     task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
     Dt len = (Dt)rlen;
@@ -479,16 +523,18 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, (Dt)A.eps0, A.max_depth, tau, a, b, tl, len);
         tau = tl;
     }
-    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters);
+    Dt v_last;
+    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, &v_last);
     double dd = (double)d;
-    if (sizeof(Dt) < sizeof(double)) {
-        // one FP64 Newton step on the same leaf
-        const typename Ev64::Ctx c64 = ev64.load((int)i);
+    if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
+        // one FP64 Newton step on the same leaf: the residual is evaluated in
+        // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
+        // only scales a residual that is already at FP32 rounding level)
+        const typename Ev64::Ctx c64 = ev64.load((int)i, inv_sc);
         const double ad = (double)a, wd = (double)b - (double)a;
         const double F = quad_from<NGL>(ev64, c64, ad, dd) - tau;
-        const double v = ev64.speed(c64, ad + dd);
-        if (v > 0.0) {
-            const double dn = dd - F / v;
+        if (v_last > Dt(0)) {
+            const double dn = dd - F * (double)nrcp(v_last);
             dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
         }
     }

@@ -51,7 +51,8 @@ typedef struct gcv_opts {
     double tolerance;  /* 0 = default; per-interval absolute tolerance */
     int normalize;     /* 1: test each curve at unit max-abs coordinate (lengths rescaled) */
     int newton_iters;  /* 0 = default: Newton steps after the Hermite start (2 FP32 + 1 FP64; 3 in FP64 mode) */
-    int reserved[6];
+    int skip_fp64_step; /* FP32 mode: 1 = no final FP64 Newton step (default 0: the step is taken) */
+    int reserved[5];
 } gcv_opts;
 
 void gcv_opts_default(gcv_opts* o);
