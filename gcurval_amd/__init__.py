@@ -96,6 +96,10 @@ _SIGS = [
     ("gcv_table_create_bspline", C.c_int,
      [C.c_void_p, C.POINTER(_BSplineBatchC), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     ("gcv_table_build", C.c_int, [C.c_void_p]),
+    ("gcv_table_build_uniform", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("gcv_table_build_seeded", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("gcv_table_root_layout", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p)]),
+    ("gcv_table_seed_floor", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_table_destroy", C.c_int, [C.c_void_p]),
     ("gcv_table_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     ("gcv_table_lengths", C.c_int, [C.c_void_p, C.c_void_p]),
@@ -338,6 +342,30 @@ class ArclengthCalculateGpu:
         self.ctx._check(lib().gcv_table_build(self.h))
         self._lengths = None
 
+    def createArclengthTableUniform(self, log2_roots: int, d_cost: "DeviceArray | None" = None):
+        """Build with 2^log2_roots roots per knot interval; optional per-interval quadrature counts."""
+        self.ctx._check(lib().gcv_table_build_uniform(self.h, log2_roots, d_cost.ptr if d_cost is not None else None))
+        self._lengths = None
+
+    def createArclengthTableSeeded(self, d_k: "DeviceArray", d_cost: "DeviceArray | None" = None):
+        """Build with 2^k_i roots for knot interval i (d_k: device int8 per interval)."""
+        self.ctx._check(lib().gcv_table_build_seeded(self.h, d_k.ptr, d_cost.ptr if d_cost is not None else None))
+        self._lengths = None
+
+    def seed_floor(self) -> np.ndarray:
+        """Per-interval minimum k of the last seeded build."""
+        p = C.c_void_p()
+        self.ctx._check(lib().gcv_table_seed_floor(self.h, C.byref(p)))
+        out = np.empty(self.n_intervals, np.int8)
+        if out.nbytes:
+            self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(out), p.value, out.nbytes))
+        return out
+
+    def root_layout(self):
+        sd, ls, ro = C.c_int(), C.c_int(), C.c_void_p()
+        self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(ro)))
+        return bool(sd.value), ls.value, ro.value
+
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)
         self.ctx._check(lib().gcv_table_lengths(self.h, _ptr(out)))
@@ -367,8 +395,16 @@ class ArclengthCalculateGpu:
         """Host copies: (int_cum[n_int], root_prefix[n_int*n_sub], deep[n_int], curve_len, int_off)."""
         ic, rp, dp, cl, io = self.device_views()
         L = lib()
-        n_int, ns = self.n_intervals, self.n_subdivided
-        out = (np.empty(n_int, np.float64), np.empty(n_int * ns, np.float64), np.empty(n_int, np.uint64),
+        n_int = self.n_intervals
+        seeded, ls, ro = self.root_layout()
+        if seeded:
+            roff = np.empty(n_int + 1, np.int64)
+            if roff.nbytes:
+                self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(roff), ro, roff.nbytes))
+            n_roots = int(roff[-1])
+        else:
+            n_roots = n_int << ls
+        out = (np.empty(n_int, np.float64), np.empty(n_roots, np.float64), np.empty(n_int, np.uint64),
                np.empty(self.n_curve, np.float64), np.empty(self.n_curve + 1, np.int64))
         for arr, p in zip(out, (ic, rp, dp, cl, io)):
             if arr.nbytes:

@@ -394,6 +394,19 @@ struct gcv_table {
     double* h = nullptr;
     int32_t* class_list = nullptr;
     int32_t* class_of_curve = nullptr;
+    // seeded builds: per-interval k, root offsets, (class, k) buckets
+    int8_t* seed_k = nullptr;             // [n_int] k used by uniform builds
+    int8_t* seed_floor = nullptr;         // [n_int] safety floor of seeded builds
+    int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
+    int64_t* root_off = nullptr;          // [n_int + 1]
+    int32_t* bucket_list = nullptr;       // [n_int]
+    unsigned long long* bucket_counts = nullptr;  // [kMaxBins]
+    unsigned long long* bucket_cursor = nullptr;  // [kMaxBins]
+    int64_t* bucket_ranges = nullptr;     // [2 kMaxBins]
+    int64_t* scan_blocks = nullptr;       // block sums of the root-offset scan
+    int64_t n_scan_blocks = 0;
+    bool seeded = false;                  // last build used per-interval root counts
+    int built_ls = 6;                     // log2 roots per interval of the last uniform build
     unsigned long long* stats = nullptr;
     std::vector<void*> owned;
     // sampling scratch: curve of the first sample of each sample block
@@ -450,13 +463,22 @@ static int log2i(int v) {
 // Allocation shared by both families.
 static int table_alloc_common(gcv_table* t) {
     int rc;
-    const int64_t nroots = t->n_int << t->log2_nsub;
+    // room for 64 roots per interval: seeded and uniform builds may use up to 2^6
+    const int64_t nroots = t->n_int << std::max(t->log2_nsub, (int)dev::kMaxSeedK);
     if ((rc = talloc(t, &t->ioff, t->n_curve + 1))) return rc;
     if ((rc = talloc(t, &t->int_cum, t->n_int))) return rc;
     if ((rc = talloc(t, &t->root_prefix, nroots))) return rc;
     if ((rc = talloc(t, &t->deep, t->n_int))) return rc;
     if ((rc = talloc(t, &t->curve_len, t->n_curve))) return rc;
     if ((rc = talloc(t, &t->stats, dev::kStatCount))) return rc;
+    // seeded-build arrays (allocated once, so builds never allocate)
+    t->n_scan_blocks = (t->n_int + 256 * dev::kScanItems - 1) / (256 * dev::kScanItems);
+    if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
+        (rc = talloc(t, &t->root_off, t->n_int + 1)) ||
+        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_counts, dev::kMaxBins)) ||
+        (rc = talloc(t, &t->bucket_cursor, dev::kMaxBins)) || (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) ||
+        (rc = talloc(t, &t->scan_blocks, t->n_scan_blocks + 1)))
+        return rc;
     // depth bound of the wave-local stack: 64 (max_depth + 1), plus one block of slack
     t->stack_cap = 64 * (t->o.max_depth + 2);
     const int nsub = 1 << t->log2_nsub;
@@ -565,10 +587,15 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
             r.count = (int64_t)list.size() - r.offset;
             t->classes.push_back(r);
         }
+        std::vector<int8_t> cidx(n);
+        for (int cv = 0; cv < n; cv++)
+            for (size_t r = 0; r < t->classes.size(); r++)
+                if (bspline_class(t->classes[r].p, t->classes[r].D) == coc[cv]) cidx[cv] = (int8_t)r;
         if ((rc = talloc(t, &t->class_list, list.size())) ||
             (rc = gcv_copy_h2d(c, t->class_list, list.data(), sizeof(int32_t) * list.size())) ||
             (rc = talloc(t, &t->class_of_curve, n)) ||
-            (rc = gcv_copy_h2d(c, t->class_of_curve, coc.data(), sizeof(int32_t) * n))) {
+            (rc = gcv_copy_h2d(c, t->class_of_curve, coc.data(), sizeof(int32_t) * n)) ||
+            (rc = talloc(t, &t->cls_idx, n)) || (rc = gcv_copy_h2d(c, t->cls_idx, cidx.data(), n))) {
             table_free(t);
             return rc;
         }
@@ -598,69 +625,192 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
     return BSplineTableDev{t->coef64, t->coef32, t->int_scale, t->u0, t->h};
 }
 
+#define LAUNCH_OR_RETURN(name, ...)            \
+    do {                                       \
+        prof_begin(c, name);                   \
+        hipLaunchKernelGGL(__VA_ARGS__);       \
+        hipError_t _le = hipGetLastError();    \
+        prof_end(c);                           \
+        if (_le != hipSuccess) return _le;     \
+    } while (0)
+
+// Build phases. Setup: statistics reset, and for B-splines the per-curve
+// scale / interval map and the per-interval derivative polynomials.
 template <typename Dt, int NGL>
-static hipError_t build_impl(gcv_table* t) {
+static hipError_t build_setup(gcv_table* t) {
     gcv_ctx* c = t->ctx;
-    const LaunchCfg L = cfg_of(c);
-    hipError_t e;
-    prof_begin(c, "k_fill_stats");
-    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, t->stats, (int)dev::kStatCount, 0ull);
-    e = hipGetLastError();
-    prof_end(c);
-    if (e != hipSuccess) return e;
-    dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
-    if (t->kind == KIND_ELLIPSE) {
-        return Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
-    }
-    // B-spline: scale + interval map, coefficients, subdivision per class, scan
+    LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, t->stats,
+                     (int)dev::kStatCount, 0ull);
+    if (t->kind == KIND_ELLIPSE) return hipSuccess;
     dev::BSplineDev B{t->n_curve, t->bat.ctrl_stride, t->bat.degree, t->bat.dim, t->bat.knot_off,
                       t->bat.ctrl_off, t->bat.knots, t->bat.ctrl};
     if (t->n_curve > 0) {
         const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 8);
-        prof_begin(c, "k_bspline_curve_setup");
-        hipLaunchKernelGGL(dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0, c->stream, B, t->ioff,
-                           t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
-        e = hipGetLastError();
-        prof_end(c);
-        if (e != hipSuccess) return e;
+        LAUNCH_OR_RETURN("k_bspline_curve_setup", dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0,
+                         c->stream, B, t->ioff, t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
     }
     if (t->n_int > 0) {
         const int64_t blocks = (t->n_int + 255) / 256;
-        prof_begin(c, "k_bspline_coef");
-        hipLaunchKernelGGL(dev::k_bspline_coef, dim3((unsigned)blocks), dim3(256), 0, c->stream, B, t->ioff, t->n_int,
-                           t->int_curve, t->int_scale, t->coef64, t->coef32, t->u0, t->h);
-        e = hipGetLastError();
-        prof_end(c);
-        if (e != hipSuccess) return e;
+        LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef, dim3((unsigned)blocks), dim3(256), 0, c->stream, B,
+                         t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32, t->u0, t->h);
     }
-    const BSplineTableDev T = bspline_dev(t);
-    for (const ClassRange& r : t->classes) {
-        dev::SubdivArgs<Dt> Ar = A;
-        if (t->class_list) {
-            Ar.int_list = t->class_list + r.offset;
-            Ar.n_slots = r.count;
-        }
-        e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, Ar);
-        if (e != hipSuccess) return e;
-    }
-    if (t->n_curve > 0) {
-        const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 16);
-        prof_begin(c, "k_curve_scan");
-        hipLaunchKernelGGL(dev::k_curve_scan, dim3((unsigned)blocks), dim3(256), 0, c->stream, t->n_curve, t->ioff,
-                           t->int_cum, t->curve_len);
-        e = hipGetLastError();
-        prof_end(c);
-    }
-    return e;
+    return hipSuccess;
 }
+
+static hipError_t build_scan(gcv_table* t) {
+    gcv_ctx* c = t->ctx;
+    if (t->kind == KIND_ELLIPSE || t->n_curve == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 16);
+    LAUNCH_OR_RETURN("k_curve_scan", dev::k_curve_scan, dim3((unsigned)blocks), dim3(256), 0, c->stream, t->n_curve,
+                     t->ioff, t->int_cum, t->curve_len);
+    return hipSuccess;
+}
+
+// Level-0 tolerance of the plain error test, tol / n_subdivided (the device
+// epsilon table of gcurval/base/constants.inl.h:8-18). Uniform and seeded
+// builds apply it to every root whatever the root count, so their leaves meet
+// at least the plain method's accuracy.
+template <typename Dt>
+static Dt eps_of(const gcv_table* t) {
+    return (Dt)((Dt)t->o.tolerance / (Dt)(1 << t->log2_nsub));
+}
+
+// Uniform build: every interval starts from 2^ls roots (the plain method when
+// ls = log2 n_subdivided); optional per-interval quadrature counts.
+template <typename Dt, int NGL>
+static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
+    hipError_t e = build_setup<Dt, NGL>(t);
+    if (e != hipSuccess) return e;
+    const LaunchCfg L = cfg_of(t->ctx);
+    dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
+    A.log2_nsub = ls;
+    A.eps0 = eps_of<Dt>(t);
+    A.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, ls, sizeof(Dt) == 4);
+    A.stack_cap = 64 * (A.max_depth + 2);
+    A.int_cost = cost;
+    if (t->kind == KIND_ELLIPSE) {
+        e = Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
+    } else {
+        const BSplineTableDev T = bspline_dev(t);
+        for (const ClassRange& r : t->classes) {
+            dev::SubdivArgs<Dt> Ar = A;
+            if (t->class_list) {
+                Ar.int_list = t->class_list + r.offset;
+                Ar.n_slots = r.count;
+            }
+            e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, Ar);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (e != hipSuccess) return e;
+    t->seeded = false;
+    t->built_ls = ls;
+    return build_scan(t);
+}
+
+// Seeded build: interval i starts from 2^k_i roots. Intervals are bucketed
+// by (evaluator class, k) on the device and root storage is compacted by an
+// exclusive scan of 2^k_i; every launch reads its bucket range on the device.
+template <typename Dt, int NGL>
+static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost) {
+    gcv_ctx* c = t->ctx;
+    hipError_t e = build_setup<Dt, NGL>(t);
+    if (e != hipSuccess) return e;
+    const int n_cls = t->kind == KIND_ELLIPSE ? 1 : (int)std::max<size_t>(1, t->classes.size());
+    const int n_bins = n_cls * dev::kSeedBins;
+    const int64_t n = t->n_int;
+    if (n > 0) {
+        const unsigned g1 = (unsigned)((n + 255) / 256);
+        if (t->kind == KIND_ELLIPSE) {
+            LAUNCH_OR_RETURN("k_seed_floor", dev::k_seed_floor_ellipse, dim3(g1), dim3(256), 0, c->stream, n, t->a,
+                             t->b, t->log2_nsub, t->seed_floor);
+        } else {
+            LAUNCH_OR_RETURN("k_seed_floor", dev::k_seed_floor_bspline, dim3(g1), dim3(256), 0, c->stream, n,
+                             t->int_curve, t->bat.degree, t->bat.dim, t->coef64, t->log2_nsub, t->seed_floor);
+        }
+        dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve};
+        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
+        LAUNCH_OR_RETURN("k_fill_bins", dev::k_fill_u64, dim3(1), dim3(128), 0, c->stream, t->bucket_counts,
+                         dev::kMaxBins, 0ull);
+        LAUNCH_OR_RETURN("k_bucket_count", dev::k_bucket_count, dim3(grid), dim3(256), 0, c->stream, n, BA, n_bins,
+                         t->bucket_counts);
+        LAUNCH_OR_RETURN("k_bucket_ranges", dev::k_bucket_ranges, dim3(1), dim3(1), 0, c->stream, t->bucket_counts,
+                         n_bins, t->bucket_ranges, t->bucket_cursor);
+        LAUNCH_OR_RETURN("k_bucket_scatter", dev::k_bucket_scatter, dim3(grid), dim3(256), 0, c->stream, n, BA,
+                         t->bucket_cursor, t->bucket_list);
+        const unsigned nb = (unsigned)t->n_scan_blocks;
+        LAUNCH_OR_RETURN("k_root_block_sums", dev::k_root_block_sums, dim3(nb), dim3(256), 0, c->stream, n, kk,
+                         t->seed_floor, t->scan_blocks);
+        LAUNCH_OR_RETURN("k_scan_block_sums", dev::k_scan_block_sums, dim3(1), dim3(256), 0, c->stream,
+                         (int64_t)nb, t->scan_blocks);
+        LAUNCH_OR_RETURN("k_root_apply", dev::k_root_apply, dim3(nb), dim3(256), 0, c->stream, n, kk,
+                         t->seed_floor, t->scan_blocks, t->root_off);
+    }
+    const LaunchCfg L = cfg_of(c);
+    const BSplineTableDev T = bspline_dev(t);
+    for (int ci = 0; ci < n_cls; ci++) {
+        for (int k = 0; k <= dev::kMaxSeedK; k++) {
+            dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
+            A.log2_nsub = k;
+            A.eps0 = eps_of<Dt>(t);
+            A.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, k, sizeof(Dt) == 4);
+            A.stack_cap = 64 * (A.max_depth + 2);
+            A.int_list = t->bucket_list;
+            A.slot_range = t->bucket_ranges + 2 * (ci * dev::kSeedBins + k);
+            A.n_slots = n;  // upper bound: sizes the persistent grid
+            A.root_off = t->root_off;
+            A.int_cost = cost;
+            if (t->kind == KIND_ELLIPSE) {
+                e = Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
+            } else {
+                const ClassRange& r = t->classes[ci];
+                e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, A);
+            }
+            if (e != hipSuccess) return e;
+        }
+    }
+    t->seeded = true;
+    return build_scan(t);
+}
+
+template <template <typename, int> class F, typename... Args>
+static hipError_t dispatch_build(gcv_table* t, Args... args) {
+    if (t->o.precision == GCV_FP32)
+        return t->o.gl_order == 3 ? F<float, 3>::run(t, args...) : F<float, 5>::run(t, args...);
+    return t->o.gl_order == 3 ? F<double, 3>::run(t, args...) : F<double, 5>::run(t, args...);
+}
+template <typename Dt, int NGL>
+struct UniformBuild {
+    static hipError_t run(gcv_table* t, int ls, int32_t* cost) { return build_uniform<Dt, NGL>(t, ls, cost); }
+};
+template <typename Dt, int NGL>
+struct SeededBuild {
+    static hipError_t run(gcv_table* t, const int8_t* kk, int32_t* cost) { return build_seeded<Dt, NGL>(t, kk, cost); }
+};
 
 extern "C" int gcv_table_build(gcv_table* t) {
     if (!t) return GCV_ERR_INVALID;
-    gcv_ctx* c = t->ctx;
-    hipError_t e;
-    if (t->o.precision == GCV_FP32) e = t->o.gl_order == 3 ? build_impl<float, 3>(t) : build_impl<float, 5>(t);
-    else e = t->o.gl_order == 3 ? build_impl<double, 3>(t) : build_impl<double, 5>(t);
-    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "table build launch failed: %s", hipGetErrorString(e));
+    hipError_t e = dispatch_build<UniformBuild>(t, t->log2_nsub, (int32_t*)nullptr);
+    if (e != hipSuccess) return set_err(t->ctx, GCV_ERR_HIP, "table build launch failed: %s", hipGetErrorString(e));
+    t->built = true;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_cost) {
+    if (!t) return GCV_ERR_INVALID;
+    if (log2_roots < 0 || log2_roots > dev::kMaxSeedK)
+        return set_err(t->ctx, GCV_ERR_INVALID, "log2_roots must be in [0, %d]", dev::kMaxSeedK);
+    hipError_t e = dispatch_build<UniformBuild>(t, log2_roots, d_cost);
+    if (e != hipSuccess) return set_err(t->ctx, GCV_ERR_HIP, "table build launch failed: %s", hipGetErrorString(e));
+    t->built = true;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_build_seeded(gcv_table* t, const int8_t* d_k, int32_t* d_cost) {
+    if (!t) return GCV_ERR_INVALID;
+    if (!d_k && t->n_int > 0) return set_err(t->ctx, GCV_ERR_INVALID, "null per-interval k array");
+    hipError_t e = dispatch_build<SeededBuild>(t, d_k, d_cost);
+    if (e != hipSuccess) return set_err(t->ctx, GCV_ERR_HIP, "seeded build launch failed: %s", hipGetErrorString(e));
     t->built = true;
     return GCV_OK;
 }
@@ -675,6 +825,20 @@ extern "C" int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_inter
     if (n_curve) *n_curve = t->n_curve;
     if (n_intervals) *n_intervals = t->n_int;
     if (n_subdivided) *n_subdivided = 1 << t->log2_nsub;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor) {
+    if (!t || !d_floor) return GCV_ERR_INVALID;
+    *d_floor = t->seed_floor;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int64_t** d_root_off) {
+    if (!t) return GCV_ERR_INVALID;
+    if (seeded) *seeded = t->seeded ? 1 : 0;
+    if (log2_roots) *log2_roots = t->built_ls;
+    if (d_root_off) *d_root_off = t->root_off;
     return GCV_OK;
 }
 
@@ -731,9 +895,11 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     A.int_cum = t->int_cum;
     A.curve_len = t->curve_len;
     A.root_prefix = t->root_prefix;
+    A.root_off = t->seeded ? t->root_off : nullptr;
     A.deep = t->deep;
-    A.log2_nsub = t->log2_nsub;
-    A.eps0 = t->eps0;
+    A.log2_nsub = t->built_ls;
+    A.eps0 = t->o.precision == GCV_FP32 ? (double)eps_of<float>(t) : eps_of<double>(t);
+    A.ls_plain = t->log2_nsub;
     A.max_depth = t->o.max_depth;
     A.newton_iters = t->o.newton_iters;
     A.fp64_step = t->o.skip_fp64_step ? 0 : 1;

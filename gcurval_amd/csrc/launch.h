@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "../../include/gcurval/detail/kernels.h"
+#include "../../include/gcurval/detail/seeded.h"
 
 namespace gcurval {
 namespace rt {

@@ -35,16 +35,19 @@ enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverf
 
 template <typename Dt>
 struct SubdivArgs {
-    int64_t n_slots;          // interval slots (n_int, or int_list length)
-    const int32_t* int_list;  // optional slot -> interval map
-    int log2_nsub;            // roots per interval = 2^log2_nsub (<= 64)
-    Dt eps0;                  // level-0 tolerance per root (tol / n_sub)
+    int64_t n_slots;            // interval slots (n_int, or int_list length)
+    const int32_t* int_list;    // optional slot -> interval map
+    const int64_t* slot_range;  // optional device {offset, count} into int_list; overrides n_slots
+    int log2_nsub;              // roots per interval = 2^log2_nsub (<= 64)
+    Dt eps0;                    // level-0 tolerance per root (tol / n_sub)
     int max_depth;
-    double* root_prefix;      // [n_int << log2_nsub] inclusive prefix in interval (normalised units)
-    double* int_len;          // [n_int] interval length (curve units)
-    double* curve_len;        // optional: one interval per curve -> also curve length
-    uint64_t* deep;           // [n_int] bit k set if root k was refined
-    int stack_cap;            // task-stack entries per wave (LDS)
+    double* root_prefix;        // inclusive prefix of root lengths in each interval (normalised units)
+    const int64_t* root_off;    // optional root base per interval (default i << log2_nsub)
+    double* int_len;            // [n_int] interval length (curve units)
+    double* curve_len;          // optional: one interval per curve -> also curve length
+    uint64_t* deep;             // [n_int] bit k set if root k was refined
+    int32_t* int_cost;          // optional: quadratures spent on each interval
+    int stack_cap;              // task-stack entries per wave (LDS)
     unsigned long long* stats;
 };
 
@@ -67,10 +70,22 @@ __device__ __forceinline__ int key_root_of(uint64_t k) { return key_root(k); }
 __device__ __forceinline__ int key_level_of(uint64_t k) { return key_level(k); }
 __device__ __forceinline__ uint32_t key_idx_of(uint64_t k) { return key_idx(k); }
 
-// Bytes of dynamic LDS per wave for a stack of `cap` entries.
+// Bytes of dynamic LDS per wave for a stack of `cap` entries: per-root FP64
+// accumulators, per-root task counters, then the stack (keys, parent lengths).
 template <typename Dt>
 __host__ __device__ constexpr int subdiv_lds_per_wave(int cap) {
-    return 64 * (int)sizeof(double) + cap * (int)(sizeof(typename KeyOf<Dt>::type) + sizeof(Dt));
+    return 64 * (int)sizeof(double) + 64 * (int)sizeof(int) +
+           cap * (int)(sizeof(typename KeyOf<Dt>::type) + sizeof(Dt));
+}
+
+// Depth cap of a root when an interval starts from 2^ls roots instead of the
+// plain 2^ls_plain: the finest width (interval / 2^(ls_plain + max_depth)) is
+// kept, within the float key / exact-coordinate limits (level <= 20 and
+// ls + level <= 24 in float; ls + level <= 53 and level <= 31 in double).
+__host__ __device__ inline int effective_depth(int max_depth, int ls_plain, int ls, bool fp32) {
+    int d = max_depth + (ls_plain > ls ? ls_plain - ls : 0);
+    const int lim = fp32 ? (20 < 24 - ls ? 20 : 24 - ls) : (31 < 53 - ls ? 31 : 53 - ls);
+    return d < lim ? d : lim;
 }
 
 // Bounds of task (root kr, level l, index idx) in interval-local x; exact.
@@ -99,22 +114,29 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     unsigned char* wbase = smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap);
     double* s_acc = (double*)wbase;
-    Key* tkey = (Key*)(wbase + 64 * sizeof(double));
-    Dt* tpar = (Dt*)(wbase + 64 * sizeof(double) + A.stack_cap * sizeof(Key));
+    int* s_cnt = (int*)(wbase + 64 * sizeof(double));
+    Key* tkey = (Key*)(wbase + 64 * (sizeof(double) + sizeof(int)));
+    Dt* tpar = (Dt*)(wbase + 64 * (sizeof(double) + sizeof(int)) + A.stack_cap * sizeof(Key));
+    const int32_t* list = A.int_list;
+    int64_t n_slots = A.n_slots;
+    if (A.slot_range) {
+        list = A.int_list + A.slot_range[0];
+        n_slots = A.slot_range[1];
+    }
     const int ls = A.log2_nsub;
     const int nsub = 1 << ls;
     const int ipw_log2 = 6 - ls;  // interval slots per wave tile (log2)
     const int jslot = lane >> ls;
     const int k = lane & (nsub - 1);
-    const int64_t n_tiles = (A.n_slots + (1 << ipw_log2) - 1) >> ipw_log2;
+    const int64_t n_tiles = (n_slots + (1 << ipw_log2) - 1) >> ipw_log2;
 
     unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0;
     int max_stack = 0, overflow = 0;
 
     for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
         const int64_t slot = (tile << ipw_log2) + jslot;
-        const bool valid = slot < A.n_slots;
-        const int i = valid ? (A.int_list ? A.int_list[slot] : (int)slot) : (A.int_list ? A.int_list[0] : 0);
+        const bool valid = slot < n_slots;
+        const int i = valid ? (list ? list[slot] : (int)slot) : (list ? list[0] : 0);
         const typename Ev::Ctx ctx = ev.load(i);
 
         Dt x0, xm, x1;
@@ -130,6 +152,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         if (fmask) {
             n_deep += popc64(fmask);
             s_acc[lane] = 0.0;
+            s_cnt[lane] = 0;
             const int pos = 2 * mbcnt(fmask);
             if (fail) {
                 tkey[pos] = make_key<Key>(lane, 1, 0u);
@@ -163,6 +186,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
                 const uint64_t tmask = ballot(take);
                 n_quad += 2ull * popc64(tmask);
                 n_tasks += popc64(tmask);
+                if (A.int_cost && take) atomicAdd(&s_cnt[r], 1);
                 uint64_t m2 = ballot(f2);
                 if (size + 2 * popc64(m2) > A.stack_cap) {
                     // Unreachable while stack_cap >= 64 (max_depth + 1); a bad
@@ -188,9 +212,13 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
 
         // in-interval inclusive prefix over the n_sub roots (FP64)
         const double pre = seg_scan_incl(valid ? (double)len : 0.0, ls);
+        double cost = 0.0;
+        if (A.int_cost) cost = seg_scan_incl(valid ? 3.0 + (fail ? 2.0 * s_cnt[lane] : 0.0) : 0.0, ls);
         if (valid) {
-            A.root_prefix[((int64_t)i << ls) + k] = pre;
+            const int64_t rbase = A.root_off ? A.root_off[i] : ((int64_t)i << ls);
+            A.root_prefix[rbase + k] = pre;
             if (k == nsub - 1) {
+                if (A.int_cost) A.int_cost[i] = (int32_t)cost;
                 const double L_int = pre * ev.scale(i);
                 A.int_len[i] = L_int;
                 if (A.curve_len) A.curve_len[i] = L_int;
@@ -353,10 +381,12 @@ struct SampleArgs {
     const int64_t* ioff;        // [n_curve+1] interval offsets
     const double* int_cum;      // [n_int] cumulative interval lengths (curve units)
     const double* curve_len;    // [n_curve]
-    const double* root_prefix;  // [n_int << log2_nsub]
+    const double* root_prefix;  // inclusive root prefix per interval
+    const int64_t* root_off;    // optional root base per interval (seeded builds); else i << log2_nsub
     const uint64_t* deep;       // [n_int]
     int log2_nsub;
     double eps0;                // level-0 tolerance per root (same value as the build)
+    int ls_plain;               // log2 roots of the plain build (depth caps of seeded roots)
     int max_depth;
     int newton_iters;
     int fp64_step;              // FP32 mode: finish with one FP64 Newton step
@@ -399,6 +429,14 @@ __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c,
     return acc * k;
 }
 
+// Composite rule: `pieces` (1 or 2) equal sub-ranges of [a, a + d].
+template <int N, class Ev, typename Dt>
+__device__ __forceinline__ Dt quad_from_n(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt d, int pieces) {
+    if (pieces <= 1) return quad_from<N>(ev, c, a, d);
+    const Dt h = d * Dt(0.5);
+    return quad_from<N>(ev, c, a, h) + quad_from<N>(ev, c, a + h, h);
+}
+
 // Inverse arc length on one leaf [a, b] of the adaptive tree: find the offset
 // d in [0, b - a] with int_a^{a+d} speed = tau. The start is the cubic Hermite
 // interpolant of the inverse map x(s) (x' = 1 / speed at both ends); the first
@@ -406,9 +444,13 @@ __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c,
 // the trapezoid rule over the (quadratically shrinking) step.
 // Returns d; *v_out receives the speed at the last evaluated point (a usable
 // Newton derivative for a final step at d).
+// `pieces` = 2 for leaves wider than 1/64 of their knot interval: the error
+// test bounds a leaf's quadrature error by tol x (leaf width / interval
+// width), so halving keeps the partial integrals within tol / 64 as for the
+// plain method's 64-way pre-split.
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
-                                          int iters, Dt* v_out) {
+                                          int iters, int pieces, Dt* v_out) {
     const Dt w = b - a;
     Dt t = len > Dt(0) ? tau * nrcp(len) : Dt(0);
     t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
@@ -425,7 +467,7 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
     *v_out = va;
     if (iters <= 0) return d;
     Dt lo = Dt(0), hi = w;
-    Dt F = quad_from<NGL>(ev, c, a, d) - tau;
+    Dt F = quad_from_n<NGL>(ev, c, a, d, pieces) - tau;
     Dt v = ev.speed(c, a + d);
     *v_out = v;
     for (int it = 0; it < iters; it++) {
@@ -504,8 +546,14 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
     const double inv_sc = 1.0 / ev.scale((int)i);
     const double sn = sl * inv_sc;
-    const int ls = A.log2_nsub, nsub = 1 << ls;
-    const int64_t ro = (int64_t)i << ls;
+    int ls = A.log2_nsub;
+    int64_t ro = (int64_t)i << ls;
+    const Dt eps0 = (Dt)A.eps0;
+    if (A.root_off) {
+        ro = A.root_off[i];
+        ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
+    }
+    const int nsub = 1 << ls;
     int64_t kk = lower_bound_f64(A.root_prefix, ro, ro + nsub, sn) - ro;
     if (kk >= nsub) kk = nsub - 1;
     const int k = (int)kk;
@@ -520,11 +568,13 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     Dt len = (Dt)rlen;
     if ((A.deep[i] >> k) & 1ull) {
         double tl;
-        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, (Dt)A.eps0, A.max_depth, tau, a, b, tl, len);
+        const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
+        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
         tau = tl;
     }
     Dt v_last;
-    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, &v_last);
+    const int pieces = (b - a) > Dt(1.0 / 64.0) ? 2 : 1;
+    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last);
     double dd = (double)d;
     if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
         // one FP64 Newton step on the same leaf: the residual is evaluated in
@@ -532,7 +582,7 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         // only scales a residual that is already at FP32 rounding level)
         const typename Ev64::Ctx c64 = ev64.load((int)i, inv_sc);
         const double ad = (double)a, wd = (double)b - (double)a;
-        const double F = quad_from<NGL>(ev64, c64, ad, dd) - tau;
+        const double F = quad_from_n<NGL>(ev64, c64, ad, dd, pieces) - tau;
         if (v_last > Dt(0)) {
             const double dn = dd - F * (double)nrcp(v_last);
             dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);

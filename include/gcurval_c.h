@@ -116,6 +116,31 @@ int gcv_table_create_bspline(gcv_ctx* ctx, const gcv_bspline_batch* batch, const
  * allocation; may be called repeatedly. */
 int gcv_table_build(gcv_table* t);
 
+/* Uniform build with 2^log2_roots roots per knot interval (0..6); the plain
+ * build is log2_roots = log2 n_subdivided. Every root is tested with the
+ * plain level-0 tolerance (tolerance / n_subdivided), halving per level.
+ * d_cost (optional, device int32 [n_intervals]) receives the quadratures
+ * each interval used. */
+int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_cost);
+
+/* Seeded build: knot interval i starts from 2^k_i roots, d_k device int8
+ * [n_intervals] with k_i in 0..6 (for example predicted by the MLP); every
+ * root still passes the plain adaptive error test (tolerance / n_subdivided
+ * at level 0, halving per level), so refinement falls back to the plain
+ * method where the seed is too coarse. No counterpart in the reference (its pre-split is the fixed
+ * N_SUBDIVIDED of gcurval/config.h:20-24). d_cost as above. */
+int gcv_table_build_seeded(gcv_table* t, const int8_t* d_k, int32_t* d_cost);
+
+/* Safety floor of the last seeded build (device int8 [n_intervals]): the
+ * smallest k an interval is allowed, from the ratio of its minimum to maximum
+ * speed (near-zero speed inside a wide root can fool the error test). A
+ * seeded build uses max(k_i, floor_i). */
+int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor);
+
+/* Root layout of the last build: seeded flag, roots per interval (log2) of
+ * a uniform build, device root offsets [n_intervals+1] of a seeded build. */
+int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int64_t** d_root_off);
+
 int gcv_table_destroy(gcv_table* t);
 int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_intervals, int* n_subdivided);
 
