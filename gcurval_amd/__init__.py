@@ -108,6 +108,18 @@ _SIGS = [
     ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5),
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
+    ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("gcv_mlp_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("gcv_mlp_features", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gcv_mlp_predict_k", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("gcv_table_build_mlp", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    ("gcv_table_seed_k", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("gcv_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("gcv_mlp_grad", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
+    ("gcv_mlp_train_step", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
     ("gcv_host_bspline_span_poly", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -365,6 +377,21 @@ class ArclengthCalculateGpu:
         sd, ls, ro = C.c_int(), C.c_int(), C.c_void_p()
         self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(ro)))
         return bool(sd.value), ls.value, ro.value
+
+    def createArclengthTableLearned(self, mlp, hedge: bool = False, d_cost: "DeviceArray | None" = None):
+        """Seeded build whose per-interval root counts come from the learned predictor."""
+        self.ctx._check(lib().gcv_table_build_mlp(self.h, mlp.h, 1 if hedge else 0,
+                                                  d_cost.ptr if d_cost is not None else None))
+        self._lengths = None
+
+    def seed_k(self) -> np.ndarray:
+        """k used by the last learned-seed build (before the safety floor)."""
+        p = C.c_void_p()
+        self.ctx._check(lib().gcv_table_seed_k(self.h, C.byref(p)))
+        out = np.empty(self.n_intervals, np.int8)
+        if out.nbytes:
+            self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(out), p.value, out.nbytes))
+        return out
 
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)

@@ -712,9 +712,9 @@ static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
 // by (evaluator class, k) on the device and root storage is compacted by an
 // exclusive scan of 2^k_i; every launch reads its bucket range on the device.
 template <typename Dt, int NGL>
-static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost) {
+static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bool setup_done = false) {
     gcv_ctx* c = t->ctx;
-    hipError_t e = build_setup<Dt, NGL>(t);
+    hipError_t e = setup_done ? hipSuccess : build_setup<Dt, NGL>(t);
     if (e != hipSuccess) return e;
     const int n_cls = t->kind == KIND_ELLIPSE ? 1 : (int)std::max<size_t>(1, t->classes.size());
     const int n_bins = n_cls * dev::kSeedBins;
@@ -786,6 +786,16 @@ struct UniformBuild {
 template <typename Dt, int NGL>
 struct SeededBuild {
     static hipError_t run(gcv_table* t, const int8_t* kk, int32_t* cost) { return build_seeded<Dt, NGL>(t, kk, cost); }
+};
+template <typename Dt, int NGL>
+struct SetupOnly {
+    static hipError_t run(gcv_table* t) { return build_setup<Dt, NGL>(t); }
+};
+template <typename Dt, int NGL>
+struct SeededAfterSetup {
+    static hipError_t run(gcv_table* t, const int8_t* kk, int32_t* cost) {
+        return build_seeded<Dt, NGL>(t, kk, cost, true);
+    }
 };
 
 extern "C" int gcv_table_build(gcv_table* t) {
@@ -956,4 +966,221 @@ extern "C" int gcv_host_bspline_span_poly(int p, int D, int s, const double* kno
     int rc = dev::bspline_span_poly(p, D, s, knots, ctrl, ctrl_stride, scale_inv, tmp, u0, h);
     for (int i = 0; i < p * D; i++) out[i] = tmp[i];
     return rc ? GCV_ERR_INVALID : GCV_OK;
+}
+
+// ===========================================================================
+// Learned seed (MLP)
+// ===========================================================================
+struct gcv_mlp {
+    gcv_ctx* ctx = nullptr;
+    float* p = nullptr;       // [kParams]
+    float* mean = nullptr;    // [16]
+    float* istd = nullptr;    // [16]
+    uint16_t* w1b = nullptr;
+    uint16_t* w2b = nullptr;
+    uint16_t* w3b = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* grad = nullptr;
+    float* loss = nullptr;
+    int64_t step = 0;
+    std::vector<void*> owned;
+};
+
+static dev::MlpParams mlp_dev(const gcv_mlp* m) {
+    return dev::MlpParams{m->p, m->mean, m->istd, m->w1b, m->w2b, m->w3b};
+}
+
+template <typename T>
+static int malloc_m(gcv_mlp* m, T** p, size_t count) {
+    void* v = nullptr;
+    int rc = gcv_alloc(m->ctx, sizeof(T) * count, &v);
+    if (rc) return rc;
+    m->owned.push_back(v);
+    *p = (T*)v;
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_create(gcv_ctx* c, gcv_mlp** out) {
+    if (!c || !out) return GCV_ERR_INVALID;
+    gcv_mlp* m = new gcv_mlp;
+    m->ctx = c;
+    int rc;
+    if ((rc = malloc_m(m, &m->p, dev::kParams)) || (rc = malloc_m(m, &m->mean, dev::kFeat)) ||
+        (rc = malloc_m(m, &m->istd, dev::kFeat)) || (rc = malloc_m(m, &m->w1b, dev::kFeat * dev::kHid)) ||
+        (rc = malloc_m(m, &m->w2b, dev::kHid * dev::kHid)) || (rc = malloc_m(m, &m->w3b, dev::kHid * 16)) ||
+        (rc = malloc_m(m, &m->adam_m, dev::kParams)) || (rc = malloc_m(m, &m->adam_v, dev::kParams)) ||
+        (rc = malloc_m(m, &m->grad, dev::kParams)) || (rc = malloc_m(m, &m->loss, 2))) {
+        for (void* p : m->owned) gcv_free(c, p);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_destroy(gcv_mlp* m) {
+    if (!m) return GCV_OK;
+    hipStreamSynchronize(m->ctx->stream);
+    for (void* p : m->owned) gcv_free(m->ctx, p);
+    delete m;
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* mean, const float* istd) {
+    if (!m || !params || !mean || !istd) return GCV_ERR_INVALID;
+    gcv_ctx* c = m->ctx;
+    int rc;
+    if ((rc = gcv_copy_h2d(c, m->p, params, sizeof(float) * dev::kParams)) ||
+        (rc = gcv_copy_h2d(c, m->mean, mean, sizeof(float) * dev::kFeat)) ||
+        (rc = gcv_copy_h2d(c, m->istd, istd, sizeof(float) * dev::kFeat)))
+        return rc;
+    std::vector<float> zero(dev::kParams, 0.f);
+    if ((rc = gcv_copy_h2d(c, m->adam_m, zero.data(), sizeof(float) * dev::kParams)) ||
+        (rc = gcv_copy_h2d(c, m->adam_v, zero.data(), sizeof(float) * dev::kParams)))
+        return rc;
+    m->step = 0;
+    hipLaunchKernelGGL(dev::k_mlp_sync_bf16, dim3((dev::kParams + 255) / 256), dim3(256), 0, c->stream, mlp_dev(m));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd) {
+    if (!m) return GCV_ERR_INVALID;
+    gcv_ctx* c = m->ctx;
+    int rc;
+    if (params && (rc = gcv_copy_d2h(c, params, m->p, sizeof(float) * dev::kParams))) return rc;
+    if (mean && (rc = gcv_copy_d2h(c, mean, m->mean, sizeof(float) * dev::kFeat))) return rc;
+    if (istd && (rc = gcv_copy_d2h(c, istd, m->istd, sizeof(float) * dev::kFeat))) return rc;
+    return GCV_OK;
+}
+
+static dev::FeatArgs feat_args(gcv_table* t) {
+    dev::FeatArgs F{};
+    F.n = t->n_int;
+    F.kind = t->kind == KIND_ELLIPSE ? 0 : 1;
+    F.a = t->a;
+    F.b = t->b;
+    F.normalize = t->o.normalize;
+    F.int_curve = t->int_curve;
+    F.degree = t->bat.degree;
+    F.dim = t->bat.dim;
+    F.ioff = t->ioff;
+    F.ctrl_off = t->bat.ctrl_off;
+    F.ctrl = t->bat.ctrl;
+    F.ctrl_stride = t->bat.ctrl_stride;
+    F.coef64 = t->coef64;
+    F.h = t->h;
+    F.tol0 = (float)(t->o.tolerance / (double)(1 << t->log2_nsub));
+    return F;
+}
+
+extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
+    if (!t || (!d_out && t->n_int > 0)) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    hipError_t e = dispatch_build<SetupOnly>(t);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "setup launch failed: %s", hipGetErrorString(e));
+    if (t->n_int == 0) return GCV_OK;
+    prof_begin(c, "k_features");
+    hipLaunchKernelGGL(dev::k_features, dim3((unsigned)((t->n_int + 255) / 256)), dim3(256), 0, c->stream,
+                       feat_args(t), d_out);
+    e = hipGetLastError();
+    prof_end(c);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "features launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
+static hipError_t launch_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
+    gcv_ctx* c = t->ctx;
+    if (t->n_int == 0) return hipSuccess;
+    const int64_t waves = (t->n_int + 63) / 64;
+    const int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8);
+    prof_begin(c, "k_mlp_predict");
+    hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), 4 * dev::kWaveLds, c->stream,
+                       feat_args(t), mlp_dev(m), hedge, d_k, d_raw);
+    hipError_t e = hipGetLastError();
+    prof_end(c);
+    return e;
+}
+
+extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
+    if (!m || !t || (!d_k && t->n_int > 0)) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    hipError_t e = dispatch_build<SetupOnly>(t);
+    if (e == hipSuccess) e = launch_predict(m, t, hedge, d_k, d_raw);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "predict launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost) {
+    if (!t || !m) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    hipError_t e = dispatch_build<SetupOnly>(t);
+    if (e == hipSuccess) e = launch_predict(m, t, hedge, t->seed_k, nullptr);
+    if (e == hipSuccess) e = dispatch_build<SeededAfterSetup>(t, (const int8_t*)t->seed_k, d_cost);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "learned-seed build failed: %s", hipGetErrorString(e));
+    t->built = true;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_seed_k(gcv_table* t, const int8_t** d_k) {
+    if (!t || !d_k) return GCV_ERR_INVALID;
+    *d_k = t->seed_k;
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_forward(gcv_mlp* m, const uint16_t* d_x, int64_t n, float* d_out) {
+    if (!m || n < 0 || (n > 0 && (!d_x || !d_out))) return GCV_ERR_INVALID;
+    gcv_ctx* c = m->ctx;
+    if (n == 0) return GCV_OK;
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t blocks = std::min<int64_t>(tiles, (int64_t)c->n_cu * 8);
+    hipLaunchKernelGGL(dev::k_mlp_forward, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_x, n, mlp_dev(m), d_out);
+    HIPCHK(c, hipGetLastError());
+    return GCV_OK;
+}
+
+static hipError_t launch_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float w_pass) {
+    gcv_ctx* c = m->ctx;
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3((dev::kParams / 2 + 255) / 256), dim3(256), 0, c->stream,
+                       (unsigned long long*)m->grad, dev::kParams / 2, 0ull);
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long*)m->loss, 1, 0ull);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t blocks = std::min<int64_t>(tiles, (int64_t)c->n_cu * 8);
+    prof_begin(c, "k_mlp_train");
+    hipLaunchKernelGGL(dev::k_mlp_train, dim3((unsigned)blocks), dim3(64), dev::kTrainLds, c->stream, d_x, d_y, n,
+                       mlp_dev(m), w_pass, m->grad, m->loss);
+    e = hipGetLastError();
+    prof_end(c);
+    return e;
+}
+
+extern "C" int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float w_pass,
+                            float* host_grad, float* host_loss) {
+    if (!m || n <= 0 || !d_x || !d_y) return GCV_ERR_INVALID;
+    hipError_t e = launch_grad(m, d_x, d_y, n, w_pass);
+    if (e != hipSuccess) return set_err(m->ctx, GCV_ERR_HIP, "grad launch failed: %s", hipGetErrorString(e));
+    int rc;
+    if (host_grad && (rc = gcv_copy_d2h(m->ctx, host_grad, m->grad, sizeof(float) * dev::kParams))) return rc;
+    if (host_loss && (rc = gcv_copy_d2h(m->ctx, host_loss, m->loss, sizeof(float)))) return rc;
+    return GCV_OK;
+}
+
+extern "C" int gcv_mlp_train_step(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float lr,
+                                  float w_pass, float* host_loss) {
+    if (!m || n <= 0 || !d_x || !d_y) return GCV_ERR_INVALID;
+    gcv_ctx* c = m->ctx;
+    hipError_t e = launch_grad(m, d_x, d_y, n, w_pass);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "train launch failed: %s", hipGetErrorString(e));
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.f - powf(b1, (float)m->step), bc2 = 1.f - powf(b2, (float)m->step);
+    hipLaunchKernelGGL(dev::k_mlp_adam, dim3((dev::kParams + 255) / 256), dim3(256), 0, c->stream, mlp_dev(m),
+                       m->adam_m, m->adam_v, m->grad, lr, b1, b2, 1e-8f, bc1, bc2);
+    HIPCHK(c, hipGetLastError());
+    if (host_loss) return gcv_copy_d2h(c, host_loss, m->loss, sizeof(float));
+    return GCV_OK;
 }

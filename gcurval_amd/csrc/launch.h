@@ -8,6 +8,7 @@
 
 #include "../../include/gcurval/detail/kernels.h"
 #include "../../include/gcurval/detail/seeded.h"
+#include "../../include/gcurval/detail/mlp.h"
 
 namespace gcurval {
 namespace rt {

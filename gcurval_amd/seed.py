@@ -1,0 +1,271 @@
+"""Learned seed of the table build (host side).
+
+The plain build starts every knot interval from n_sub = 64 equal roots and
+runs the halves-versus-whole error test on each (the reference's GPU engine,
+gcurval/include/gcurval/arclengthCalculateGpu.h:266-297 and :360-380). Most
+intervals of a smooth curve pass that test with far fewer roots. A seeded
+build (gcv_table_build_seeded) starts interval i from 2^k_i roots, k_i in
+0..6, still testing every root against the plain level-0 tolerance tol/n_sub,
+so accuracy is that of the adaptive test whatever k_i is; the seed only moves
+the work.
+
+This module drives the device predictor of k_i: a 16-64-64-2 perceptron on
+BF16 matrix cores (gcv_mlp_*). Training labels come from the library itself:
+per-interval quadrature counts of uniform-k builds for k = 0..6; the label of
+an interval is the cheapest k at or above its safety floor, and the second
+output learns whether that seed needs no refinement.
+
+``python -m gcurval_amd.seed --train`` regenerates ``seed_weights.npz``.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import time
+
+import numpy as np
+
+from . import (GCV_FP32, ArclengthCalculateGpu, Context, DeviceArray, _ptr, lib)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "seed_weights.npz")
+
+N_FEAT, N_HID, N_OUT = 16, 64, 2
+N_PARAMS = N_FEAT * N_HID + N_HID + N_HID * N_HID + N_HID + N_HID * N_OUT + N_OUT
+MAX_K = 6
+
+FEATURE_NAMES = [
+    "log2 speed x=0", "log2 speed x=1/4", "log2 speed x=1/2", "log2 speed x=3/4", "log2 speed x=1",
+    "log2 speed max/min", "cos(d0,d1)", "cos(d0,d1/2)", "cos(d1/2,d1)", "log2 tol0/length",
+    "log2 lead coef/length", "log2 edge max/min", "min edge cos", "log2 span x intervals", "class code",
+    "min/max speed (17)",
+]
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """float32 -> BF16 bit patterns, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def he_init(seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, np.sqrt(2 / N_FEAT), N_FEAT * N_HID), np.zeros(N_HID),
+             rng.normal(0, np.sqrt(2 / N_HID), N_HID * N_HID), np.zeros(N_HID),
+             rng.normal(0, np.sqrt(1 / N_HID), N_HID * N_OUT), np.zeros(N_OUT)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+class SeedMLP:
+    """Device-resident predictor of per-interval log2 root counts."""
+
+    def __init__(self, ctx: Context, params=None, mean=None, istd=None, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().gcv_mlp_create(ctx.h, C.byref(h)))
+        self.h = h.value
+        self.set_params(he_init(seed) if params is None else params,
+                        np.zeros(N_FEAT, np.float32) if mean is None else mean,
+                        np.ones(N_FEAT, np.float32) if istd is None else istd)
+
+    @classmethod
+    def pretrained(cls, ctx: Context, path: str = WEIGHTS) -> "SeedMLP":
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"learned-seed weights not found: {path}")
+        with np.load(path, allow_pickle=False) as z:
+            return cls(ctx, z["params"], z["mean"], z["istd"])
+
+    def set_params(self, params, mean, istd):
+        p = np.ascontiguousarray(params, np.float32)
+        m = np.ascontiguousarray(mean, np.float32)
+        s = np.ascontiguousarray(istd, np.float32)
+        if p.size != N_PARAMS or m.size != N_FEAT or s.size != N_FEAT:
+            raise ValueError("parameter sizes must be 5378, 16, 16")
+        self.ctx._check(lib().gcv_mlp_set_params(self.h, _ptr(p), _ptr(m), _ptr(s)))
+
+    def get_params(self):
+        p = np.empty(N_PARAMS, np.float32)
+        m = np.empty(N_FEAT, np.float32)
+        s = np.empty(N_FEAT, np.float32)
+        self.ctx._check(lib().gcv_mlp_get_params(self.h, _ptr(p), _ptr(m), _ptr(s)))
+        return p, m, s
+
+    def save(self, path: str = WEIGHTS, **meta):
+        p, m, s = self.get_params()
+        np.savez(path, params=p, mean=m, istd=s, meta=np.array(json.dumps(meta)))
+
+    def features(self, eng: ArclengthCalculateGpu) -> np.ndarray:
+        n = eng.n_intervals
+        d = self.ctx.empty(n * N_FEAT, np.float32)
+        try:
+            self.ctx._check(lib().gcv_mlp_features(eng.h, d.ptr))
+            return d.download().reshape(n, N_FEAT)
+        finally:
+            d.free()
+
+    def predict(self, eng: ArclengthCalculateGpu, hedge: bool = False):
+        """(k int8 [n_intervals], raw outputs float [n_intervals][2])."""
+        n = eng.n_intervals
+        dk = self.ctx.empty(n, np.int8)
+        dr = self.ctx.empty(n * N_OUT, np.float32)
+        try:
+            self.ctx._check(lib().gcv_mlp_predict_k(self.h, eng.h, 1 if hedge else 0, dk.ptr, dr.ptr))
+            return dk.download(), dr.download().reshape(n, N_OUT)
+        finally:
+            dk.free()
+            dr.free()
+
+    def forward(self, d_x: DeviceArray | int, n: int) -> np.ndarray:
+        d = self.ctx.empty(n * N_OUT, np.float32)
+        try:
+            self.ctx._check(lib().gcv_mlp_forward(self.h, _dptr(d_x), n, d.ptr))
+            return d.download().reshape(n, N_OUT)
+        finally:
+            d.free()
+
+    def grad(self, d_x, d_y, n: int, w_pass: float):
+        g = np.empty(N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self.ctx._check(lib().gcv_mlp_grad(self.h, _dptr(d_x), _dptr(d_y), n, w_pass, _ptr(g), _ptr(loss)))
+        return float(loss[0]) / n, g
+
+    def train_step(self, d_x, d_y, n: int, lr: float, w_pass: float, want_loss: bool = False):
+        loss = np.zeros(1, np.float32)
+        self.ctx._check(lib().gcv_mlp_train_step(self.h, _dptr(d_x), _dptr(d_y), n, lr, w_pass,
+                                                 _ptr(loss) if want_loss else None))
+        return float(loss[0]) / n if want_loss else None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gcv_mlp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _dptr(x) -> int:
+    return x.ptr if isinstance(x, DeviceArray) else int(x)
+
+
+# ---------------------------------------------------------------------------
+# Labels and training
+# ---------------------------------------------------------------------------
+def cost_matrix(ctx: Context, batch, precision=GCV_FP32, **opts):
+    """Per-interval quadrature counts of uniform-k builds, k = 0..6, and the
+    safety floor: (cost int64 [7][n_intervals], floor int8 [n_intervals])."""
+    with ArclengthCalculateGpu(ctx, batch, precision=precision, normalize=True, **opts) as eng:
+        n = eng.n_intervals
+        d_cost = ctx.empty(n, np.int32)
+        cost = np.empty((MAX_K + 1, n), np.int64)
+        for k in range(MAX_K + 1):
+            eng.createArclengthTableUniform(k, d_cost)
+            cost[k] = d_cost.download()
+        d_k = ctx.array(np.zeros(n, np.int8))
+        eng.createArclengthTableSeeded(d_k)
+        floor = eng.seed_floor()
+        d_k.free()
+        d_cost.free()
+    return cost, floor
+
+
+def labels_from_costs(cost: np.ndarray, floor: np.ndarray) -> np.ndarray:
+    """y[:, 0] = cheapest k at or above the floor; y[:, 1] = 1 when that seed
+    needs no refinement (3 quadratures per root)."""
+    ks = np.arange(MAX_K + 1)[:, None]
+    masked = np.where(ks >= floor[None, :].astype(np.int64), cost, np.iinfo(np.int64).max)
+    k = np.argmin(masked, axis=0)
+    c = cost[k, np.arange(cost.shape[1])]
+    y = np.stack([k.astype(np.float32), (c == 3 * (1 << k)).astype(np.float32)], axis=1)
+    return y
+
+
+def training_batches(seed: int = 100):
+    """Synthetic B-spline and ellipse batches spanning the classes the predictor serves."""
+    from . import workloads as W
+    out = []
+    for j in range(4):
+        out.append(W.bspline_batch(1500, seed=seed + j, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(16, 512),
+                                   hairpin_frac=0.2))
+    out.append(W.bspline_batch(1500, seed=seed + 10, degrees=(3,), dims=(3,), n_ctrl_range=(8, 1024)))
+    out.append(W.bspline_batch(1000, seed=seed + 11, degrees=(1, 4), dims=(2, 3), n_ctrl_range=(8, 256),
+                               hairpin_frac=0.1))
+    out.append(W.ellipse_batch(100000, seed=seed + 20))
+    return out
+
+
+def make_dataset(ctx: Context, batches, mlp: SeedMLP):
+    X, Y = [], []
+    for b in batches:
+        cost, floor = cost_matrix(ctx, b)
+        with ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
+            X.append(mlp.features(eng))
+        Y.append(labels_from_costs(cost, floor))
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def standardisation(X: np.ndarray):
+    mean = X.mean(0).astype(np.float32)
+    std = X.std(0)
+    return mean, (1.0 / np.maximum(std, 1e-3)).astype(np.float32)
+
+
+def train(ctx: Context, X: np.ndarray, Y: np.ndarray, epochs: int = 30, batch: int = 16384, lr: float = 3e-3,
+          w_pass: float = 0.5, seed: int = 0, log=print) -> SeedMLP:
+    mean, istd = standardisation(X)
+    mlp = SeedMLP(ctx, he_init(seed), mean, istd)
+    Xs = bf16_bits((X - mean) * istd)
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    d_x = ctx.empty(n * N_FEAT, np.uint16)
+    d_y = ctx.empty(n * N_OUT, np.float32)
+    try:
+        for ep in range(epochs):
+            perm = rng.permutation(n)
+            d_x.upload(np.ascontiguousarray(Xs[perm]))
+            d_y.upload(np.ascontiguousarray(Y[perm]))
+            step_lr = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))
+            tot = 0.0
+            for s in range(0, n - batch + 1, batch):
+                last = s + 2 * batch > n
+                l = mlp.train_step(d_x.ptr + s * N_FEAT * 2, d_y.ptr + s * N_OUT * 4, batch, step_lr, w_pass,
+                                   want_loss=last)
+                if last:
+                    tot = l
+            if log:
+                log(f"epoch {ep:3d} lr {step_lr:.2e} last-batch loss {tot:.4f}")
+    finally:
+        d_x.free()
+        d_y.free()
+    return mlp
+
+
+def _main():
+    ap = argparse.ArgumentParser(description="train the learned seed")
+    ap.add_argument("--train", action="store_true")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args()
+    if not a.train:
+        ap.print_help()
+        return
+    from . import workloads as W  # noqa: F401
+    with Context(0) as ctx:
+        t0 = time.time()
+        probe = SeedMLP(ctx)
+        X, Y = make_dataset(ctx, training_batches(), probe)
+        probe.close()
+        print(f"dataset {X.shape[0]} intervals in {time.time() - t0:.1f}s; label histogram "
+              f"{np.bincount(Y[:, 0].astype(int), minlength=7).tolist()}", flush=True)
+        mlp = train(ctx, X, Y, epochs=a.epochs)
+        mlp.save(a.out, n_rows=int(X.shape[0]), epochs=a.epochs)
+        print(f"saved {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -1,0 +1,663 @@
+// Learned seed of the table build: a 16 -> 64 -> 64 -> 2 multilayer
+// perceptron (ReLU) that predicts, per knot interval, log2 of the initial
+// root count (output 0, regression) and the probability that every root of
+// that seed passes the level-0 error test (output 1, logit).
+//
+// Every GEMM runs on BF16 matrix cores with FP32 accumulation
+// (v_mfma_f32_16x16x16bf16_1k, wave64): a wave owns a tile of 64 rows
+// (4 M-tiles of 16), activations move between layers through LDS, and the
+// weight gradients accumulate over the wave's rows in MFMA accumulators
+// before one atomic flush per wave.
+//
+// MFMA 16x16x16 operand layout (lane l, j = 0..3):
+//   A[m][k]: m = l % 16, k = 4 (l / 16) + j
+//   B[k][n]: n = l % 16, k = 4 (l / 16) + j
+//   D[m][n]: n = l % 16, m = 4 (l / 16) + j
+#pragma once
+
+#include "device.h"
+#include "curves.h"
+#include "seeded.h"
+
+namespace gcurval {
+namespace dev {
+
+constexpr int kFeat = 16;
+constexpr int kHid = 64;
+constexpr int kOut = 2;
+constexpr int kParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut;  // 5378
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// Parameter block (FP32 master copy + BF16 operands), offsets into one array.
+struct MlpParams {
+    float* p;          // [kParams] W1[16][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+    float* feat_mean;  // [16]
+    float* feat_istd;  // [16]
+    uint16_t* w1b;     // [16][64]
+    uint16_t* w2b;     // [64][64]
+    uint16_t* w3b;     // [64][16]  (columns >= 2 are zero)
+};
+__host__ __device__ constexpr int off_W1() { return 0; }
+__host__ __device__ constexpr int off_b1() { return kFeat * kHid; }
+__host__ __device__ constexpr int off_W2() { return off_b1() + kHid; }
+__host__ __device__ constexpr int off_b2() { return off_W2() + kHid * kHid; }
+__host__ __device__ constexpr int off_W3() { return off_b2() + kHid; }
+__host__ __device__ constexpr int off_b3() { return off_W3() + kHid * kOut; }
+
+// Orders one wave's LDS writes before its other lanes read them (the wave's
+// LDS operations issue in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// A operand from an LDS tile T[16][ld] (bf16), columns k0..k0+15.
+__device__ __forceinline__ bf16x4 load_a(const uint16_t* T, int ld, int k0) {
+    const int l = lane_id();
+    const uint16_t* r = T + (l & 15) * ld + k0 + 4 * (l >> 4);
+    bf16x4 v;
+    v[0] = (short)r[0];
+    v[1] = (short)r[1];
+    v[2] = (short)r[2];
+    v[3] = (short)r[3];
+    return v;
+}
+// A operand of the transpose of an LDS tile T[16 rows][ld]: A[m][k] = T[k][m0 + m].
+__device__ __forceinline__ bf16x4 load_at(const uint16_t* T, int ld, int m0) {
+    const int l = lane_id();
+    const int m = m0 + (l & 15), k = 4 * (l >> 4);
+    bf16x4 v;
+    v[0] = (short)T[(k + 0) * ld + m];
+    v[1] = (short)T[(k + 1) * ld + m];
+    v[2] = (short)T[(k + 2) * ld + m];
+    v[3] = (short)T[(k + 3) * ld + m];
+    return v;
+}
+// B operand from a row-major global matrix W[K][ldw] (bf16): rows k0.., cols n0..
+__device__ __forceinline__ bf16x4 load_b(const uint16_t* W, int ldw, int k0, int n0) {
+    const int l = lane_id();
+    const int n = n0 + (l & 15), k = k0 + 4 * (l >> 4);
+    bf16x4 v;
+    v[0] = (short)W[(k + 0) * ldw + n];
+    v[1] = (short)W[(k + 1) * ldw + n];
+    v[2] = (short)W[(k + 2) * ldw + n];
+    v[3] = (short)W[(k + 3) * ldw + n];
+    return v;
+}
+// B operand of the transpose of W[N][ldw]: B[k][n] = W[n0 + n][k0 + k].
+__device__ __forceinline__ bf16x4 load_bt(const uint16_t* W, int ldw, int k0, int n0) {
+    const int l = lane_id();
+    const int n = n0 + (l & 15), k = k0 + 4 * (l >> 4);
+    const uint16_t* r = W + n * ldw + k;
+    bf16x4 v;
+    v[0] = (short)r[0];
+    v[1] = (short)r[1];
+    v[2] = (short)r[2];
+    v[3] = (short)r[3];
+    return v;
+}
+// B operand from an LDS tile T[16 rows][ld], rows k (batch), columns n0..
+__device__ __forceinline__ bf16x4 load_b_lds(const uint16_t* T, int ld, int n0) {
+    const int l = lane_id();
+    const int n = n0 + (l & 15), k = 4 * (l >> 4);
+    bf16x4 v;
+    v[0] = (short)T[(k + 0) * ld + n];
+    v[1] = (short)T[(k + 1) * ld + n];
+    v[2] = (short)T[(k + 2) * ld + n];
+    v[3] = (short)T[(k + 3) * ld + n];
+    return v;
+}
+
+// Store a D tile (+bias, ReLU if asked) as bf16 into LDS T[16][ld] at column n0;
+// optionally keep the FP32 values.
+__device__ __forceinline__ void store_d(f32x4 d, const float* bias, bool relu, uint16_t* T, int ld, int n0,
+                                        float* keep, int keep_ld) {
+    const int l = lane_id();
+    const int n = n0 + (l & 15);
+    const float bb = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int m = 4 * (l >> 4) + j;
+        float v = d[j] + bb;
+        if (relu) v = v > 0.0f ? v : 0.0f;
+        T[m * ld + n] = f2bf(v);
+        if (keep) keep[m * keep_ld + n] = v;
+    }
+}
+
+// Forward pass of one 16-row tile. X: LDS [16][16] bf16 (standardised
+// features); H1, H2: LDS [16][64] bf16 activations (kept for training);
+// out: [16][2] FP32 (lane-local writes through `out_store`).
+__device__ __forceinline__ void mlp_forward_tile(const MlpParams& P, const uint16_t* X, uint16_t* H1,
+                                                 uint16_t* H2, float* out, int out_ld) {
+    const float* b1 = P.p + off_b1();
+    const float* b2 = P.p + off_b2();
+    const float* b3 = P.p + off_b3();
+    const bf16x4 ax = load_a(X, kFeat, 0);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma16(ax, load_b(P.w1b, kHid, 0, nt * 16), acc);
+        store_d(acc, b1, true, H1, kHid, nt * 16, nullptr, 0);
+    }
+    wave_lds_sync();
+    bf16x4 ah[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) ah[kt] = load_a(H1, kHid, kt * 16);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) acc = mfma16(ah[kt], load_b(P.w2b, kHid, kt * 16, nt * 16), acc);
+        store_d(acc, b2, true, H2, kHid, nt * 16, nullptr, 0);
+    }
+    wave_lds_sync();
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) acc = mfma16(load_a(H2, kHid, kt * 16), load_b(P.w3b, 16, kt * 16, 0), acc);
+    const int l = lane_id();
+    if ((l & 15) < kOut) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[(4 * (l >> 4) + j) * out_ld + (l & 15)] = acc[j] + b3[l & 15];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-interval features (16, FP32, before standardisation), all in the
+// normalised units the error test sees:
+//   0-4   log2 |dC/dx| at x = 0, 1/4, 1/2, 3/4, 1
+//   5     log2 of max / min of those speeds
+//   6-8   cosines between dC/dx at (0, 1), (0, 1/2), (1/2, 1)
+//   9     log2 (level-0 tolerance / Simpson length estimate)
+//   10    log2 (|leading coefficient| / length estimate)
+//   11    log2 of longest / shortest edge of the span's control polygon
+//   12    smallest cosine between consecutive control-polygon edges
+//   13    log2 (knot span x intervals of the curve)
+//   14    class code (degree / 5 + dim / 10; 0 for ellipses)
+//   15    min / max speed over 17 samples (the safety-floor quantity)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void features_common(const float sp[5], const float dv[5][3], float tol0, float* f) {
+    float mn = sp[0], mx = sp[0];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        f[q] = log2f(sp[q] + 1e-12f);
+        mn = fminf(mn, sp[q]);
+        mx = fmaxf(mx, sp[q]);
+    }
+    f[5] = log2f((mx + 1e-12f) / (mn + 1e-12f));
+    const int pa[3] = {0, 0, 2}, pb[3] = {4, 2, 4};
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int a = pa[t], b = pb[t];
+        const float dot = dv[a][0] * dv[b][0] + dv[a][1] * dv[b][1] + dv[a][2] * dv[b][2];
+        f[6 + t] = dot / (sp[a] * sp[b] + 1e-20f);
+    }
+    const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
+    f[9] = log2f(tol0 / Lest);
+}
+
+template <int P, int D>
+__device__ void features_bspline(const float* c, double h, int n_int_curve, const double* ctrl, int pstride, int s,
+                                 float tol0, float* f) {
+    float sp[5], dv[5][3];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const float x = 0.25f * q;
+        float s2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; d++) dv[q][d] = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            float v = c[d * P + P - 1];
+#pragma unroll
+            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
+            dv[q][d] = v;
+            s2 += v * v;
+        }
+        sp[q] = sqrtf(s2);
+    }
+    features_common(sp, dv, tol0, f);
+    const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
+    float lead = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; d++) lead += c[d * P + P - 1] * c[d * P + P - 1];
+    f[10] = P > 1 ? log2f(sqrtf(lead) / Lest + 1e-12f) : -40.f;
+    // the span's control polygon P_{s-p} .. P_s
+    float emin = 1e30f, emax = 0.f, cmin = 1.f;
+    float pe[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int e = 0; e < P; e++) {
+        float ev[3] = {0.f, 0.f, 0.f}, el = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            ev[d] = (float)(ctrl[(s - P + e + 1) * pstride + d] - ctrl[(s - P + e) * pstride + d]);
+            el += ev[d] * ev[d];
+        }
+        el = sqrtf(el);
+        emin = fminf(emin, el);
+        emax = fmaxf(emax, el);
+        if (e > 0) {
+            const float pl = sqrtf(pe[0] * pe[0] + pe[1] * pe[1] + pe[2] * pe[2]);
+            const float dot = pe[0] * ev[0] + pe[1] * ev[1] + pe[2] * ev[2];
+            cmin = fminf(cmin, dot / (pl * el + 1e-20f));
+        }
+        pe[0] = ev[0];
+        pe[1] = ev[1];
+        pe[2] = ev[2];
+    }
+    f[11] = log2f((emax + 1e-12f) / (emin + 1e-12f));
+    f[12] = cmin;
+    f[13] = log2f((float)(h * n_int_curve) + 1e-12f);
+    f[14] = P * 0.2f + D * 0.1f;
+    // 17-sample min / max speed
+    float m2 = 1e30f, x2 = 0.f;
+    for (int q = 0; q <= 16; q++) {
+        const float x = q * (1.0f / 16.0f);
+        float s2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            float v = c[d * P + P - 1];
+#pragma unroll
+            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
+            s2 += v * v;
+        }
+        m2 = fminf(m2, s2);
+        x2 = fmaxf(x2, s2);
+    }
+    f[15] = x2 > 0.f ? sqrtf(m2 / x2) : 1.f;
+}
+
+// Ellipse (normalised semi-axes a', b'): dC/dx = 2 pi (-a' sin 2 pi x, b' cos 2 pi x).
+__device__ void features_ellipse(double a, double b, float tol0, float* f) {
+    const float A = (float)(kTwoPi * a), B = (float)(kTwoPi * b);
+    float sp[5], dv[5][3];
+    const float sn[5] = {0.f, 1.f, 0.f, -1.f, 0.f}, cs[5] = {1.f, 0.f, -1.f, 0.f, 1.f};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        dv[q][0] = -A * sn[q];
+        dv[q][1] = B * cs[q];
+        dv[q][2] = 0.f;
+        sp[q] = sqrtf(dv[q][0] * dv[q][0] + dv[q][1] * dv[q][1]);
+    }
+    features_common(sp, dv, tol0, f);
+    const float lo = fminf(fabsf(A), fabsf(B)), hi = fmaxf(fabsf(A), fabsf(B));
+    f[10] = log2f((hi + 1e-12f) / (lo + 1e-12f));
+    f[11] = 0.f;
+    f[12] = 1.f;
+    f[13] = 0.f;
+    f[14] = 0.f;
+    f[15] = hi > 0.f ? lo / hi : 1.f;
+}
+
+struct FeatArgs {
+    int64_t n;                 // intervals
+    int kind;                  // 0 ellipse, 1 B-spline
+    const double* a;           // ellipse
+    const double* b;
+    int normalize;
+    // B-spline
+    const int32_t* int_curve;
+    const int32_t* degree;
+    const int32_t* dim;
+    const int64_t* ioff;
+    const int64_t* ctrl_off;
+    const double* ctrl;
+    int ctrl_stride;
+    const double* coef64;      // normalised power-basis coefficients
+    const double* h;
+    float tol0;                // level-0 tolerance (normalised units)
+};
+
+__device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, float* f) {
+    if (F.kind == 0) {
+        const double aa = F.a[i], bb = F.b[i];
+        const double s = F.normalize ? fmax(fmax(fabs(aa), fabs(bb)), 1e-300) : 1.0;
+        features_ellipse(aa / s, bb / s, F.tol0, f);
+        return;
+    }
+    const int c = F.int_curve[i];
+    const int p = F.degree[c], D = F.dim[c];
+    const int segid = (int)(i - F.ioff[c]);
+    const int nint = (int)(F.ioff[c + 1] - F.ioff[c]);
+    const double* ctrl = F.ctrl + F.ctrl_off[c] * F.ctrl_stride;
+    float cf[kCoefStride];
+    for (int j = 0; j < p * D; j++) cf[j] = (float)F.coef64[i * kCoefStride + j];
+    // control-polygon features are ratios, so raw control points serve
+#define GCV_FEAT_CASE(PP, DD)                                                                      \
+    case 10 * PP + DD:                                                                             \
+        features_bspline<PP, DD>(cf, F.h[i], nint, ctrl, F.ctrl_stride, segid + PP, F.tol0, f);   \
+        break;
+    switch (10 * p + D) {
+        GCV_FEAT_CASE(1, 2)
+        GCV_FEAT_CASE(1, 3)
+        GCV_FEAT_CASE(2, 2)
+        GCV_FEAT_CASE(2, 3)
+        GCV_FEAT_CASE(3, 2)
+        GCV_FEAT_CASE(3, 3)
+        GCV_FEAT_CASE(4, 2)
+        GCV_FEAT_CASE(4, 3)
+        GCV_FEAT_CASE(5, 2)
+        GCV_FEAT_CASE(5, 3)
+        default:
+            for (int j = 0; j < kFeat; j++) f[j] = 0.f;
+    }
+#undef GCV_FEAT_CASE
+}
+
+// Raw features of every interval: out[i * 16 + j].
+static __global__ __launch_bounds__(256) void k_features(FeatArgs F, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F.n) return;
+    float f[kFeat];
+    interval_features(F, i, f);
+#pragma unroll
+    for (int j = 0; j < kFeat; j++) out[i * kFeat + j] = f[j];
+}
+
+// ---------------------------------------------------------------------------
+// Inference: features -> standardise -> MLP -> k = clamp(rint(out0)) (+1 when
+// the pass head predicts a level-0 failure and hedge is set). One wave per 64
+// intervals; LDS per wave: X 4x[16][16], H1 [16][64], H2 [16][64] (bf16).
+// ---------------------------------------------------------------------------
+constexpr int kWaveLds = (4 * 16 * kFeat + 2 * 16 * kHid) * 2 + 64 * kOut * 4;
+
+static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParams P, int hedge, int8_t* kk,
+                                                            float* raw_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = lane_id();
+    const int w = threadIdx.x >> 6;
+    unsigned char* base = smem + (size_t)w * kWaveLds;
+    uint16_t* X = (uint16_t*)base;                      // [64][16]
+    uint16_t* H1 = X + 64 * kFeat;                      // [16][64]
+    uint16_t* H2 = H1 + 16 * kHid;                      // [16][64]
+    float* O = (float*)(H2 + 16 * kHid);                // [64][2]
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t t0 = gw * 64; t0 < F.n; t0 += nw * 64) {
+        const int64_t i = t0 + lane;
+        float f[kFeat];
+        if (i < F.n) interval_features(F, i, f);
+        else
+            for (int j = 0; j < kFeat; j++) f[j] = 0.f;
+#pragma unroll
+        for (int j = 0; j < kFeat; j++) X[lane * kFeat + j] = f2bf((f[j] - P.feat_mean[j]) * P.feat_istd[j]);
+        wave_lds_sync();
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            mlp_forward_tile(P, X + mt * 16 * kFeat, H1, H2, O + mt * 16 * kOut, kOut);
+            wave_lds_sync();
+        }
+        if (i < F.n) {
+            const float o0 = O[lane * kOut], o1 = O[lane * kOut + 1];
+            int k = (int)rintf(o0);
+            if (hedge && o1 < 0.f) k += 1;
+            kk[i] = (int8_t)clamp_k(k);
+            if (raw_out) {
+                raw_out[i * kOut] = o0;
+                raw_out[i * kOut + 1] = o1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: forward + backward over a batch of standardised features X (bf16
+// [n][16]) with targets y ([n][2]: log2 seed, pass flag). Loss per row:
+//   (out0 - y0)^2 + w_pass * BCE(sigmoid(out1), y1), averaged over n.
+// Gradients accumulate into grad[kParams] (FP32, zeroed beforehand); the
+// summed loss into loss[0].
+// ---------------------------------------------------------------------------
+constexpr int kTrainLds = (16 * kFeat + 3 * 16 * kHid + 16 * 16) * 2 + (2 * 16 * kHid) * 4;
+
+static __global__ __launch_bounds__(64) void k_mlp_train(const uint16_t* Xg, const float* yg, int64_t n, MlpParams P,
+                                                         float w_pass, float* grad, float* loss) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* X = (uint16_t*)smem;          // [16][16]
+    uint16_t* H1 = X + 16 * kFeat;          // [16][64]
+    uint16_t* H2 = H1 + 16 * kHid;          // [16][64]
+    uint16_t* dH = H2 + 16 * kHid;          // [16][64] dH2 then dH1 (bf16)
+    uint16_t* dO = dH + 16 * kHid;          // [16][16] dOut padded (bf16)
+    float* H1f = (float*)(dO + 16 * 16);    // [16][64] FP32 activations (ReLU masks)
+    float* H2f = H1f + 16 * kHid;           // [16][64]
+    const int l = lane_id();
+    const float* b1 = P.p + off_b1();
+    const float* b2 = P.p + off_b2();
+    const float* b3 = P.p + off_b3();
+
+    // weight-gradient accumulators over this wave's tiles
+    f32x4 gW1[4], gW2[16], gW3[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) gW1[t] = gW3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 16; t++) gW2[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float gb1[4] = {0.f, 0.f, 0.f, 0.f}, gb2[4] = {0.f, 0.f, 0.f, 0.f};  // per column col + 16 nt
+    float gb3 = 0.f, lsum = 0.f;
+
+    const int64_t ntile = (n + 15) / 16;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t r0 = tile * 16;
+        // stage X tile (rows past n are zero and carry zero gradient)
+        for (int e = l; e < 16 * kFeat; e += 64) {
+            const int64_t r = r0 + e / kFeat;
+            X[e] = r < n ? Xg[r0 * kFeat + e] : (uint16_t)0;
+        }
+        wave_lds_sync();
+        // forward, keeping FP32 activations
+        const bf16x4 ax = load_a(X, kFeat, 0);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16(ax, load_b(P.w1b, kHid, 0, nt * 16), acc);
+            store_d(acc, b1, true, H1, kHid, nt * 16, H1f, kHid);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+                acc = mfma16(load_a(H1, kHid, kt * 16), load_b(P.w2b, kHid, kt * 16, nt * 16), acc);
+            store_d(acc, b2, true, H2, kHid, nt * 16, H2f, kHid);
+        }
+        wave_lds_sync();
+        f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) o = mfma16(load_a(H2, kHid, kt * 16), load_b(P.w3b, 16, kt * 16, 0), o);
+        // output gradient, lane holds rows 4(l/16)+j of column l%16
+        const int col = l & 15;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int m = 4 * (l >> 4) + j;
+            const int64_t r = r0 + m;
+            float g = 0.f;
+            if (r < n && col < kOut) {
+                const float out = o[j] + b3[col];
+                const float y = yg[r * kOut + col];
+                if (col == 0) {
+                    const float d = out - y;
+                    lsum += d * d;
+                    g = 2.f * d;
+                } else {
+                    const float sgm = 1.f / (1.f + expf(-out));
+                    lsum += w_pass * (fmaxf(out, 0.f) - out * y + log1pf(expf(-fabsf(out))));
+                    g = w_pass * (sgm - y);
+                }
+                g *= 1.0f / (float)n;
+            }
+            dO[m * 16 + col] = f2bf(g);
+            if (col < kOut) gb3 += g;
+        }
+        wave_lds_sync();
+        // dW3 += H2^T dO  (M = 64 hidden in 4 tiles, K = 16 rows, N = 16 -> 2 used)
+        const bf16x4 bo = load_b_lds(dO, 16, 0);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) gW3[mt] = mfma16(load_at(H2, kHid, mt * 16), bo, gW3[mt]);
+        // dH2 = (dO W3^T) * [H2 > 0]  (K = 16 with only 2 nonzero)
+        const bf16x4 ad = load_a(dO, 16, 0);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16(ad, load_bt(P.w3b, 16, 0, nt * 16), acc);
+            const int nn = nt * 16 + col;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = 4 * (l >> 4) + j;
+                const float v = H2f[m * kHid + nn] > 0.f ? acc[j] : 0.f;
+                dH[m * kHid + nn] = f2bf(v);
+                gb2[nt] += v;
+            }
+        }
+        wave_lds_sync();
+        // dW2 += H1^T dH2  (M = 64 in 4 tiles, K = 16 rows, N = 64 in 4 tiles)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const bf16x4 a = load_at(H1, kHid, mt * 16);
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) gW2[mt * 4 + nt] = mfma16(a, load_b_lds(dH, kHid, nt * 16), gW2[mt * 4 + nt]);
+        }
+        // dH1 = (dH2 W2^T) * [H1 > 0]
+        f32x4 d1[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+                acc = mfma16(load_a(dH, kHid, kt * 16), load_bt(P.w2b, kHid, kt * 16, nt * 16), acc);
+            d1[nt] = acc;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int nn = nt * 16 + col;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = 4 * (l >> 4) + j;
+                const float v = H1f[m * kHid + nn] > 0.f ? d1[nt][j] : 0.f;
+                dH[m * kHid + nn] = f2bf(v);
+                gb1[nt] += v;
+            }
+        }
+        wave_lds_sync();
+        // dW1 += X^T dH1  (M = 16 features, K = 16 rows, N = 64 in 4 tiles)
+        const bf16x4 axt = load_at(X, kFeat, 0);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) gW1[nt] = mfma16(axt, load_b_lds(dH, kHid, nt * 16), gW1[nt]);
+        wave_lds_sync();
+    }
+    // flush: D tiles are [m = 4(l/16)+j][n = l%16]
+    const int col = l & 15;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            atomicAdd(&grad[off_W1() + (4 * (l >> 4) + j) * kHid + nt * 16 + col], gW1[nt][j]);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                atomicAdd(&grad[off_W2() + (mt * 16 + 4 * (l >> 4) + j) * kHid + nt * 16 + col], gW2[mt * 4 + nt][j]);
+    if (col < kOut) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                atomicAdd(&grad[off_W3() + (mt * 16 + 4 * (l >> 4) + j) * kOut + col], gW3[mt][j]);
+    }
+    // bias gradients: lanes l, l^16, l^32, l^48 hold the same columns; reduce over them
+    for (int o = 16; o < 64; o <<= 1) {
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            gb1[nt] += __shfl_xor(gb1[nt], o, 64);
+            gb2[nt] += __shfl_xor(gb2[nt], o, 64);
+        }
+        gb3 += __shfl_xor(gb3, o, 64);
+    }
+    if (l < 16) {
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            atomicAdd(&grad[off_b1() + nt * 16 + l], gb1[nt]);
+            atomicAdd(&grad[off_b2() + nt * 16 + l], gb2[nt]);
+        }
+        if (l < kOut) atomicAdd(&grad[off_b3() + l], gb3);
+    }
+    for (int o = 1; o < 64; o <<= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (l == 0) atomicAdd(loss, lsum);
+}
+
+// BF16 operand copies of the weights (W3 padded to 16 columns).
+__device__ __forceinline__ void sync_bf16_one(const MlpParams& P, int i) {
+    if (i < off_b1()) {
+        P.w1b[i] = f2bf(P.p[i]);
+    } else if (i >= off_W2() && i < off_b2()) {
+        P.w2b[i - off_W2()] = f2bf(P.p[i]);
+    } else if (i >= off_W3() && i < off_b3()) {
+        const int r = (i - off_W3()) / kOut, cc = (i - off_W3()) % kOut;
+        P.w3b[r * 16 + cc] = f2bf(P.p[i]);
+    }
+}
+
+static __global__ void k_mlp_sync_bf16(MlpParams P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kHid * 16 && (i % 16) >= kOut) P.w3b[i] = 0;  // padding columns of W3
+    if (i < kParams) sync_bf16_one(P, i);
+}
+
+// Adam step on the FP32 master weights, then their BF16 copies.
+static __global__ void k_mlp_adam(MlpParams P, float* m, float* v, const float* grad, float lr, float beta1,
+                                  float beta2, float eps, float bc1, float bc2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    const float g = grad[i];
+    const float mi = beta1 * m[i] + (1.f - beta1) * g;
+    const float vi = beta2 * v[i] + (1.f - beta2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    P.p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    sync_bf16_one(P, i);
+}
+
+// Forward only over n standardised bf16 rows (tests and evaluation): out[n][2].
+static __global__ __launch_bounds__(64) void k_mlp_forward(const uint16_t* Xg, int64_t n, MlpParams P, float* outg) {
+    __shared__ uint16_t X[16 * kFeat];
+    __shared__ uint16_t H1[16 * kHid];
+    __shared__ uint16_t H2[16 * kHid];
+    __shared__ float O[16 * kOut];
+    const int l = lane_id();
+    const int64_t ntile = (n + 15) / 16;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t r0 = tile * 16;
+        for (int e = l; e < 16 * kFeat; e += 64) X[e] = (r0 + e / kFeat) < n ? Xg[r0 * kFeat + e] : (uint16_t)0;
+        wave_lds_sync();
+        mlp_forward_tile(P, X, H1, H2, O, kOut);
+        wave_lds_sync();
+        for (int e = l; e < 16 * kOut; e += 64)
+            if (r0 + e / kOut < n) outg[r0 * kOut + e] = O[e];
+        wave_lds_sync();
+    }
+}
+
+}  // namespace dev
+}  // namespace gcurval

@@ -174,6 +174,42 @@ int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, u
 int gcv_sample(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_pos,
                int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg);
 
+/* ---- learned seed: a 16-64-64-2 MLP on BF16 matrix cores (FP32 accumulate)
+ * predicting, per knot interval, log2 of the initial root count and whether
+ * every root of that seed passes the level-0 error test. No counterpart in the
+ * reference; the adaptive error test still verifies every root (see
+ * gcv_table_build_seeded). ---- */
+typedef struct gcv_mlp gcv_mlp;
+int gcv_mlp_create(gcv_ctx* ctx, gcv_mlp** out);
+int gcv_mlp_destroy(gcv_mlp* m);
+/* Host FP32 parameters in the order W1[16][64], b1[64], W2[64][64], b2[64],
+ * W3[64][2], b3[2] (5378 values) and the feature standardisation
+ * (x - mean) * istd (16 + 16). Setting resets the optimiser state. */
+int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* mean, const float* istd);
+int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd);
+/* Raw 16 features of every knot interval of a table (device float
+ * [n_intervals][16]); runs the table's setup kernels first. */
+int gcv_mlp_features(gcv_table* t, float* d_out);
+/* Predicted k (device int8 [n_intervals]) and optionally the raw outputs
+ * (device float [n_intervals][2]); hedge=1 adds one level where the pass head
+ * predicts a level-0 failure. */
+int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw);
+/* Setup + prediction + seeded build in one enqueue (no host sync). */
+int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost);
+/* k predicted by the last gcv_table_build_mlp (device int8 [n_intervals]). */
+int gcv_table_seed_k(gcv_table* t, const int8_t** d_k);
+/* Forward pass over n standardised feature rows given as BF16 bits (device
+ * uint16 [n][16]) into device float [n][2]. */
+int gcv_mlp_forward(gcv_mlp* m, const uint16_t* d_x, int64_t n, float* d_out);
+/* Loss and parameter gradients (host float [5378]) of
+ * mean((out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1)) over n rows, targets
+ * device float [n][2]. Synchronises. */
+int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float w_pass, float* host_grad,
+                 float* host_loss);
+/* One Adam step (beta 0.9 / 0.999) on that loss; host_loss optional (syncs). */
+int gcv_mlp_train_step(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float lr, float w_pass,
+                       float* host_loss);
+
 /* Host-side helpers that run without a GPU (used by the CPU test suite). */
 /* Power-basis derivative coefficients of one B-spline span, as computed by
  * the setup kernel (out[d*p + j] multiplies x^j, x in [0,1] across the span). */
