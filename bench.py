@@ -38,6 +38,11 @@ def f_node_bspline(p, D):
     return 2 * D * (p - 1) + (2 * D - 1) + 1 + 2 + 2
 
 
+# kernels reported per step; a name is a prefix of the launch names it sums
+KERNELS = ("k_subdivide", "k_curve_scan", "k_bspline_curve_setup", "k_bspline_coef", "k_sample", "k_fill_stats",
+           "k_seed_floor", "k_bucket", "k_root", "k_scan_block_sums", "k_mlp_predict", "k_block_curve")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -57,6 +62,15 @@ def workload(name):
         batch = W.bspline_c2(n)
         counts = 32 * batch.intervals_per_curve()
         cfg = {"workload": "c2_cubic3d_bsplines", "n_curve": n, "n_ctrl": "round(exp(U[ln 8, ln 2048]))",
+               "samples_per_interval": 32, "quadrature": "fp32 GL-3, fp64 scan", "n_subdivided": 64,
+               "tolerance": 1e-6, "normalize": True}
+        return batch, counts, cfg
+    if name == "c3":
+        # the cubic 3-D distribution above at ~4M knot intervals (different curves: seed 419)
+        n = int(os.environ.get("GCV_BENCH_CURVES", 11_000))
+        batch = W.bspline_batch(n, 419, degrees=(3,), dims=(3,), n_ctrl_range=(8, 2048), cusp_frac=0.1)
+        counts = 32 * batch.intervals_per_curve()
+        cfg = {"workload": "c3_cubic3d_bsplines_4M", "n_curve": n, "n_ctrl": "round(exp(U[ln 8, ln 2048]))",
                "samples_per_interval": 32, "quadrature": "fp32 GL-3, fp64 scan", "n_subdivided": 64,
                "tolerance": 1e-6, "normalize": True}
         return batch, counts, cfg
@@ -136,6 +150,10 @@ def main():
     ap.add_argument("--config", default=os.environ.get("GCV_BENCH_CONFIG", "c1"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--path", choices=("plain", "learned"), default="plain",
+                    help="table build: plain 64-way pre-split, or the learned per-interval seed")
+    ap.add_argument("--compare-seed", action="store_true",
+                    help="also time the other build path and report both")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time the library's streaming copy kernel over MIB mebibytes")
     args = ap.parse_args()
@@ -163,38 +181,71 @@ def main():
     np.cumsum(counts, out=off[1:])
     d_off = ctx.array(off)
     d_out = ctx.empty(n_total, np.float64)
+    mlp = None
+    if args.path == "learned" or args.compare_seed:
+        from gcurval_amd.seed import SeedMLP
+        mlp = SeedMLP.pretrained(ctx)
 
-    def step():
-        eng.createArclengthTable()
-        eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
+    def run(path):
+        """Warm up, then time args.steps steps of one build path; per-kernel events on."""
+        def step():
+            if path == "learned":
+                eng.createArclengthTableLearned(mlp)
+            else:
+                eng.createArclengthTable()
+            eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
 
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    log(f"[bench] warmup done ({args.warmup} steps)")
+        for _ in range(args.warmup):
+            step()
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(args.steps):
+            step()
+        ms = ctx.timer_stop()
+        ctx.sync()
+        kernels = {}
+        for name in KERNELS:
+            t, n = ctx.profile_query(name)
+            if n:
+                kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n}
+        t_all, _ = ctx.profile_query("k_")
+        t_smp, _ = ctx.profile_query("k_sample")
+        ctx.profile(False)
+        st = eng.stats()
+        log(f"[bench] {path}: timed {args.steps} steps in {ms:.3f} ms; build stats {st}")
+        return {"ms": ms, "kernels": kernels, "stats": st, "build_ms": (t_all - t_smp) / args.steps,
+                "sample_ms": t_smp / args.steps, "lengths": eng.getAllCurveArclengths()}
 
-    # timed region: K steps between two stream synchronisations, per-kernel events on
-    ctx.profile(True)
-    ctx.profile_reset()
-    ctx.sync()
-    ctx.timer_start()
-    for _ in range(args.steps):
-        step()
-    ms = ctx.timer_stop()
-    ctx.sync()
-    kernels = {}
-    for name in ("k_subdivide", "k_curve_scan", "k_bspline_curve_setup", "k_bspline_coef", "k_sample",
-                 "k_fill_stats"):
-        t, n = ctx.profile_query(name)
-        if n:
-            kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n}
-    ctx.profile(False)
-    st = eng.stats()
-    log(f"[bench] timed {args.steps} steps in {ms:.3f} ms; build stats {st}")
+    main_run = run(args.path)
+    ms, kernels, st = main_run["ms"], main_run["kernels"], main_run["stats"]
     ms_per_step = ms / args.steps
-    build_ms = sum(v["avg_ms"] * v["launches"] for k, v in kernels.items() if k != "k_sample") / args.steps
-    sample_ms = kernels.get("k_sample", {"total_ms": 0})["total_ms"] / args.steps
+    build_ms, sample_ms = main_run["build_ms"], main_run["sample_ms"]
     n_int = batch.n_intervals
+    comparison = None
+    if args.compare_seed:
+        other = run("plain" if args.path == "learned" else "learned")
+        runs = {args.path: main_run, ("plain" if args.path == "learned" else "learned"): other}
+        pl, le = runs["plain"], runs["learned"]
+        comparison = {
+            "plain": {"build_ms": pl["build_ms"], "step_ms": pl["ms"] / args.steps,
+                      "quadratures": pl["stats"]["quadratures"],
+                      "build_intervals_per_sec": n_int / (pl["build_ms"] * 1e-3),
+                      "kernel_ms_per_step": {k: v["total_ms"] / args.steps for k, v in pl["kernels"].items()}},
+            "learned": {"build_ms": le["build_ms"], "step_ms": le["ms"] / args.steps,
+                        "quadratures": le["stats"]["quadratures"],
+                        "build_intervals_per_sec": n_int / (le["build_ms"] * 1e-3),
+                        "predict_ms": le["kernels"].get("k_mlp_predict", {"avg_ms": 0.0})["avg_ms"],
+                        "kernel_ms_per_step": {k: v["total_ms"] / args.steps for k, v in le["kernels"].items()}},
+            "build_speedup": pl["build_ms"] / le["build_ms"],
+            "step_speedup": pl["ms"] / le["ms"],
+            "quadrature_ratio": pl["stats"]["quadratures"] / max(1, le["stats"]["quadratures"]),
+            "max_rel_length_diff": float(np.max(np.abs(le["lengths"] - pl["lengths"]) / pl["lengths"])),
+        }
+        log(f"[bench] learned vs plain: build {comparison['build_speedup']:.2f}x, "
+            f"step {comparison['step_speedup']:.2f}x, quadratures {comparison['quadrature_ratio']:.2f}x")
 
     # dominant kernel roofline (FP32 VALU, algorithmic FLOP from counted node evaluations)
     if batch.kind == "ellipse":
@@ -207,17 +258,20 @@ def main():
     iters = 2
     sub = kernels.get("k_subdivide", {"avg_ms": 0.0})
     smp = kernels.get("k_sample", {"avg_ms": 0.0})
+    # subdivision launches per step (one per evaluator class / k bucket)
+    sub_step_ms = sub.get("total_ms", 0.0) / args.steps
+    smp_step_ms = smp.get("total_ms", 0.0) / args.steps
     quad_flop = (st["quadratures"] * (ngl * fnode + 5))
     # sampler per sample: 2 speeds (Hermite start) + one NGL quadrature and its
     # speed + (iters - 1) trapezoid speeds in FP32, then one NGL quadrature in FP64
     smp_flop = n_total * ((2 + ngl + iters + ngl) * fnode + 2 * 5)
-    if smp["avg_ms"] >= sub["avg_ms"]:
-        dom, flop, dur = "k_sample", smp_flop, smp["avg_ms"]
+    if smp_step_ms >= sub_step_ms:
+        dom, flop, dur = "k_sample", smp_flop, smp_step_ms
     else:
-        dom, flop, dur = "k_subdivide", quad_flop, sub["avg_ms"]
+        dom, flop, dur = "k_subdivide", quad_flop, sub_step_ms
     achieved = flop / (dur * 1e-3) / 1e12 if dur > 0 else 0.0
     # FP32 vector-ALU bound; the peak is MI355X's published FP32 figure.
-    # traffic: HBM bytes per launch of the same kernel on the same workload from
+    # traffic: HBM bytes per step of the same kernel on the same workload from
     # the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a
     # streaming copy), when they exist.
     traffic = None
@@ -228,8 +282,8 @@ def main():
             traffic = rec["read_bytes"] + rec["write_bytes"]
     roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                "flop_per_launch": flop, "avg_launch_ms": dur}
-    hbm_out_gbs = (n_total * 8) / (smp["avg_ms"] * 1e-3) / 1e9 if smp["avg_ms"] > 0 else 0.0
+                "flop_per_step": flop, "kernel_ms_per_step": dur}
+    hbm_out_gbs = (n_total * 8) / (smp_step_ms * 1e-3) / 1e9 if smp_step_ms > 0 else 0.0
 
     result = {
         "metric": "curve-intervals/sec and samples/sec at <=1e-6 rel error (plain GPU path, 1 GPU)",
@@ -257,8 +311,10 @@ def main():
         "build_stats": st,
         "roofline": roofline,
         "device": {"arch": arch, "n_cu": n_cu},
-        "path": "plain",
+        "path": args.path,
     }
+    if comparison:
+        result["seed_comparison"] = comparison
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

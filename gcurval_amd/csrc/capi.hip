@@ -400,8 +400,7 @@ struct gcv_table {
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
     int64_t* root_off = nullptr;          // [n_int + 1]
     int32_t* bucket_list = nullptr;       // [n_int]
-    unsigned long long* bucket_counts = nullptr;  // [kMaxBins]
-    unsigned long long* bucket_cursor = nullptr;  // [kMaxBins]
+    int32_t* bucket_hist = nullptr;               // [kMaxBins * kBucketBlocks] per-block bin counts
     int64_t* bucket_ranges = nullptr;     // [2 kMaxBins]
     int64_t* scan_blocks = nullptr;       // block sums of the root-offset scan
     int64_t n_scan_blocks = 0;
@@ -470,13 +469,13 @@ static int table_alloc_common(gcv_table* t) {
     if ((rc = talloc(t, &t->root_prefix, nroots))) return rc;
     if ((rc = talloc(t, &t->deep, t->n_int))) return rc;
     if ((rc = talloc(t, &t->curve_len, t->n_curve))) return rc;
-    if ((rc = talloc(t, &t->stats, dev::kStatCount))) return rc;
+    if ((rc = talloc(t, &t->stats, dev::kStatWords))) return rc;
     // seeded-build arrays (allocated once, so builds never allocate)
     t->n_scan_blocks = (t->n_int + 256 * dev::kScanItems - 1) / (256 * dev::kScanItems);
     if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
         (rc = talloc(t, &t->root_off, t->n_int + 1)) ||
-        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_counts, dev::kMaxBins)) ||
-        (rc = talloc(t, &t->bucket_cursor, dev::kMaxBins)) || (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) ||
+        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketBlocks)) ||
+        (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) ||
         (rc = talloc(t, &t->scan_blocks, t->n_scan_blocks + 1)))
         return rc;
     // depth bound of the wave-local stack: 64 (max_depth + 1), plus one block of slack
@@ -637,10 +636,11 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
 // Build phases. Setup: statistics reset, and for B-splines the per-curve
 // scale / interval map and the per-interval derivative polynomials.
 template <typename Dt, int NGL>
-static hipError_t build_setup(gcv_table* t) {
+static hipError_t build_setup(gcv_table* t, bool reset_stats = true) {
     gcv_ctx* c = t->ctx;
-    LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, t->stats,
-                     (int)dev::kStatCount, 0ull);
+    if (reset_stats)
+        LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3((dev::kStatWords + 255) / 256), dim3(256), 0,
+                         c->stream, t->stats, dev::kStatWords, 0ull);
     if (t->kind == KIND_ELLIPSE) return hipSuccess;
     dev::BSplineDev B{t->n_curve, t->bat.ctrl_stride, t->bat.degree, t->bat.dim, t->bat.knot_off,
                       t->bat.ctrl_off, t->bat.knots, t->bat.ctrl};
@@ -712,7 +712,8 @@ static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
 // by (evaluator class, k) on the device and root storage is compacted by an
 // exclusive scan of 2^k_i; every launch reads its bucket range on the device.
 template <typename Dt, int NGL>
-static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bool setup_done = false) {
+static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bool setup_done = false,
+                               bool floor_done = false) {
     gcv_ctx* c = t->ctx;
     hipError_t e = setup_done ? hipSuccess : build_setup<Dt, NGL>(t);
     if (e != hipSuccess) return e;
@@ -721,23 +722,25 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
     const int64_t n = t->n_int;
     if (n > 0) {
         const unsigned g1 = (unsigned)((n + 255) / 256);
-        if (t->kind == KIND_ELLIPSE) {
+        if (floor_done) {
+            // the learned predictor wrote the floor with its features
+        } else if (t->kind == KIND_ELLIPSE) {
             LAUNCH_OR_RETURN("k_seed_floor", dev::k_seed_floor_ellipse, dim3(g1), dim3(256), 0, c->stream, n, t->a,
                              t->b, t->log2_nsub, t->seed_floor);
         } else {
             LAUNCH_OR_RETURN("k_seed_floor", dev::k_seed_floor_bspline, dim3(g1), dim3(256), 0, c->stream, n,
-                             t->int_curve, t->bat.degree, t->bat.dim, t->coef64, t->log2_nsub, t->seed_floor);
+                             t->int_curve, t->bat.degree, t->bat.dim, t->coef32, t->coef64, t->log2_nsub,
+                             t->seed_floor);
         }
         dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve};
-        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 16);
-        LAUNCH_OR_RETURN("k_fill_bins", dev::k_fill_u64, dim3(1), dim3(128), 0, c->stream, t->bucket_counts,
-                         dev::kMaxBins, 0ull);
-        LAUNCH_OR_RETURN("k_bucket_count", dev::k_bucket_count, dim3(grid), dim3(256), 0, c->stream, n, BA, n_bins,
-                         t->bucket_counts);
-        LAUNCH_OR_RETURN("k_bucket_ranges", dev::k_bucket_ranges, dim3(1), dim3(1), 0, c->stream, t->bucket_counts,
-                         n_bins, t->bucket_ranges, t->bucket_cursor);
-        LAUNCH_OR_RETURN("k_bucket_scatter", dev::k_bucket_scatter, dim3(grid), dim3(256), 0, c->stream, n, BA,
-                         t->bucket_cursor, t->bucket_list);
+        const int nblk = (int)std::min<int64_t>(dev::kBucketBlocks, (n + 2047) / 2048);
+        const int64_t chunk = (n + nblk - 1) / nblk;
+        LAUNCH_OR_RETURN("k_bucket_hist", dev::k_bucket_hist, dim3(nblk), dim3(256), 0, c->stream, n, BA, n_bins,
+                         chunk, t->bucket_hist);
+        LAUNCH_OR_RETURN("k_bucket_scan", dev::k_bucket_scan, dim3(1), dim3(1024), 0, c->stream, n_bins, nblk,
+                         t->bucket_hist, t->bucket_ranges);
+        LAUNCH_OR_RETURN("k_bucket_place", dev::k_bucket_place, dim3(nblk), dim3(256), 0, c->stream, n, BA, n_bins,
+                         chunk, (const int32_t*)t->bucket_hist, t->bucket_list);
         const unsigned nb = (unsigned)t->n_scan_blocks;
         LAUNCH_OR_RETURN("k_root_block_sums", dev::k_root_block_sums, dim3(nb), dim3(256), 0, c->stream, n, kk,
                          t->seed_floor, t->scan_blocks);
@@ -789,12 +792,12 @@ struct SeededBuild {
 };
 template <typename Dt, int NGL>
 struct SetupOnly {
-    static hipError_t run(gcv_table* t) { return build_setup<Dt, NGL>(t); }
+    static hipError_t run(gcv_table* t, bool reset_stats) { return build_setup<Dt, NGL>(t, reset_stats); }
 };
 template <typename Dt, int NGL>
 struct SeededAfterSetup {
     static hipError_t run(gcv_table* t, const int8_t* kk, int32_t* cost) {
-        return build_seeded<Dt, NGL>(t, kk, cost, true);
+        return build_seeded<Dt, NGL>(t, kk, cost, true, true);
     }
 };
 
@@ -874,9 +877,18 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
                                uint64_t* overflow, uint64_t* tasks) {
     if (!t) return GCV_ERR_INVALID;
     if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
-    unsigned long long s[dev::kStatCount];
-    int rc = gcv_copy_d2h(t->ctx, s, t->stats, sizeof s);
+    std::vector<unsigned long long> rows(dev::kStatWords);
+    int rc = gcv_copy_d2h(t->ctx, rows.data(), t->stats, sizeof(unsigned long long) * dev::kStatWords);
     if (rc) return rc;
+    unsigned long long s[dev::kStatCount] = {};
+    for (int r = 0; r < dev::kStatSlots; r++) {
+        const unsigned long long* row = rows.data() + r * dev::kStatStride;
+        s[dev::kStatQuad] += row[dev::kStatQuad];
+        s[dev::kStatDeepRoots] += row[dev::kStatDeepRoots];
+        s[dev::kStatTasks] += row[dev::kStatTasks];
+        s[dev::kStatMaxStack] = std::max(s[dev::kStatMaxStack], row[dev::kStatMaxStack]);
+        s[dev::kStatOverflow] = std::max(s[dev::kStatOverflow], row[dev::kStatOverflow]);
+    }
     if (quadratures) *quadratures = s[dev::kStatQuad];
     if (deep_roots) *deep_roots = s[dev::kStatDeepRoots];
     if (max_stack) *max_stack = s[dev::kStatMaxStack];
@@ -1071,6 +1083,8 @@ static dev::FeatArgs feat_args(gcv_table* t) {
     F.ctrl = t->bat.ctrl;
     F.ctrl_stride = t->bat.ctrl_stride;
     F.coef64 = t->coef64;
+    F.coef32 = t->coef32;
+    F.ls_plain = t->log2_nsub;
     F.h = t->h;
     F.tol0 = (float)(t->o.tolerance / (double)(1 << t->log2_nsub));
     return F;
@@ -1079,7 +1093,7 @@ static dev::FeatArgs feat_args(gcv_table* t) {
 extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
     if (!t || (!d_out && t->n_int > 0)) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
-    hipError_t e = dispatch_build<SetupOnly>(t);
+    hipError_t e = dispatch_build<SetupOnly>(t, false);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "setup launch failed: %s", hipGetErrorString(e));
     if (t->n_int == 0) return GCV_OK;
     prof_begin(c, "k_features");
@@ -1091,14 +1105,14 @@ extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
     return GCV_OK;
 }
 
-static hipError_t launch_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
+static hipError_t launch_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, int8_t* d_floor, float* d_raw) {
     gcv_ctx* c = t->ctx;
     if (t->n_int == 0) return hipSuccess;
     const int64_t waves = (t->n_int + 63) / 64;
     const int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8);
     prof_begin(c, "k_mlp_predict");
     hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), 4 * dev::kWaveLds, c->stream,
-                       feat_args(t), mlp_dev(m), hedge, d_k, d_raw);
+                       feat_args(t), mlp_dev(m), hedge, d_k, d_floor, d_raw);
     hipError_t e = hipGetLastError();
     prof_end(c);
     return e;
@@ -1107,8 +1121,8 @@ static hipError_t launch_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_
 extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
     if (!m || !t || (!d_k && t->n_int > 0)) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
-    hipError_t e = dispatch_build<SetupOnly>(t);
-    if (e == hipSuccess) e = launch_predict(m, t, hedge, d_k, d_raw);
+    hipError_t e = dispatch_build<SetupOnly>(t, false);
+    if (e == hipSuccess) e = launch_predict(m, t, hedge, d_k, nullptr, d_raw);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "predict launch failed: %s", hipGetErrorString(e));
     return GCV_OK;
 }
@@ -1116,8 +1130,8 @@ extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_
 extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost) {
     if (!t || !m) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
-    hipError_t e = dispatch_build<SetupOnly>(t);
-    if (e == hipSuccess) e = launch_predict(m, t, hedge, t->seed_k, nullptr);
+    hipError_t e = dispatch_build<SetupOnly>(t, true);
+    if (e == hipSuccess) e = launch_predict(m, t, hedge, t->seed_k, t->seed_floor, nullptr);
     if (e == hipSuccess) e = dispatch_build<SeededAfterSetup>(t, (const int8_t*)t->seed_k, d_cost);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "learned-seed build failed: %s", hipGetErrorString(e));
     t->built = true;

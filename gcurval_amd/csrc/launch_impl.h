@@ -17,6 +17,8 @@ void Launch<Dt, NGL>::subdivide_shape(int stack_cap, int* waves_per_block, size_
     *lds_bytes = (size_t)w * per_wave;
 }
 
+constexpr int kMaxLdsKey = 257;  // LDS sizes in 256-byte units, up to 64 KiB per block
+
 template <typename Dt, int NGL, class Ev>
 static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name) {
     if (A.n_slots <= 0) return hipSuccess;
@@ -26,10 +28,17 @@ static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivAr
     const int threads = 64 * wpb;
     const int ipw = 64 >> A.log2_nsub;
     const int64_t n_tiles = (A.n_slots + ipw - 1) / ipw;
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subdivide<Dt, NGL, Ev>, threads, lds);
-    if (e != hipSuccess) return e;
-    if (per_cu < 1) per_cu = 1;
+    // resident blocks per CU for this LDS size (queried once per size and instantiation)
+    static int cached_per_cu[4][kMaxLdsKey] = {};
+    const int lds_key = (lds & 255) ? kMaxLdsKey : (int)(lds >> 8);  // sizes here are multiples of 512
+    int per_cu = (wpb <= 4 && lds_key < kMaxLdsKey) ? cached_per_cu[wpb - 1][lds_key] : 0;
+    hipError_t e = hipSuccess;
+    if (per_cu == 0) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subdivide<Dt, NGL, Ev>, threads, lds);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) per_cu = 1;
+        if (wpb <= 4 && lds_key < kMaxLdsKey) cached_per_cu[wpb - 1][lds_key] = per_cu;
+    }
     int64_t blocks = (int64_t)c.n_cu * per_cu;
     const int64_t need = (n_tiles + wpb - 1) / wpb;
     if (blocks > need) blocks = need;

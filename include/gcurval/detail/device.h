@@ -69,6 +69,20 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 __device__ __forceinline__ double shfl_up_d(double v, int d) { return __shfl_up(v, (unsigned)d, 64); }
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
+// Lane `src`'s copy of a trivially copyable value, one 32-bit word at a time.
+template <class T>
+__device__ __forceinline__ T shfl_pod(const T& v, int src) {
+    static_assert(sizeof(T) % 4 == 0, "shfl_pod needs a multiple of 4 bytes");
+    constexpr int kWords = (int)(sizeof(T) / 4);
+    int w[kWords];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int j = 0; j < kWords; j++) w[j] = __shfl(w[j], src, 64);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // Inclusive scan over aligned segments of 2^log2seg lanes, in double.
 __device__ __forceinline__ double seg_scan_incl(double v, int log2seg) {
     const int k = lane_id() & ((1 << log2seg) - 1);

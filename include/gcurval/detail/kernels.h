@@ -32,6 +32,13 @@ namespace dev {
 // ---------------------------------------------------------------------------
 enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverflow = 3, kStatTasks = 4,
                 kStatCount = 8 };
+// Build statistics live in kStatSlots rows of kStatStride counters (one
+// 128-byte line each); block b adds into row b % kStatSlots, and readers sum
+// (or take the maximum over) the rows. Spreading the rows keeps thousands of
+// waves from serialising on one address.
+constexpr int kStatSlots = 32;
+constexpr int kStatStride = 16;
+constexpr int kStatWords = kStatSlots * kStatStride;
 
 template <typename Dt>
 struct SubdivArgs {
@@ -175,8 +182,8 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
                 const int r = key_root_of(kk);
                 const int l = key_level_of(kk);
                 const uint32_t idx = key_idx_of(kk);
-                const int ir = __shfl(i, r, 64);
-                const typename Ev::Ctx cr = (ipw_log2 == 0) ? ctx : ev.load(ir);
+                // lane r holds the context of root r's interval
+                const typename Ev::Ctx cr = (ipw_log2 == 0) ? ctx : shfl_pod(ctx, r);
                 Dt a0, am, a1;
                 task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
                 const Dt L2 = quad<NGL>(ev, cr, a0, am);
@@ -228,11 +235,12 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         }
     }
     if (lane == 0) {
-        atomicAdd(&A.stats[kStatQuad], n_quad);
-        atomicAdd(&A.stats[kStatDeepRoots], n_deep);
-        atomicAdd(&A.stats[kStatTasks], n_tasks);
-        atomicMax(&A.stats[kStatMaxStack], (unsigned long long)max_stack);
-        if (overflow) atomicMax(&A.stats[kStatOverflow], 1ull);
+        unsigned long long* st = A.stats + (blockIdx.x & (kStatSlots - 1)) * kStatStride;
+        if (n_quad) atomicAdd(&st[kStatQuad], n_quad);
+        if (n_deep) atomicAdd(&st[kStatDeepRoots], n_deep);
+        if (n_tasks) atomicAdd(&st[kStatTasks], n_tasks);
+        if (max_stack) atomicMax(&st[kStatMaxStack], (unsigned long long)max_stack);
+        if (overflow) atomicMax(&st[kStatOverflow], 1ull);
     }
 }
 

@@ -146,20 +146,49 @@ __device__ __forceinline__ void store_d(f32x4 d, const float* bias, bool relu, u
     }
 }
 
+// Weight operands (BF16 B fragments) and this lane's bias columns, loaded once
+// per wave: W1 4 fragments, W2 4x4, W3 4 (48 VGPRs).
+struct MlpFrags {
+    bf16x4 w1[4], w2[16], w3[4];
+    float b1[4], b2[4], b3;
+};
+__device__ __forceinline__ MlpFrags load_frags(const MlpParams& P) {
+    MlpFrags F;
+    const int col = lane_id() & 15;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        F.w1[nt] = load_b(P.w1b, kHid, 0, nt * 16);
+        F.w3[nt] = load_b(P.w3b, 16, nt * 16, 0);
+        F.b1[nt] = P.p[off_b1() + nt * 16 + col];
+        F.b2[nt] = P.p[off_b2() + nt * 16 + col];
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) F.w2[kt * 4 + nt] = load_b(P.w2b, kHid, kt * 16, nt * 16);
+    }
+    F.b3 = col < kOut ? P.p[off_b3() + col] : 0.0f;
+    return F;
+}
+
+// D tile + bias column, ReLU, to BF16 in LDS T[16][ld] at column n0.
+__device__ __forceinline__ void store_relu(f32x4 d, float bias, uint16_t* T, int ld, int n0) {
+    const int l = lane_id();
+    const int n = n0 + (l & 15);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float v = d[j] + bias;
+        T[(4 * (l >> 4) + j) * ld + n] = f2bf(v > 0.0f ? v : 0.0f);
+    }
+}
+
 // Forward pass of one 16-row tile. X: LDS [16][16] bf16 (standardised
-// features); H1, H2: LDS [16][64] bf16 activations (kept for training);
-// out: [16][2] FP32 (lane-local writes through `out_store`).
-__device__ __forceinline__ void mlp_forward_tile(const MlpParams& P, const uint16_t* X, uint16_t* H1,
-                                                 uint16_t* H2, float* out, int out_ld) {
-    const float* b1 = P.p + off_b1();
-    const float* b2 = P.p + off_b2();
-    const float* b3 = P.p + off_b3();
+// features); H1, H2: LDS [16][64] bf16 scratch; out: [16][out_ld] FP32.
+__device__ __forceinline__ void mlp_forward_tile(const MlpFrags& W, const uint16_t* X, uint16_t* H1, uint16_t* H2,
+                                                 float* out, int out_ld) {
     const bf16x4 ax = load_a(X, kFeat, 0);
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(ax, load_b(P.w1b, kHid, 0, nt * 16), acc);
-        store_d(acc, b1, true, H1, kHid, nt * 16, nullptr, 0);
+        acc = mfma16(ax, W.w1[nt], acc);
+        store_relu(acc, W.b1[nt], H1, kHid, nt * 16);
     }
     wave_lds_sync();
     bf16x4 ah[4];
@@ -169,17 +198,17 @@ __device__ __forceinline__ void mlp_forward_tile(const MlpParams& P, const uint1
     for (int nt = 0; nt < 4; nt++) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) acc = mfma16(ah[kt], load_b(P.w2b, kHid, kt * 16, nt * 16), acc);
-        store_d(acc, b2, true, H2, kHid, nt * 16, nullptr, 0);
+        for (int kt = 0; kt < 4; kt++) acc = mfma16(ah[kt], W.w2[kt * 4 + nt], acc);
+        store_relu(acc, W.b2[nt], H2, kHid, nt * 16);
     }
     wave_lds_sync();
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) acc = mfma16(load_a(H2, kHid, kt * 16), load_b(P.w3b, 16, kt * 16, 0), acc);
+    for (int kt = 0; kt < 4; kt++) acc = mfma16(load_a(H2, kHid, kt * 16), W.w3[kt], acc);
     const int l = lane_id();
     if ((l & 15) < kOut) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) out[(4 * (l >> 4) + j) * out_ld + (l & 15)] = acc[j] + b3[l & 15];
+        for (int j = 0; j < 4; j++) out[(4 * (l >> 4) + j) * out_ld + (l & 15)] = acc[j] + W.b3;
     }
 }
 
@@ -219,7 +248,7 @@ __device__ __forceinline__ void features_common(const float sp[5], const float d
 
 template <int P, int D>
 __device__ void features_bspline(const float* c, double h, int n_int_curve, const double* ctrl, int pstride, int s,
-                                 float tol0, float* f) {
+                                 float tol0, float* f, float* r2_out) {
     float sp[5], dv[5][3];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
@@ -270,22 +299,10 @@ __device__ void features_bspline(const float* c, double h, int n_int_curve, cons
     f[12] = cmin;
     f[13] = log2f((float)(h * n_int_curve) + 1e-12f);
     f[14] = P * 0.2f + D * 0.1f;
-    // 17-sample min / max speed
-    float m2 = 1e30f, x2 = 0.f;
-    for (int q = 0; q <= 16; q++) {
-        const float x = q * (1.0f / 16.0f);
-        float s2 = 0.f;
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            float v = c[d * P + P - 1];
-#pragma unroll
-            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
-            s2 += v * v;
-        }
-        m2 = fminf(m2, s2);
-        x2 = fmaxf(x2, s2);
-    }
-    f[15] = x2 > 0.f ? sqrtf(m2 / x2) : 1.f;
+    // 17-sample min / max speed (the safety-floor quantity)
+    const float r2 = speed2_ratio<P, D>(c);
+    f[15] = sqrtf(r2);
+    *r2_out = r2;
 }
 
 // Ellipse (normalised semi-axes a', b'): dC/dx = 2 pi (-a' sin 2 pi x, b' cos 2 pi x).
@@ -325,15 +342,19 @@ struct FeatArgs {
     const double* ctrl;
     int ctrl_stride;
     const double* coef64;      // normalised power-basis coefficients
+    const float* coef32;       // FP32 copy (FP32 builds), preferred when present
+    int ls_plain;              // log2 n_subdivided (safety floor of the plain root count)
     const double* h;
     float tol0;                // level-0 tolerance (normalised units)
 };
 
-__device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, float* f) {
+// Features of interval i and its squared min / max speed ratio (safety floor).
+__device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, float* f, float* r2) {
     if (F.kind == 0) {
         const double aa = F.a[i], bb = F.b[i];
         const double s = F.normalize ? fmax(fmax(fabs(aa), fabs(bb)), 1e-300) : 1.0;
         features_ellipse(aa / s, bb / s, F.tol0, f);
+        *r2 = ellipse_speed2_ratio(aa, bb);
         return;
     }
     const int c = F.int_curve[i];
@@ -342,11 +363,12 @@ __device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, 
     const int nint = (int)(F.ioff[c + 1] - F.ioff[c]);
     const double* ctrl = F.ctrl + F.ctrl_off[c] * F.ctrl_stride;
     float cf[kCoefStride];
-    for (int j = 0; j < p * D; j++) cf[j] = (float)F.coef64[i * kCoefStride + j];
+    load_coef_f32(F.coef32, F.coef64, i, p * D, cf);
+    *r2 = 1.0f;
     // control-polygon features are ratios, so raw control points serve
-#define GCV_FEAT_CASE(PP, DD)                                                                      \
-    case 10 * PP + DD:                                                                             \
-        features_bspline<PP, DD>(cf, F.h[i], nint, ctrl, F.ctrl_stride, segid + PP, F.tol0, f);   \
+#define GCV_FEAT_CASE(PP, DD)                                                                        \
+    case 10 * PP + DD:                                                                               \
+        features_bspline<PP, DD>(cf, F.h[i], nint, ctrl, F.ctrl_stride, segid + PP, F.tol0, f, r2); \
         break;
     switch (10 * p + D) {
         GCV_FEAT_CASE(1, 2)
@@ -369,8 +391,8 @@ __device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, 
 static __global__ __launch_bounds__(256) void k_features(FeatArgs F, float* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= F.n) return;
-    float f[kFeat];
-    interval_features(F, i, f);
+    float f[kFeat], r2;
+    interval_features(F, i, f, &r2);
 #pragma unroll
     for (int j = 0; j < kFeat; j++) out[i * kFeat + j] = f[j];
 }
@@ -383,7 +405,7 @@ static __global__ __launch_bounds__(256) void k_features(FeatArgs F, float* out)
 constexpr int kWaveLds = (4 * 16 * kFeat + 2 * 16 * kHid) * 2 + 64 * kOut * 4;
 
 static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParams P, int hedge, int8_t* kk,
-                                                            float* raw_out) {
+                                                            int8_t* kmin, float* raw_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;
@@ -394,18 +416,26 @@ static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParam
     float* O = (float*)(H2 + 16 * kHid);                // [64][2]
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    if (gw * 64 >= F.n) return;
+    const MlpFrags Wf = load_frags(P);
+    float mean[kFeat], istd[kFeat];
+#pragma unroll
+    for (int j = 0; j < kFeat; j++) {
+        mean[j] = P.feat_mean[j];
+        istd[j] = P.feat_istd[j];
+    }
     for (int64_t t0 = gw * 64; t0 < F.n; t0 += nw * 64) {
         const int64_t i = t0 + lane;
-        float f[kFeat];
-        if (i < F.n) interval_features(F, i, f);
+        float f[kFeat], r2 = 1.0f;
+        if (i < F.n) interval_features(F, i, f, &r2);
         else
             for (int j = 0; j < kFeat; j++) f[j] = 0.f;
 #pragma unroll
-        for (int j = 0; j < kFeat; j++) X[lane * kFeat + j] = f2bf((f[j] - P.feat_mean[j]) * P.feat_istd[j]);
+        for (int j = 0; j < kFeat; j++) X[lane * kFeat + j] = f2bf((f[j] - mean[j]) * istd[j]);
         wave_lds_sync();
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
-            mlp_forward_tile(P, X + mt * 16 * kFeat, H1, H2, O + mt * 16 * kOut, kOut);
+            mlp_forward_tile(Wf, X + mt * 16 * kFeat, H1, H2, O + mt * 16 * kOut, kOut);
             wave_lds_sync();
         }
         if (i < F.n) {
@@ -413,11 +443,13 @@ static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParam
             int k = (int)rintf(o0);
             if (hedge && o1 < 0.f) k += 1;
             kk[i] = (int8_t)clamp_k(k);
+            if (kmin) kmin[i] = floor_from_ratio(r2, F.ls_plain);
             if (raw_out) {
                 raw_out[i * kOut] = o0;
                 raw_out[i * kOut + 1] = o1;
             }
         }
+        wave_lds_sync();
     }
 }
 
@@ -647,11 +679,12 @@ static __global__ __launch_bounds__(64) void k_mlp_forward(const uint16_t* Xg, i
     __shared__ float O[16 * kOut];
     const int l = lane_id();
     const int64_t ntile = (n + 15) / 16;
+    const MlpFrags Wf = load_frags(P);
     for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const int64_t r0 = tile * 16;
         for (int e = l; e < 16 * kFeat; e += 64) X[e] = (r0 + e / kFeat) < n ? Xg[r0 * kFeat + e] : (uint16_t)0;
         wave_lds_sync();
-        mlp_forward_tile(P, X, H1, H2, O, kOut);
+        mlp_forward_tile(Wf, X, H1, H2, O, kOut);
         wave_lds_sync();
         for (int e = l; e < 16 * kOut; e += 64)
             if (r0 + e / kOut < n) outg[r0 * kOut + e] = O[e];

@@ -50,45 +50,49 @@ constexpr double kFloorPlain = 0.05;
 constexpr double kFloorMid = 0.2;
 constexpr int kFloorMidK = 4;
 
-__device__ __forceinline__ int8_t floor_from_ratio(double r2, int ls_plain) {
-    if (!(r2 >= kFloorPlain * kFloorPlain)) return (int8_t)ls_plain;
-    if (r2 < kFloorMid * kFloorMid) return (int8_t)(kFloorMidK < ls_plain ? kFloorMidK : ls_plain);
+__device__ __forceinline__ int8_t floor_from_ratio(float r2, int ls_plain) {
+    if (!(r2 >= float(kFloorPlain * kFloorPlain))) return (int8_t)ls_plain;
+    if (r2 < float(kFloorMid * kFloorMid)) return (int8_t)(kFloorMidK < ls_plain ? kFloorMidK : ls_plain);
     return 0;
 }
 
+// min / max of |dC/dx|^2 over x = j / 16, j = 0..16 (power-basis coefficients
+// c[d * P + e] of the derivative, FP32).
 template <int P, int D>
-__device__ __forceinline__ double speed2_ratio(const double* c) {
-    double mn = 1e300, mx = 0.0;
+__device__ __forceinline__ float speed2_ratio(const float* c) {
+    float mn = 3.0e38f, mx = 0.0f;
     for (int j = 0; j <= 16; j++) {
-        const double x = j * (1.0 / 16.0);
-        double s2 = 0.0;
+        const float x = j * (1.0f / 16.0f);
+        float s2 = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            double v = c[d * P + P - 1];
+            float v = c[d * P + P - 1];
 #pragma unroll
-            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
-            s2 += v * v;
+            for (int e = P - 2; e >= 0; e--) v = fmaf(v, x, c[d * P + e]);
+            s2 = fmaf(v, v, s2);
         }
-        mn = s2 < mn ? s2 : mn;
-        mx = s2 > mx ? s2 : mx;
+        mn = fminf(mn, s2);
+        mx = fmaxf(mx, s2);
     }
-    return mx > 0.0 ? mn / mx : 1.0;
+    return mx > 0.0f ? mn / mx : 1.0f;
 }
 
-static __global__ __launch_bounds__(256) void k_seed_floor_bspline(int64_t n, const int32_t* int_curve,
-                                                                   const int32_t* degree, const int32_t* dim,
-                                                                   const double* coef64, int ls_plain,
-                                                                   int8_t* kmin) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = int_curve[i];
-    const double* cf = coef64 + i * kCoefStride;
-    double r2 = 1.0;
+// FP32 copy of an interval's derivative coefficients (from the FP32 table when
+// the build keeps one, else rounded from the FP64 table).
+__device__ __forceinline__ void load_coef_f32(const float* coef32, const double* coef64, int64_t i, int n,
+                                              float* cf) {
+    if (coef32) {
+        for (int j = 0; j < n; j++) cf[j] = coef32[i * kCoefStride + j];
+    } else {
+        for (int j = 0; j < n; j++) cf[j] = (float)coef64[i * kCoefStride + j];
+    }
+}
+
+__device__ __forceinline__ float bspline_speed2_ratio(int p, int D, const float* cf) {
 #define GCV_FLOOR_CASE(PP, DD) \
     case 10 * PP + DD:         \
-        r2 = speed2_ratio<PP, DD>(cf); \
-        break;
-    switch (10 * degree[c] + dim[c]) {
+        return speed2_ratio<PP, DD>(cf);
+    switch (10 * p + D) {
         GCV_FLOOR_CASE(1, 2)
         GCV_FLOOR_CASE(1, 3)
         GCV_FLOOR_CASE(2, 2)
@@ -100,70 +104,132 @@ static __global__ __launch_bounds__(256) void k_seed_floor_bspline(int64_t n, co
         GCV_FLOOR_CASE(5, 2)
         GCV_FLOOR_CASE(5, 3)
         default:
-            break;
+            return 1.0f;
     }
 #undef GCV_FLOOR_CASE
-    kmin[i] = floor_from_ratio(r2, ls_plain);
+}
+
+static __global__ __launch_bounds__(256) void k_seed_floor_bspline(int64_t n, const int32_t* int_curve,
+                                                                   const int32_t* degree, const int32_t* dim,
+                                                                   const float* coef32, const double* coef64,
+                                                                   int ls_plain, int8_t* kmin) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = int_curve[i];
+    const int p = degree[c], D = dim[c];
+    float cf[kCoefStride];
+    load_coef_f32(coef32, coef64, i, p * D, cf);
+    kmin[i] = floor_from_ratio(bspline_speed2_ratio(p, D, cf), ls_plain);
 }
 
 // Ellipse: speed^2 = (2 pi)^2 (a^2 sin^2 + b^2 cos^2) spans [min(a,b)^2, max(a,b)^2].
+__device__ __forceinline__ float ellipse_speed2_ratio(double a, double b) {
+    const double aa = fabs(a), bb = fabs(b);
+    const double mn = aa < bb ? aa : bb, mx = aa > bb ? aa : bb;
+    return (float)(mx > 0.0 ? (mn * mn) / (mx * mx) : 1.0);
+}
+
 static __global__ __launch_bounds__(256) void k_seed_floor_ellipse(int64_t n, const double* a, const double* b,
                                                                    int ls_plain, int8_t* kmin) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double aa = fabs(a[i]), bb = fabs(b[i]);
-    const double mn = aa < bb ? aa : bb, mx = aa > bb ? aa : bb;
-    kmin[i] = floor_from_ratio(mx > 0.0 ? (mn * mn) / (mx * mx) : 1.0, ls_plain);
+    kmin[i] = floor_from_ratio(ellipse_speed2_ratio(a[i], b[i]), ls_plain);
 }
 
-// Histogram of bins over intervals into counts[0..n_bins) (zeroed beforehand).
-static __global__ __launch_bounds__(256) void k_bucket_count(int64_t n, BinArgs B, int n_bins,
-                                                             unsigned long long* counts) {
-    __shared__ unsigned int h[kMaxBins];
+// Stable counting sort of intervals by bin (one radix pass). Block j owns
+// the contiguous chunk [j chunk, (j + 1) chunk) of intervals.
+//   k_bucket_hist   per-block bin counts, bin-major: hist[b * nblk + j]
+//   k_bucket_scan   one block: exclusive scan of hist in place, and
+//                   ranges[2b] = first slot, ranges[2b + 1] = count of bin b
+//   k_bucket_place  list[hist[b * nblk + j] + rank] = i, rank = order of i
+//                   among the chunk's intervals of bin b
+// The list is ordered by (bin, interval), so a wave tile takes neighbouring
+// intervals of one curve.
+constexpr int kBucketBlocks = 1024;  // most blocks of the bucketing passes
+
+static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B, int n_bins, int64_t chunk,
+                                                            int32_t* hist) {
+    __shared__ int h[kMaxBins];
     for (int j = threadIdx.x; j < kMaxBins; j += blockDim.x) h[j] = 0;
     __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        atomicAdd(&h[bin_of(B, i)], 1u);
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&h[bin_of(B, i)], 1);
     __syncthreads();
-    for (int j = threadIdx.x; j < n_bins; j += blockDim.x)
-        if (h[j]) atomicAdd(&counts[j], (unsigned long long)h[j]);
+    for (int j = threadIdx.x; j < n_bins; j += blockDim.x) hist[(int64_t)j * gridDim.x + blockIdx.x] = h[j];
 }
 
-// ranges[2b] = offset, ranges[2b+1] = count of bin b; cursor[b] = offset.
-static __global__ void k_bucket_ranges(const unsigned long long* counts, int n_bins, int64_t* ranges,
-                                       unsigned long long* cursor) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    unsigned long long off = 0;
-    for (int b = 0; b < n_bins; b++) {
-        ranges[2 * b] = (int64_t)off;
-        ranges[2 * b + 1] = (int64_t)counts[b];
-        cursor[b] = off;
-        off += counts[b];
+static __global__ __launch_bounds__(1024) void k_bucket_scan(int n_bins, int nblk, int32_t* hist, int64_t* ranges) {
+    __shared__ int32_t s_wave[16];
+    const int m = n_bins * nblk;
+    const int per = (m + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int b0 = (int)threadIdx.x * per;
+    const int b1 = b0 + per < m ? b0 + per : m;
+    int32_t sum = 0;
+    for (int j = b0; j < b1; j++) sum += hist[j];
+    // block exclusive scan of the per-thread sums
+    const int lane = lane_id(), w = (int)threadIdx.x >> 6;
+    int32_t x = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t t = __shfl_up(x, (unsigned)d, 64);
+        if (lane >= d) x += t;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    int32_t run = x - sum, total = 0;
+    for (int j = 0; j < (int)(blockDim.x >> 6); j++) {
+        if (j < w) run += s_wave[j];
+        total += s_wave[j];
+    }
+    for (int j = b0; j < b1; j++) {
+        const int32_t v = hist[j];
+        hist[j] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < n_bins; b += blockDim.x) {
+        const int32_t start = hist[b * nblk];
+        const int32_t end = b + 1 < n_bins ? hist[(b + 1) * nblk] : total;
+        ranges[2 * b] = start;
+        ranges[2 * b + 1] = end - start;
     }
 }
 
-// Scatter interval ids into their bins: one atomic per wave per distinct bin.
-static __global__ __launch_bounds__(256) void k_bucket_scatter(int64_t n, BinArgs B, unsigned long long* cursor,
-                                                               int32_t* list) {
-    const int lane = lane_id();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, BinArgs B, int n_bins, int64_t chunk,
+                                                             const int32_t* hist, int32_t* list) {
+    __shared__ int32_t s_base[kMaxBins];
+    __shared__ int32_t s_wcnt[4][kMaxBins];
+    const int lane = lane_id(), w = (int)threadIdx.x >> 6;
+    for (int j = threadIdx.x; j < n_bins; j += blockDim.x) s_base[j] = hist[(int64_t)j * gridDim.x + blockIdx.x];
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t base = b0; base < b1; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
-        const bool valid = i < n;
+        const bool valid = i < b1;
         const int b = valid ? bin_of(B, i) : 0;
+        for (int j = lane; j < n_bins; j += 64) s_wcnt[w][j] = 0;
+        __syncthreads();
+        // rank within the wave, one ballot per distinct bin
+        int rank = 0;
         uint64_t todo = ballot(valid);
         while (todo) {
             const int leader = __ffsll((unsigned long long)todo) - 1;
             const int bl = __shfl(b, leader, 64);
-            const bool mine = valid && b == bl;
-            const uint64_t m = ballot(mine);
-            unsigned long long pos = 0;
-            if (lane == leader) pos = atomicAdd(&cursor[bl], (unsigned long long)popc64(m));
-            pos = __shfl(pos, leader, 64);
-            if (mine) list[pos + mbcnt(m)] = (int32_t)i;
-            todo &= ~m;
+            const uint64_t mm = ballot(valid && b == bl);
+            if (valid && b == bl) rank = mbcnt(mm);
+            if (lane == leader) s_wcnt[w][bl] = popc64(mm);
+            todo &= ~mm;
         }
+        __syncthreads();
+        if (valid) {
+            int32_t pos = s_base[b] + rank;
+            for (int v = 0; v < w; v++) pos += s_wcnt[v][b];
+            list[pos] = (int32_t)i;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < n_bins; j += blockDim.x)
+            s_base[j] += s_wcnt[0][j] + s_wcnt[1][j] + s_wcnt[2][j] + s_wcnt[3][j];
+        __syncthreads();
     }
 }
 

@@ -188,7 +188,8 @@ int gcv_mlp_destroy(gcv_mlp* m);
 int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* mean, const float* istd);
 int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd);
 /* Raw 16 features of every knot interval of a table (device float
- * [n_intervals][16]); runs the table's setup kernels first. */
+ * [n_intervals][16]); runs the table's setup kernels first (the build
+ * statistics of gcv_table_stats are left as they are). */
 int gcv_mlp_features(gcv_table* t, float* d_out);
 /* Predicted k (device int8 [n_intervals]) and optionally the raw outputs
  * (device float [n_intervals][2]); hedge=1 adds one level where the pass head
