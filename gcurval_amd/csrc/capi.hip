@@ -397,6 +397,7 @@ struct gcv_table {
     // seeded builds: per-interval k, root offsets, (class, k) buckets
     int8_t* seed_k = nullptr;             // [n_int] k used by uniform builds
     int8_t* seed_floor = nullptr;         // [n_int] safety floor of seeded builds
+    uint16_t* feat_x = nullptr;           // [n_int][16] learned-seed input rows (allocated on first use)
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
     int64_t* root_off = nullptr;          // [n_int + 1]
     int32_t* bucket_list = nullptr;       // [n_int]
@@ -636,7 +637,7 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
 // Build phases. Setup: statistics reset, and for B-splines the per-curve
 // scale / interval map and the per-interval derivative polynomials.
 template <typename Dt, int NGL>
-static hipError_t build_setup(gcv_table* t, bool reset_stats = true) {
+static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const dev::FeatSetup* fs = nullptr) {
     gcv_ctx* c = t->ctx;
     if (reset_stats)
         LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3((dev::kStatWords + 255) / 256), dim3(256), 0,
@@ -651,8 +652,15 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true) {
     }
     if (t->n_int > 0) {
         const int64_t blocks = (t->n_int + 255) / 256;
-        LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef, dim3((unsigned)blocks), dim3(256), 0, c->stream, B,
-                         t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32, t->u0, t->h);
+        if (fs) {
+            LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef<true>, dim3((unsigned)blocks), dim3(256), 0,
+                             c->stream, B, t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32,
+                             t->u0, t->h, *fs);
+        } else {
+            LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef<false>, dim3((unsigned)blocks), dim3(256), 0,
+                             c->stream, B, t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32,
+                             t->u0, t->h, dev::FeatSetup{});
+        }
     }
     return hipSuccess;
 }
@@ -793,6 +801,12 @@ struct SeededBuild {
 template <typename Dt, int NGL>
 struct SetupOnly {
     static hipError_t run(gcv_table* t, bool reset_stats) { return build_setup<Dt, NGL>(t, reset_stats); }
+};
+template <typename Dt, int NGL>
+struct SetupWithFeatures {
+    static hipError_t run(gcv_table* t, bool reset_stats, const dev::FeatSetup* fs) {
+        return build_setup<Dt, NGL>(t, reset_stats, fs);
+    }
 };
 template <typename Dt, int NGL>
 struct SeededAfterSetup {
@@ -1105,33 +1119,50 @@ extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
     return GCV_OK;
 }
 
-static hipError_t launch_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, int8_t* d_floor, float* d_raw) {
+// Setup pass plus prediction: B-spline tables get their standardised feature
+// rows and safety floor from the setup kernel and stream them through the
+// network; ellipses compute features inside the prediction kernel.
+static hipError_t setup_and_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw,
+                                    bool reset_stats) {
     gcv_ctx* c = t->ctx;
-    if (t->n_int == 0) return hipSuccess;
-    const int64_t waves = (t->n_int + 63) / 64;
-    const int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8);
-    prof_begin(c, "k_mlp_predict");
-    hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), 4 * dev::kWaveLds, c->stream,
-                       feat_args(t), mlp_dev(m), hedge, d_k, d_floor, d_raw);
-    hipError_t e = hipGetLastError();
+    const int64_t n = t->n_int;
+    const int64_t waves = (n + 63) / 64;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8));
+    hipError_t e;
+    if (t->kind == KIND_BSPLINE) {
+        if (!t->feat_x && n > 0 && talloc(t, &t->feat_x, n * dev::kFeat)) return hipErrorOutOfMemory;
+        const dev::FeatSetup fs{t->feat_x, t->seed_floor, m->mean, m->istd,
+                                (float)(t->o.tolerance / (double)(1 << t->log2_nsub)), t->log2_nsub};
+        e = dispatch_build<SetupWithFeatures>(t, reset_stats, &fs);
+        if (e != hipSuccess || n == 0) return e;
+        prof_begin(c, "k_mlp_predict");
+        hipLaunchKernelGGL(dev::k_mlp_predict_rows, dim3((unsigned)blocks), dim3(256), 0, c->stream,
+                           (const uint16_t*)t->feat_x, n, mlp_dev(m), hedge, d_k, d_raw);
+    } else {
+        e = dispatch_build<SetupOnly>(t, reset_stats);
+        if (e != hipSuccess || n == 0) return e;
+        prof_begin(c, "k_mlp_predict");
+        hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), 4 * dev::kWaveLds, c->stream,
+                           feat_args(t), mlp_dev(m), hedge, d_k, t->seed_floor, d_raw);
+    }
+    e = hipGetLastError();
     prof_end(c);
     return e;
 }
 
 extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
     if (!m || !t || (!d_k && t->n_int > 0)) return GCV_ERR_INVALID;
-    gcv_ctx* c = t->ctx;
-    hipError_t e = dispatch_build<SetupOnly>(t, false);
-    if (e == hipSuccess) e = launch_predict(m, t, hedge, d_k, nullptr, d_raw);
-    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "predict launch failed: %s", hipGetErrorString(e));
+    if (m->ctx != t->ctx) return set_err(t->ctx, GCV_ERR_INVALID, "predictor and table belong to different contexts");
+    hipError_t e = setup_and_predict(m, t, hedge, d_k, d_raw, false);
+    if (e != hipSuccess) return set_err(t->ctx, GCV_ERR_HIP, "predict launch failed: %s", hipGetErrorString(e));
     return GCV_OK;
 }
 
 extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost) {
     if (!t || !m) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
-    hipError_t e = dispatch_build<SetupOnly>(t, true);
-    if (e == hipSuccess) e = launch_predict(m, t, hedge, t->seed_k, t->seed_floor, nullptr);
+    if (m->ctx != c) return set_err(c, GCV_ERR_INVALID, "predictor and table belong to different contexts");
+    hipError_t e = setup_and_predict(m, t, hedge, t->seed_k, nullptr, true);
     if (e == hipSuccess) e = dispatch_build<SeededAfterSetup>(t, (const int8_t*)t->seed_k, d_cost);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "learned-seed build failed: %s", hipGetErrorString(e));
     t->built = true;

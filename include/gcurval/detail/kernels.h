@@ -23,6 +23,7 @@
 
 #include "device.h"
 #include "curves.h"
+#include "features.h"
 
 namespace gcurval {
 namespace dev {
@@ -327,37 +328,46 @@ static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B
 }
 
 // One interval's coefficient block; p * D used slots (the rest of the
-// 16-slot block is never read).
-template <int P, int D>
-__device__ __forceinline__ void coef_one(int s, const double* T, const double* Pc, int pstride, double isc,
-                                         double* c64, float* c32, double* u0, double* h) {
+// 16-slot block is never read). With kFeatures, also the learned seed's
+// feature row and safety floor of the interval.
+template <int P, int D, bool kFeatures>
+__device__ __forceinline__ void coef_one(int64_t i, int s, const double* T, const double* Pc, int pstride,
+                                         double isc, double* c64, float* c32, double* u0, double* h,
+                                         const FeatSetup& F, int nint) {
     double out[P * D];
     double uu, hh;
     span_poly<P, D>(s, T, Pc, pstride, isc, out, &uu, &hh);
     *u0 = uu;
     *h = hh;
+    float cf[P * D];
 #pragma unroll
     for (int j = 0; j < P * D; j++) {
         c64[j] = out[j];
-        if (c32) c32[j] = (float)out[j];
+        cf[j] = (float)out[j];
+        if (c32) c32[j] = cf[j];
     }
+    if (kFeatures) setup_features<P, D>(F, i, cf, hh, nint, Pc, pstride, s);
 }
 
-static __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int64_t* ioff, int64_t n_int,
+template <bool kFeatures>
+__global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int64_t* ioff, int64_t n_int,
                                                       const int32_t* int_curve, const double* int_scale,
-                                                      double* coef64, float* coef32, double* u0, double* h) {
+                                                      double* coef64, float* coef32, double* u0, double* h,
+                                                      FeatSetup F) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_int) return;
     const int c = int_curve[i];
     const int p = B.degree[c], D = B.dim[c];
-    const int segid = (int)(i - ioff[c]);
+    const int64_t i0 = ioff[c];
+    const int segid = (int)(i - i0);
+    const int nint = kFeatures ? (int)(ioff[c + 1] - i0) : 0;
     const double* T = B.knots + B.knot_off[c];
     const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
     const double isc = 1.0 / int_scale[i];
-#define GCV_COEF_CASE(PP, DD)                                                                   \
-    case 10 * PP + DD:                                                                          \
-        coef_one<PP, DD>(segid + PP, T, Pc, B.ctrl_stride, isc, coef64 + i * kCoefStride,     \
-                         coef32 ? coef32 + i * kCoefStride : nullptr, &u0[i], &h[i]);          \
+#define GCV_COEF_CASE(PP, DD)                                                                              \
+    case 10 * PP + DD:                                                                                     \
+        coef_one<PP, DD, kFeatures>(i, segid + PP, T, Pc, B.ctrl_stride, isc, coef64 + i * kCoefStride,    \
+                                    coef32 ? coef32 + i * kCoefStride : nullptr, &u0[i], &h[i], F, nint); \
         break;
     switch (10 * p + D) {
         GCV_COEF_CASE(1, 2)

@@ -18,31 +18,17 @@
 #include "device.h"
 #include "curves.h"
 #include "seeded.h"
+#include "features.h"
 
 namespace gcurval {
 namespace dev {
 
-constexpr int kFeat = 16;
 constexpr int kHid = 64;
 constexpr int kOut = 2;
 constexpr int kParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut;  // 5378
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// float -> bf16 bits, round to nearest even (finite inputs)
-__host__ __device__ inline uint16_t f2bf(float f) {
-    uint32_t u;
-    __builtin_memcpy(&u, &f, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__host__ __device__ inline float bf2f(uint16_t b) {
-    uint32_t u = (uint32_t)b << 16;
-    float f;
-    __builtin_memcpy(&f, &u, 4);
-    return f;
-}
 
 // Parameter block (FP32 master copy + BF16 operands), offsets into one array.
 struct MlpParams {
@@ -146,185 +132,91 @@ __device__ __forceinline__ void store_d(f32x4 d, const float* bias, bool relu, u
     }
 }
 
-// Weight operands (BF16 B fragments) and this lane's bias columns, loaded once
-// per wave: W1 4 fragments, W2 4x4, W3 4 (48 VGPRs).
+// Inference runs the network transposed, H^T = W^T X^T, with 16 rows as the
+// N dimension of each product: the accumulator of one layer (lane l holds
+// hidden units 4 (l / 16) + j of row l % 16) is then exactly the B operand of
+// the next, so activations never leave registers, and the input operand is 8
+// consecutive bytes of a row-major feature row. The weight A operands
+// W^T[m][k] = W[k][m] are loaded once per wave (48 VGPRs).
 struct MlpFrags {
     bf16x4 w1[4], w2[16], w3[4];
-    float b1[4], b2[4], b3;
+    f32x4 b1[4], b2[4];
+    float b3[kOut];
 };
 __device__ __forceinline__ MlpFrags load_frags(const MlpParams& P) {
     MlpFrags F;
-    const int col = lane_id() & 15;
+    const int h0 = 4 * (lane_id() >> 4);
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        F.w1[nt] = load_b(P.w1b, kHid, 0, nt * 16);
-        F.w3[nt] = load_b(P.w3b, 16, nt * 16, 0);
-        F.b1[nt] = P.p[off_b1() + nt * 16 + col];
-        F.b2[nt] = P.p[off_b2() + nt * 16 + col];
+    for (int mt = 0; mt < 4; mt++) {
+        F.w1[mt] = load_b(P.w1b, kHid, 0, mt * 16);
+        F.w3[mt] = load_b(P.w3b, 16, mt * 16, 0);
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) F.w2[kt * 4 + nt] = load_b(P.w2b, kHid, kt * 16, nt * 16);
+        for (int kt = 0; kt < 4; kt++) F.w2[kt * 4 + mt] = load_b(P.w2b, kHid, kt * 16, mt * 16);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            F.b1[mt][j] = P.p[off_b1() + mt * 16 + h0 + j];
+            F.b2[mt][j] = P.p[off_b2() + mt * 16 + h0 + j];
+        }
     }
-    F.b3 = col < kOut ? P.p[off_b3() + col] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < kOut; j++) F.b3[j] = P.p[off_b3() + j];
     return F;
 }
 
-// D tile + bias column, ReLU, to BF16 in LDS T[16][ld] at column n0.
-__device__ __forceinline__ void store_relu(f32x4 d, float bias, uint16_t* T, int ld, int n0) {
-    const int l = lane_id();
-    const int n = n0 + (l & 15);
+__device__ __forceinline__ bf16x4 bias_relu_bf16(f32x4 d, f32x4 b) {
+    bf16x4 v;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const float v = d[j] + bias;
-        T[(4 * (l >> 4) + j) * ld + n] = f2bf(v > 0.0f ? v : 0.0f);
+        const float x = d[j] + b[j];
+        v[j] = (short)f2bf(x > 0.0f ? x : 0.0f);
     }
+    return v;
 }
 
-// Forward pass of one 16-row tile. X: LDS [16][16] bf16 (standardised
-// features); H1, H2: LDS [16][64] bf16 scratch; out: [16][out_ld] FP32.
-__device__ __forceinline__ void mlp_forward_tile(const MlpFrags& W, const uint16_t* X, uint16_t* H1, uint16_t* H2,
-                                                 float* out, int out_ld) {
-    const bf16x4 ax = load_a(X, kFeat, 0);
+// Forward pass of 16 rows: x is the B operand X^T (lane l: row l % 16,
+// features 4 (l / 16) .. +3). Lanes 0..15 receive outputs 0 and 1 of row l.
+__device__ __forceinline__ void mlp_forward16(const MlpFrags& W, bf16x4 x, float& o0, float& o1) {
+    bf16x4 a1[4], a2[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma16(ax, W.w1[nt], acc);
-        store_relu(acc, W.b1[nt], H1, kHid, nt * 16);
+    for (int mt = 0; mt < 4; mt++) {
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        a1[mt] = bias_relu_bf16(mfma16(W.w1[mt], x, z), W.b1[mt]);
     }
-    wave_lds_sync();
-    bf16x4 ah[4];
+    f32x4 h2[4];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) ah[kt] = load_a(H1, kHid, kt * 16);
+    for (int mt = 0; mt < 4; mt++) h2[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < 4; kt++)
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) acc = mfma16(ah[kt], W.w2[kt * 4 + nt], acc);
-        store_relu(acc, W.b2[nt], H2, kHid, nt * 16);
-    }
-    wave_lds_sync();
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int mt = 0; mt < 4; mt++) h2[mt] = mfma16(W.w2[kt * 4 + mt], a1[kt], h2[mt]);
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) acc = mfma16(load_a(H2, kHid, kt * 16), W.w3[kt], acc);
+    for (int mt = 0; mt < 4; mt++) a2[mt] = bias_relu_bf16(h2[mt], W.b2[mt]);
+    f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) o = mfma16(W.w3[kt], a2[kt], o);
+    o0 = o[0] + W.b3[0];
+    o1 = o[1] + W.b3[1];
+}
+
+// B operand X^T from a row-major BF16 row block (rows r0.., 16 features).
+__device__ __forceinline__ bf16x4 load_xt(const uint16_t* rows, int64_t r0, int64_t n) {
     const int l = lane_id();
-    if ((l & 15) < kOut) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) out[(4 * (l >> 4) + j) * out_ld + (l & 15)] = acc[j] + W.b3;
+    const int64_t r = r0 + (l & 15);
+    bf16x4 v = {0, 0, 0, 0};
+    if (r < n) {
+        const uint2 q = *(const uint2*)(rows + r * kFeat + 4 * (l >> 4));
+        v[0] = (short)(q.x & 0xFFFFu);
+        v[1] = (short)(q.x >> 16);
+        v[2] = (short)(q.y & 0xFFFFu);
+        v[3] = (short)(q.y >> 16);
     }
+    return v;
 }
 
-// ---------------------------------------------------------------------------
-// Per-interval features (16, FP32, before standardisation), all in the
-// normalised units the error test sees:
-//   0-4   log2 |dC/dx| at x = 0, 1/4, 1/2, 3/4, 1
-//   5     log2 of max / min of those speeds
-//   6-8   cosines between dC/dx at (0, 1), (0, 1/2), (1/2, 1)
-//   9     log2 (level-0 tolerance / Simpson length estimate)
-//   10    log2 (|leading coefficient| / length estimate)
-//   11    log2 of longest / shortest edge of the span's control polygon
-//   12    smallest cosine between consecutive control-polygon edges
-//   13    log2 (knot span x intervals of the curve)
-//   14    class code (degree / 5 + dim / 10; 0 for ellipses)
-//   15    min / max speed over 17 samples (the safety-floor quantity)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void features_common(const float sp[5], const float dv[5][3], float tol0, float* f) {
-    float mn = sp[0], mx = sp[0];
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        f[q] = log2f(sp[q] + 1e-12f);
-        mn = fminf(mn, sp[q]);
-        mx = fmaxf(mx, sp[q]);
-    }
-    f[5] = log2f((mx + 1e-12f) / (mn + 1e-12f));
-    const int pa[3] = {0, 0, 2}, pb[3] = {4, 2, 4};
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        const int a = pa[t], b = pb[t];
-        const float dot = dv[a][0] * dv[b][0] + dv[a][1] * dv[b][1] + dv[a][2] * dv[b][2];
-        f[6 + t] = dot / (sp[a] * sp[b] + 1e-20f);
-    }
-    const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
-    f[9] = log2f(tol0 / Lest);
-}
-
-template <int P, int D>
-__device__ void features_bspline(const float* c, double h, int n_int_curve, const double* ctrl, int pstride, int s,
-                                 float tol0, float* f, float* r2_out) {
-    float sp[5], dv[5][3];
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        const float x = 0.25f * q;
-        float s2 = 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; d++) dv[q][d] = 0.f;
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            float v = c[d * P + P - 1];
-#pragma unroll
-            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
-            dv[q][d] = v;
-            s2 += v * v;
-        }
-        sp[q] = sqrtf(s2);
-    }
-    features_common(sp, dv, tol0, f);
-    const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
-    float lead = 0.f;
-#pragma unroll
-    for (int d = 0; d < D; d++) lead += c[d * P + P - 1] * c[d * P + P - 1];
-    f[10] = P > 1 ? log2f(sqrtf(lead) / Lest + 1e-12f) : -40.f;
-    // the span's control polygon P_{s-p} .. P_s
-    float emin = 1e30f, emax = 0.f, cmin = 1.f;
-    float pe[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int e = 0; e < P; e++) {
-        float ev[3] = {0.f, 0.f, 0.f}, el = 0.f;
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            ev[d] = (float)(ctrl[(s - P + e + 1) * pstride + d] - ctrl[(s - P + e) * pstride + d]);
-            el += ev[d] * ev[d];
-        }
-        el = sqrtf(el);
-        emin = fminf(emin, el);
-        emax = fmaxf(emax, el);
-        if (e > 0) {
-            const float pl = sqrtf(pe[0] * pe[0] + pe[1] * pe[1] + pe[2] * pe[2]);
-            const float dot = pe[0] * ev[0] + pe[1] * ev[1] + pe[2] * ev[2];
-            cmin = fminf(cmin, dot / (pl * el + 1e-20f));
-        }
-        pe[0] = ev[0];
-        pe[1] = ev[1];
-        pe[2] = ev[2];
-    }
-    f[11] = log2f((emax + 1e-12f) / (emin + 1e-12f));
-    f[12] = cmin;
-    f[13] = log2f((float)(h * n_int_curve) + 1e-12f);
-    f[14] = P * 0.2f + D * 0.1f;
-    // 17-sample min / max speed (the safety-floor quantity)
-    const float r2 = speed2_ratio<P, D>(c);
-    f[15] = sqrtf(r2);
-    *r2_out = r2;
-}
-
-// Ellipse (normalised semi-axes a', b'): dC/dx = 2 pi (-a' sin 2 pi x, b' cos 2 pi x).
-__device__ void features_ellipse(double a, double b, float tol0, float* f) {
-    const float A = (float)(kTwoPi * a), B = (float)(kTwoPi * b);
-    float sp[5], dv[5][3];
-    const float sn[5] = {0.f, 1.f, 0.f, -1.f, 0.f}, cs[5] = {1.f, 0.f, -1.f, 0.f, 1.f};
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        dv[q][0] = -A * sn[q];
-        dv[q][1] = B * cs[q];
-        dv[q][2] = 0.f;
-        sp[q] = sqrtf(dv[q][0] * dv[q][0] + dv[q][1] * dv[q][1]);
-    }
-    features_common(sp, dv, tol0, f);
-    const float lo = fminf(fabsf(A), fabsf(B)), hi = fmaxf(fabsf(A), fabsf(B));
-    f[10] = log2f((hi + 1e-12f) / (lo + 1e-12f));
-    f[11] = 0.f;
-    f[12] = 1.f;
-    f[13] = 0.f;
-    f[14] = 0.f;
-    f[15] = hi > 0.f ? lo / hi : 1.f;
+__device__ __forceinline__ int8_t k_from_outputs(float o0, float o1, int hedge) {
+    int k = (int)rintf(o0);
+    if (hedge && o1 < 0.f) k += 1;
+    return (int8_t)clamp_k(k);
 }
 
 struct FeatArgs {
@@ -399,21 +291,20 @@ static __global__ __launch_bounds__(256) void k_features(FeatArgs F, float* out)
 
 // ---------------------------------------------------------------------------
 // Inference: features -> standardise -> MLP -> k = clamp(rint(out0)) (+1 when
-// the pass head predicts a level-0 failure and hedge is set). One wave per 64
-// intervals; LDS per wave: X 4x[16][16], H1 [16][64], H2 [16][64] (bf16).
+// the pass head predicts a level-0 failure and hedge is set).
+//   k_mlp_predict       features computed here (ellipse tables); one lane per
+//                       interval, rows staged once through LDS
+//   k_mlp_predict_rows  rows precomputed by the B-spline setup pass
 // ---------------------------------------------------------------------------
-constexpr int kWaveLds = (4 * 16 * kFeat + 2 * 16 * kHid) * 2 + 64 * kOut * 4;
+constexpr int kRowLd = kFeat + 4;  // LDS row stride (elements): 40 bytes, conflict-free 8-byte reads
+constexpr int kWaveLds = 64 * kRowLd * 2;
 
 static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParams P, int hedge, int8_t* kk,
                                                             int8_t* kmin, float* raw_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;
-    unsigned char* base = smem + (size_t)w * kWaveLds;
-    uint16_t* X = (uint16_t*)base;                      // [64][16]
-    uint16_t* H1 = X + 64 * kFeat;                      // [16][64]
-    uint16_t* H2 = H1 + 16 * kHid;                      // [16][64]
-    float* O = (float*)(H2 + 16 * kHid);                // [64][2]
+    uint16_t* X = (uint16_t*)(smem + (size_t)w * kWaveLds);  // [64][kRowLd]
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
     if (gw * 64 >= F.n) return;
@@ -430,26 +321,57 @@ static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParam
         if (i < F.n) interval_features(F, i, f, &r2);
         else
             for (int j = 0; j < kFeat; j++) f[j] = 0.f;
+        if (i < F.n && kmin) kmin[i] = floor_from_ratio(r2, F.ls_plain);
 #pragma unroll
-        for (int j = 0; j < kFeat; j++) X[lane * kFeat + j] = f2bf((f[j] - mean[j]) * istd[j]);
+        for (int j = 0; j < kFeat; j++) X[lane * kRowLd + j] = f2bf((f[j] - mean[j]) * istd[j]);
         wave_lds_sync();
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            mlp_forward_tile(Wf, X + mt * 16 * kFeat, H1, H2, O + mt * 16 * kOut, kOut);
-            wave_lds_sync();
-        }
-        if (i < F.n) {
-            const float o0 = O[lane * kOut], o1 = O[lane * kOut + 1];
-            int k = (int)rintf(o0);
-            if (hedge && o1 < 0.f) k += 1;
-            kk[i] = (int8_t)clamp_k(k);
-            if (kmin) kmin[i] = floor_from_ratio(r2, F.ls_plain);
-            if (raw_out) {
-                raw_out[i * kOut] = o0;
-                raw_out[i * kOut + 1] = o1;
+        for (int nt = 0; nt < 4; nt++) {
+            const uint2 q = *(const uint2*)(X + (nt * 16 + (lane & 15)) * kRowLd + 4 * (lane >> 4));
+            bf16x4 x;
+            x[0] = (short)(q.x & 0xFFFFu);
+            x[1] = (short)(q.x >> 16);
+            x[2] = (short)(q.y & 0xFFFFu);
+            x[3] = (short)(q.y >> 16);
+            float o0, o1;
+            mlp_forward16(Wf, x, o0, o1);
+            const int64_t r = t0 + nt * 16 + lane;
+            if (lane < 16 && r < F.n) {
+                kk[r] = k_from_outputs(o0, o1, hedge);
+                if (raw_out) {
+                    raw_out[r * kOut] = o0;
+                    raw_out[r * kOut + 1] = o1;
+                }
             }
         }
         wave_lds_sync();
+    }
+}
+
+static __global__ __launch_bounds__(256) void k_mlp_predict_rows(const uint16_t* xb, int64_t n, MlpParams P,
+                                                                 int hedge, int8_t* kk, float* raw_out) {
+    const int lane = lane_id();
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    if (gw * 64 >= n) return;
+    const MlpFrags Wf = load_frags(P);
+    for (int64_t t0 = gw * 64; t0 < n; t0 += nw * 64) {
+        bf16x4 x[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) x[nt] = load_xt(xb, t0 + nt * 16, n);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            float o0, o1;
+            mlp_forward16(Wf, x[nt], o0, o1);
+            const int64_t r = t0 + nt * 16 + lane;
+            if (lane < 16 && r < n) {
+                kk[r] = k_from_outputs(o0, o1, hedge);
+                if (raw_out) {
+                    raw_out[r * kOut] = o0;
+                    raw_out[r * kOut + 1] = o1;
+                }
+            }
+        }
     }
 }
 
@@ -673,22 +595,17 @@ static __global__ void k_mlp_adam(MlpParams P, float* m, float* v, const float* 
 
 // Forward only over n standardised bf16 rows (tests and evaluation): out[n][2].
 static __global__ __launch_bounds__(64) void k_mlp_forward(const uint16_t* Xg, int64_t n, MlpParams P, float* outg) {
-    __shared__ uint16_t X[16 * kFeat];
-    __shared__ uint16_t H1[16 * kHid];
-    __shared__ uint16_t H2[16 * kHid];
-    __shared__ float O[16 * kOut];
     const int l = lane_id();
     const int64_t ntile = (n + 15) / 16;
     const MlpFrags Wf = load_frags(P);
     for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const int64_t r0 = tile * 16;
-        for (int e = l; e < 16 * kFeat; e += 64) X[e] = (r0 + e / kFeat) < n ? Xg[r0 * kFeat + e] : (uint16_t)0;
-        wave_lds_sync();
-        mlp_forward_tile(Wf, X, H1, H2, O, kOut);
-        wave_lds_sync();
-        for (int e = l; e < 16 * kOut; e += 64)
-            if (r0 + e / kOut < n) outg[r0 * kOut + e] = O[e];
-        wave_lds_sync();
+        float o0, o1;
+        mlp_forward16(Wf, load_xt(Xg, r0, n), o0, o1);
+        if (l < 16 && r0 + l < n) {
+            outg[(r0 + l) * kOut] = o0;
+            outg[(r0 + l) * kOut + 1] = o1;
+        }
     }
 }
 
