@@ -388,7 +388,8 @@ struct gcv_table {
     double* curve_scale = nullptr;
     double* int_scale = nullptr;
     int32_t* int_curve = nullptr;
-    double* coef64 = nullptr;
+    double* coef64 = nullptr;             // FP64 builds: [n_int * cstride]
+    int cstride = 1;                      // coefficient values per interval (max p * D over the batch)
     float* coef32 = nullptr;
     double* u0 = nullptr;
     double* h = nullptr;
@@ -558,12 +559,16 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
     if ((rc = table_alloc_common(t))) { table_free(t); return rc; }
     if ((rc = gcv_copy_h2d(c, t->ioff, ioff.data(), sizeof(int64_t) * (n + 1)))) { table_free(t); return rc; }
     if ((rc = talloc(t, &t->curve_scale, n)) || (rc = talloc(t, &t->int_scale, t->n_int)) ||
-        (rc = talloc(t, &t->int_curve, t->n_int)) || (rc = talloc(t, &t->coef64, t->n_int * dev::kCoefStride)) ||
-        (rc = talloc(t, &t->u0, t->n_int)) || (rc = talloc(t, &t->h, t->n_int))) {
+        (rc = talloc(t, &t->int_curve, t->n_int)) || (rc = talloc(t, &t->u0, t->n_int)) ||
+        (rc = talloc(t, &t->h, t->n_int))) {
         table_free(t);
         return rc;
     }
-    if (t->o.precision == GCV_FP32 && (rc = talloc(t, &t->coef32, t->n_int * dev::kCoefStride))) {
+    for (auto& kv : by_class) t->cstride = std::max(t->cstride, (kv.first / 10) * (kv.first % 10));
+    // FP32 builds keep FP32 coefficients only (the sampler's FP64 step evaluates them in FP64)
+    rc = t->o.precision == GCV_FP32 ? talloc(t, &t->coef32, t->n_int * t->cstride)
+                                    : talloc(t, &t->coef64, t->n_int * t->cstride);
+    if (rc) {
         table_free(t);
         return rc;
     }
@@ -622,7 +627,7 @@ static dev::SubdivArgs<Dt> subdiv_args(gcv_table* t) {
 }
 
 static BSplineTableDev bspline_dev(gcv_table* t) {
-    return BSplineTableDev{t->coef64, t->coef32, t->int_scale, t->u0, t->h};
+    return BSplineTableDev{t->coef64, t->coef32, t->cstride, t->int_scale, t->u0, t->h};
 }
 
 #define LAUNCH_OR_RETURN(name, ...)            \
@@ -650,17 +655,37 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const dev::
         LAUNCH_OR_RETURN("k_bspline_curve_setup", dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0,
                          c->stream, B, t->ioff, t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
     }
-    if (t->n_int > 0) {
-        const int64_t blocks = (t->n_int + 255) / 256;
-        if (fs) {
-            LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef<true>, dim3((unsigned)blocks), dim3(256), 0,
-                             c->stream, B, t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32,
-                             t->u0, t->h, *fs);
-        } else {
-            LAUNCH_OR_RETURN("k_bspline_coef", dev::k_bspline_coef<false>, dim3((unsigned)blocks), dim3(256), 0,
-                             c->stream, B, t->ioff, t->n_int, t->int_curve, t->int_scale, t->coef64, t->coef32,
-                             t->u0, t->h, dev::FeatSetup{});
+    const dev::FeatSetup F = fs ? *fs : dev::FeatSetup{};
+    for (const ClassRange& r : t->classes) {
+        if (r.count == 0) continue;
+        const int32_t* list = t->class_list ? t->class_list + r.offset : nullptr;
+        const unsigned blocks = (unsigned)((r.count + 255) / 256);
+#define GCV_COEF_CASE(PP, DD)                                                                                 \
+    case 10 * PP + DD:                                                                                        \
+        if (fs)                                                                                               \
+            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD, true>), dim3(blocks), dim3(256),  \
+                             0, c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,  \
+                             t->coef32, t->cstride, t->u0, t->h, F);                                                      \
+        else                                                                                                  \
+            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD, false>), dim3(blocks), dim3(256), \
+                             0, c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,  \
+                             t->coef32, t->cstride, t->u0, t->h, F);                                                      \
+        break;
+        switch (bspline_class(r.p, r.D)) {
+            GCV_COEF_CASE(1, 2)
+            GCV_COEF_CASE(1, 3)
+            GCV_COEF_CASE(2, 2)
+            GCV_COEF_CASE(2, 3)
+            GCV_COEF_CASE(3, 2)
+            GCV_COEF_CASE(3, 3)
+            GCV_COEF_CASE(4, 2)
+            GCV_COEF_CASE(4, 3)
+            GCV_COEF_CASE(5, 2)
+            GCV_COEF_CASE(5, 3)
+            default:
+                return hipErrorInvalidValue;
         }
+#undef GCV_COEF_CASE
     }
     return hipSuccess;
 }
@@ -737,7 +762,7 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
                              t->b, t->log2_nsub, t->seed_floor);
         } else {
             LAUNCH_OR_RETURN("k_seed_floor", dev::k_seed_floor_bspline, dim3(g1), dim3(256), 0, c->stream, n,
-                             t->int_curve, t->bat.degree, t->bat.dim, t->coef32, t->coef64, t->log2_nsub,
+                             t->int_curve, t->bat.degree, t->bat.dim, t->coef32, t->coef64, t->cstride, t->log2_nsub,
                              t->seed_floor);
         }
         dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve};
@@ -1098,6 +1123,7 @@ static dev::FeatArgs feat_args(gcv_table* t) {
     F.ctrl_stride = t->bat.ctrl_stride;
     F.coef64 = t->coef64;
     F.coef32 = t->coef32;
+    F.cstride = t->cstride;
     F.ls_plain = t->log2_nsub;
     F.h = t->h;
     F.tol0 = (float)(t->o.tolerance / (double)(1 << t->log2_nsub));

@@ -26,8 +26,9 @@ void prof_end(void* owner);
 
 // Device arrays of a B-spline table needed by the evaluators.
 struct BSplineTableDev {
-    const double* coef64;
-    const float* coef32;
+    const double* coef64;  // FP64 builds
+    const float* coef32;   // FP32 builds
+    int cstride;           // values per interval block
     const double* int_scale;
     const double* u0;
     const double* h;

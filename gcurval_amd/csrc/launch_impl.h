@@ -61,13 +61,29 @@ struct MakeEv;
 template <int P, int D>
 struct MakeEv<float, P, D> {
     __host__ __device__ static BSplineEval<float, P, D> get(const BSplineTableDev& T) {
-        return BSplineEval<float, P, D>{T.coef32, T.int_scale, T.u0, T.h};
+        return BSplineEval<float, P, D>{T.coef32, T.cstride, T.int_scale, T.u0, T.h};
     }
 };
 template <int P, int D>
 struct MakeEv<double, P, D> {
     __host__ __device__ static BSplineEval<double, P, D> get(const BSplineTableDev& T) {
-        return BSplineEval<double, P, D>{T.coef64, T.int_scale, T.u0, T.h};
+        return BSplineEval<double, P, D>{T.coef64, T.cstride, T.int_scale, T.u0, T.h};
+    }
+};
+// FP64 evaluator of the sampler's final Newton step: an FP32 build keeps only
+// FP32 coefficients and evaluates them in FP64 arithmetic.
+template <typename Dt, int P, int D>
+struct MakeEv64;
+template <int P, int D>
+struct MakeEv64<float, P, D> {
+    __host__ __device__ static BSplineEval<double, P, D, float> get(const BSplineTableDev& T) {
+        return BSplineEval<double, P, D, float>{T.coef32, T.cstride, T.int_scale, T.u0, T.h};
+    }
+};
+template <int P, int D>
+struct MakeEv64<double, P, D> {
+    __host__ __device__ static BSplineEval<double, P, D> get(const BSplineTableDev& T) {
+        return MakeEv<double, P, D>::get(T);
     }
 };
 
@@ -122,7 +138,7 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
     double u = 0.0;
 #define GCV_SAMPLE_CASE(PP, DD)                                                                     \
     case 10 * PP + DD:                                                                             \
-        u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv<double, PP, DD>::get(T), A, c, g, \
+        u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv64<Dt, PP, DD>::get(T), A, c, g,   \
                                 &seg);                                                             \
         break;
     switch (cls) {

@@ -224,11 +224,13 @@ struct EllipseEval {
 
 // ---------------------------------------------------------------------------
 // B-spline evaluator for one (degree P, dimension D) class; coefficient
-// blocks are precomputed per interval (kCoefStride values each).
+// blocks are precomputed per interval (cstride values each, stored as Ct and
+// evaluated in Dt).
 // ---------------------------------------------------------------------------
-template <typename Dt, int P, int D>
+template <typename Dt, int P, int D, typename Ct = Dt>
 struct BSplineEval {
-    const Dt* coef;           // [n_int * kCoefStride]
+    const Ct* coef;           // [n_int * cstride]
+    int cstride;              // values per interval block (>= P * D)
     const double* int_scale;  // per interval normalisation scale (may be null -> 1)
     const double* u0;         // per interval knot T[s]
     const double* h;          // per interval span width
@@ -240,9 +242,9 @@ struct BSplineEval {
     __device__ __forceinline__ double scale(int i) const { return int_scale ? int_scale[i] : 1.0; }
     __device__ __forceinline__ Ctx load(int i) const {
         Ctx k;
-        const Dt* src = coef + (int64_t)i * kCoefStride;
+        const Ct* src = coef + (int64_t)i * cstride;
 #pragma unroll
-        for (int j = 0; j < P * D; j++) k.c[j] = src[j];
+        for (int j = 0; j < P * D; j++) k.c[j] = Dt(src[j]);
         return k;
     }
     // coefficients are stored pre-normalised

@@ -327,8 +327,8 @@ static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B
     }
 }
 
-// One interval's coefficient block; p * D used slots (the rest of the
-// 16-slot block is never read). With kFeatures, also the learned seed's
+// One interval's coefficient block (P * D values, into whichever of the FP64
+// and FP32 tables the build keeps). With kFeatures, also the learned seed's
 // feature row and safety floor of the interval.
 template <int P, int D, bool kFeatures>
 __device__ __forceinline__ void coef_one(int64_t i, int s, const double* T, const double* Pc, int pstride,
@@ -342,48 +342,33 @@ __device__ __forceinline__ void coef_one(int64_t i, int s, const double* T, cons
     float cf[P * D];
 #pragma unroll
     for (int j = 0; j < P * D; j++) {
-        c64[j] = out[j];
         cf[j] = (float)out[j];
+        if (c64) c64[j] = out[j];
         if (c32) c32[j] = cf[j];
     }
     if (kFeatures) setup_features<P, D>(F, i, cf, hh, nint, Pc, pstride, s);
 }
 
-template <bool kFeatures>
-__global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int64_t* ioff, int64_t n_int,
-                                                      const int32_t* int_curve, const double* int_scale,
-                                                      double* coef64, float* coef32, double* u0, double* h,
-                                                      FeatSetup F) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_int) return;
+// Coefficient pass of one evaluator class (degree P, dimension D), one thread
+// per interval; `list` holds the class's intervals in a mixed batch (null:
+// every interval is of this class).
+template <int P, int D, bool kFeatures>
+__global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int32_t* list, int64_t n_slots,
+                                                      const int64_t* ioff, const int32_t* int_curve,
+                                                      const double* int_scale, double* coef64, float* coef32,
+                                                      int cstride, double* u0, double* h, FeatSetup F) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n_slots) return;
+    const int64_t i = list ? (int64_t)list[slot] : slot;
     const int c = int_curve[i];
-    const int p = B.degree[c], D = B.dim[c];
     const int64_t i0 = ioff[c];
-    const int segid = (int)(i - i0);
     const int nint = kFeatures ? (int)(ioff[c + 1] - i0) : 0;
     const double* T = B.knots + B.knot_off[c];
     const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
-    const double isc = 1.0 / int_scale[i];
-#define GCV_COEF_CASE(PP, DD)                                                                              \
-    case 10 * PP + DD:                                                                                     \
-        coef_one<PP, DD, kFeatures>(i, segid + PP, T, Pc, B.ctrl_stride, isc, coef64 + i * kCoefStride,    \
-                                    coef32 ? coef32 + i * kCoefStride : nullptr, &u0[i], &h[i], F, nint); \
-        break;
-    switch (10 * p + D) {
-        GCV_COEF_CASE(1, 2)
-        GCV_COEF_CASE(1, 3)
-        GCV_COEF_CASE(2, 2)
-        GCV_COEF_CASE(2, 3)
-        GCV_COEF_CASE(3, 2)
-        GCV_COEF_CASE(3, 3)
-        GCV_COEF_CASE(4, 2)
-        GCV_COEF_CASE(4, 3)
-        GCV_COEF_CASE(5, 2)
-        GCV_COEF_CASE(5, 3)
-        default:
-            break;
-    }
-#undef GCV_COEF_CASE
+    coef_one<P, D, kFeatures>(i, (int)(i - i0) + P, T, Pc, B.ctrl_stride, 1.0 / int_scale[i],
+                              coef64 ? coef64 + i * cstride : nullptr, coef32 ? coef32 + i * cstride : nullptr,
+                              &u0[i], &h[i],
+                              F, nint);
 }
 
 // ---------------------------------------------------------------------------

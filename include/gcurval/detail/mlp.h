@@ -234,7 +234,8 @@ struct FeatArgs {
     const double* ctrl;
     int ctrl_stride;
     const double* coef64;      // normalised power-basis coefficients
-    const float* coef32;       // FP32 copy (FP32 builds), preferred when present
+    const float* coef32;       // FP32 table (FP32 builds), preferred when present
+    int cstride;               // values per interval block
     int ls_plain;              // log2 n_subdivided (safety floor of the plain root count)
     const double* h;
     float tol0;                // level-0 tolerance (normalised units)
@@ -255,7 +256,7 @@ __device__ __forceinline__ void interval_features(const FeatArgs& F, int64_t i, 
     const int nint = (int)(F.ioff[c + 1] - F.ioff[c]);
     const double* ctrl = F.ctrl + F.ctrl_off[c] * F.ctrl_stride;
     float cf[kCoefStride];
-    load_coef_f32(F.coef32, F.coef64, i, p * D, cf);
+    load_coef_f32(F.coef32, F.coef64, F.cstride, i, p * D, cf);
     *r2 = 1.0f;
     // control-polygon features are ratios, so raw control points serve
 #define GCV_FEAT_CASE(PP, DD)                                                                        \

@@ -79,12 +79,12 @@ __device__ __forceinline__ float speed2_ratio(const float* c) {
 
 // FP32 copy of an interval's derivative coefficients (from the FP32 table when
 // the build keeps one, else rounded from the FP64 table).
-__device__ __forceinline__ void load_coef_f32(const float* coef32, const double* coef64, int64_t i, int n,
-                                              float* cf) {
+__device__ __forceinline__ void load_coef_f32(const float* coef32, const double* coef64, int cstride, int64_t i,
+                                              int n, float* cf) {
     if (coef32) {
-        for (int j = 0; j < n; j++) cf[j] = coef32[i * kCoefStride + j];
+        for (int j = 0; j < n; j++) cf[j] = coef32[i * cstride + j];
     } else {
-        for (int j = 0; j < n; j++) cf[j] = (float)coef64[i * kCoefStride + j];
+        for (int j = 0; j < n; j++) cf[j] = (float)coef64[i * cstride + j];
     }
 }
 
@@ -112,13 +112,13 @@ __device__ __forceinline__ float bspline_speed2_ratio(int p, int D, const float*
 static __global__ __launch_bounds__(256) void k_seed_floor_bspline(int64_t n, const int32_t* int_curve,
                                                                    const int32_t* degree, const int32_t* dim,
                                                                    const float* coef32, const double* coef64,
-                                                                   int ls_plain, int8_t* kmin) {
+                                                                   int cstride, int ls_plain, int8_t* kmin) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = int_curve[i];
     const int p = degree[c], D = dim[c];
     float cf[kCoefStride];
-    load_coef_f32(coef32, coef64, i, p * D, cf);
+    load_coef_f32(coef32, coef64, cstride, i, p * D, cf);
     kmin[i] = floor_from_ratio(bspline_speed2_ratio(p, D, cf), ls_plain);
 }
 
