@@ -154,6 +154,7 @@ def main():
                     help="table build: plain 64-way pre-split, or the learned per-interval seed")
     ap.add_argument("--compare-seed", action="store_true",
                     help="also time the other build path and report both")
+    ap.add_argument("--seed-weights", default=None, help="learned-seed weights (.npz); default: the packaged ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time the library's streaming copy kernel over MIB mebibytes")
     args = ap.parse_args()
@@ -184,7 +185,7 @@ def main():
     mlp = None
     if args.path == "learned" or args.compare_seed:
         from gcurval_amd.seed import SeedMLP
-        mlp = SeedMLP.pretrained(ctx)
+        mlp = SeedMLP.pretrained(ctx, args.seed_weights) if args.seed_weights else SeedMLP.pretrained(ctx)
 
     def run(path):
         """Warm up, then time args.steps steps of one build path; per-kernel events on."""

@@ -1,11 +1,14 @@
-"""Derivation of the polynomial coefficients of abs_sincos_2pi
-(include/gcurval/detail/curves.h).
+"""Derivation of the polynomial coefficients of sin2_quarter
+(include/gcurval/detail/curves.h), the squared sine behind the ellipse speed.
 
-With r = 4x, q = rint(r), f = r - q in [-1/2, 1/2]:
-  sin(pi f / 2) = f S(f^2),  cos(pi f / 2) = C(f^2),  g = f^2 in [0, 1/4].
-FP32: S (degree 3) and C (degree 4) by iteratively reweighted least squares
-toward minimax relative error on a Chebyshev grid in g, then rounded to float.
-FP64: Taylor coefficients of sin(pi f/2)/f (9 terms) and cos(pi f/2) (10 terms).
+With r = 4x, q = rint(r), f = r - q in [-1/2, 1/2] and g = f^2 in [0, 1/4]:
+  sin^2(pi f / 2) = g U(g),   U(g) = sin^2(pi f / 2) / f^2,
+and sin^2(2 pi x), cos^2(2 pi x) are g U(g) and 1 - g U(g), swapped when q is
+odd. g U(g) <= 1/2, so both squares keep full relative accuracy near zero.
+FP32: U of degree 4 by iteratively reweighted least squares toward minimax
+relative error on a Chebyshev grid in g, then rounded to float.
+FP64: Taylor coefficients of U (11 terms; the first omitted term is below
+2e-17 relative on g <= 1/4).
 Run: python -m gcurval_amd.fit_sincos
 """
 import mpmath as mp
@@ -14,35 +17,41 @@ import numpy as np
 mp.mp.dps = 40
 
 
-def fit_fp32():
+def u_exact(g):
+    f = mp.sqrt(mp.mpf(g))
+    return mp.sin(mp.pi / 2 * f) ** 2 / f ** 2
+
+
+def fit_fp32(deg=4):
     g = (np.cos(np.linspace(0, np.pi, 3000)) + 1) / 2 * 0.25
     g = g[g > 0]
-    f = np.sqrt(g)
-    ys = np.array([float(mp.sin(mp.pi / 2 * mp.mpf(x)) / mp.mpf(x)) for x in f])
-    yc = np.array([float(mp.cos(mp.pi / 2 * mp.mpf(x))) for x in f])
-    out = []
-    for y, deg in ((ys, 3), (yc, 4)):
-        w = np.ones_like(g)
-        A = np.vander(g, deg + 1, increasing=True)
-        for _ in range(40):
-            c, *_ = np.linalg.lstsq(A * w[:, None], y * w, rcond=None)
-            r = np.abs(A @ c - y) / np.abs(y)
-            w *= 1 + 5 * r / r.max()
-        out.append([float(np.float32(v)) for v in c])
-    return out
+    y = np.array([float(u_exact(x)) for x in g])
+    w = np.ones_like(g)
+    A = np.vander(g, deg + 1, increasing=True)
+    for _ in range(40):
+        c, *_ = np.linalg.lstsq(A * w[:, None], y * w, rcond=None)
+        r = np.abs(A @ c - y) / np.abs(y)
+        w *= 1 + 5 * r / r.max()
+    return [float(np.float32(v)) for v in c]
 
 
-def taylor_fp64():
-    h = mp.pi / 2
-    S = [float((-1) ** k * h ** (2 * k + 1) / mp.factorial(2 * k + 1)) for k in range(9)]
-    C = [float((-1) ** k * h ** (2 * k) / mp.factorial(2 * k)) for k in range(10)]
-    return S, C
+def taylor_fp64(terms=11):
+    # sin^2 t = sum_{k>=1} (-1)^(k+1) 2^(2k-1) t^(2k) / (2k)!,  t^2 = (pi/2)^2 g
+    h2 = (mp.pi / 2) ** 2
+    return [float((-1) ** (k + 1) * 2 ** (2 * k - 1) * h2 ** k / mp.factorial(2 * k)) for k in range(1, terms + 1)]
+
+
+def max_rel_error(coefs, dtype):
+    g = np.linspace(1e-6, 0.25, 20001)
+    acc = np.zeros_like(g, dtype=dtype)
+    for c in coefs[::-1]:
+        acc = acc * g.astype(dtype) + dtype(c)
+    ref = np.array([float(u_exact(x)) for x in g[::50]])
+    return float(np.max(np.abs(acc[::50].astype(np.float64) - ref) / ref))
 
 
 if __name__ == "__main__":
-    s32, c32 = fit_fp32()
-    s64, c64 = taylor_fp64()
-    print("FP32 S", s32)
-    print("FP32 C", c32)
-    print("FP64 S", s64)
-    print("FP64 C", c64)
+    u32 = fit_fp32()
+    u64 = taylor_fp64()
+    print("FP32 U", u32, "max rel err", max_rel_error(u32, np.float32))
+    print("FP64 U", u64, "max rel err", max_rel_error(u64, np.float64))

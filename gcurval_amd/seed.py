@@ -12,8 +12,10 @@ the work.
 This module drives the device predictor of k_i: a 16-64-64-2 perceptron on
 BF16 matrix cores (gcv_mlp_*). Training labels come from the library itself:
 per-interval quadrature counts of uniform-k builds for k = 0..6; the label of
-an interval is the cheapest k at or above its safety floor, and the second
-output learns whether that seed needs no refinement.
+an interval is the smallest k at or above its safety floor whose roots all
+pass the level-0 test (or, with the "cheapest" policy, the k of fewest
+quadratures), and the second output learns whether that seed needs no
+refinement.
 
 ``python -m gcurval_amd.seed --train`` regenerates ``seed_weights.npz``.
 """
@@ -174,38 +176,54 @@ def cost_matrix(ctx: Context, batch, precision=GCV_FP32, **opts):
     return cost, floor
 
 
-def labels_from_costs(cost: np.ndarray, floor: np.ndarray) -> np.ndarray:
-    """y[:, 0] = cheapest k at or above the floor; y[:, 1] = 1 when that seed
-    needs no refinement (3 quadratures per root)."""
+def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "clean") -> np.ndarray:
+    """Per-interval targets from the uniform-k quadrature counts.
+
+    y[:, 0]: with policy "clean", the smallest k at or above the floor whose
+    2^k roots all pass the level-0 test (3 quadratures per root; k = 6 when
+    none does), so the table has almost no refined roots and the sampler no
+    tree re-walks; with "cheapest", the k of fewest quadratures.
+    y[:, 1] = 1 when the interval needs no refinement at that k.
+    """
     ks = np.arange(MAX_K + 1)[:, None]
-    masked = np.where(ks >= floor[None, :].astype(np.int64), cost, np.iinfo(np.int64).max)
-    k = np.argmin(masked, axis=0)
+    ok = ks >= floor[None, :].astype(np.int64)
+    clean = (cost == 3 * (1 << ks)) & ok
+    if policy == "clean":
+        k = np.where(clean.any(0), np.argmax(clean, axis=0), MAX_K)
+    elif policy == "cheapest":
+        k = np.argmin(np.where(ok, cost, np.iinfo(np.int64).max), axis=0)
+    else:
+        raise ValueError(f"unknown label policy {policy}")
     c = cost[k, np.arange(cost.shape[1])]
-    y = np.stack([k.astype(np.float32), (c == 3 * (1 << k)).astype(np.float32)], axis=1)
-    return y
+    return np.stack([k.astype(np.float32), (c == 3 * (1 << k)).astype(np.float32)], axis=1)
 
 
 def training_batches(seed: int = 100):
-    """Synthetic B-spline and ellipse batches spanning the classes the predictor serves."""
+    """Synthetic batches spanning the classes the predictor serves (about 2.2M
+    intervals): cubic 3-D random walks with near-cusps, mixed degrees 2/3/5 in
+    2-D/3-D with near-collinear loops, degrees 1/4, and ellipses. The seeds
+    differ from every benchmark and test batch."""
     from . import workloads as W
     out = []
+    for j in range(3):
+        out.append(W.bspline_batch(2500, seed=seed + j, degrees=(3,), dims=(3,), n_ctrl_range=(8, 2048),
+                                   cusp_frac=0.1))
     for j in range(4):
-        out.append(W.bspline_batch(1500, seed=seed + j, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(16, 512),
-                                   hairpin_frac=0.2))
-    out.append(W.bspline_batch(1500, seed=seed + 10, degrees=(3,), dims=(3,), n_ctrl_range=(8, 1024)))
+        out.append(W.bspline_batch(1500, seed=seed + 5 + j, degrees=(2, 3, 5), dims=(2, 3),
+                                   n_ctrl_range=(16, 1024), hairpin_frac=0.2))
     out.append(W.bspline_batch(1000, seed=seed + 11, degrees=(1, 4), dims=(2, 3), n_ctrl_range=(8, 256),
                                hairpin_frac=0.1))
-    out.append(W.ellipse_batch(100000, seed=seed + 20))
+    out.append(W.ellipse_batch(200000, seed=seed + 20))
     return out
 
 
-def make_dataset(ctx: Context, batches, mlp: SeedMLP):
+def make_dataset(ctx: Context, batches, mlp: SeedMLP, policy: str = "clean"):
     X, Y = [], []
     for b in batches:
         cost, floor = cost_matrix(ctx, b)
         with ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
             X.append(mlp.features(eng))
-        Y.append(labels_from_costs(cost, floor))
+        Y.append(labels_from_costs(cost, floor, policy))
     return np.concatenate(X), np.concatenate(Y)
 
 
@@ -250,6 +268,7 @@ def _main():
     ap.add_argument("--train", action="store_true")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--policy", default="clean", choices=("clean", "cheapest"))
     a = ap.parse_args()
     if not a.train:
         ap.print_help()
@@ -258,12 +277,12 @@ def _main():
     with Context(0) as ctx:
         t0 = time.time()
         probe = SeedMLP(ctx)
-        X, Y = make_dataset(ctx, training_batches(), probe)
+        X, Y = make_dataset(ctx, training_batches(), probe, a.policy)
         probe.close()
         print(f"dataset {X.shape[0]} intervals in {time.time() - t0:.1f}s; label histogram "
               f"{np.bincount(Y[:, 0].astype(int), minlength=7).tolist()}", flush=True)
         mlp = train(ctx, X, Y, epochs=a.epochs)
-        mlp.save(a.out, n_rows=int(X.shape[0]), epochs=a.epochs)
+        mlp.save(a.out, n_rows=int(X.shape[0]), epochs=a.epochs, policy=a.policy)
         print(f"saved {a.out}", flush=True)
 
 

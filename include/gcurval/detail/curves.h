@@ -130,61 +130,54 @@ GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, co
 #if defined(__HIPCC__)
 
 // ---------------------------------------------------------------------------
-// |sin 2 pi x| and |cos 2 pi x| for x in [0, 1]: r = 4x and its nearest
-// integer q are exact, f = r - q lies in [-1/2, 1/2], and
-//   sin(pi f / 2) = f S(f^2),  cos(pi f / 2) = C(f^2),
-// swapped when q is odd (signs are not needed: only squares are used).
-// FP32: S, C are weighted least-squares fits on f^2 in [0, 1/4]
-// (gcurval_amd/fit_sincos.py; max abs error 8.6e-8 over x in [0, 1]).
-// FP64: S, C are the Taylor series of sin(pi f/2)/f and cos(pi f/2) to
-// f^16 / f^18 (max abs error 1.1e-16).
+// sin^2(2 pi x) and cos^2(2 pi x) for x in [0, 1] from one polynomial: r = 4x
+// and its nearest integer q are exact, f = r - q lies in [-1/2, 1/2], and with
+// g = f^2 the squared quarter-period sine sin^2(pi f / 2) = g U(g) <= 1/2;
+// sin^2(2 pi x) and cos^2(2 pi x) are g U(g) and 1 - g U(g), swapped when q
+// is odd, so both keep full relative accuracy near their zeros.
+// FP32: U is a degree-4 weighted least-squares fit (max relative error
+// 1.2e-7 in FP32 evaluation); FP64: 11 Taylor terms (2.2e-16). Derivation:
+// gcurval_amd/fit_sincos.py.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void abs_sincos_2pi(float x, float& as, float& ac) {
+__device__ __forceinline__ void sincos2_2pi(float x, float& s2, float& c2) {
     const float r = 4.0f * x;
     const float q = rintf(r);
     const float f = r - q;
     const float g = f * f;
-    const float sp = f * fmaf(fmaf(fmaf(-0.004601657390594482f, g, 0.07968003302812576f), g, -0.6459634900093079f), g,
-                              1.5707963705062866f);
-    const float cp = fmaf(fmaf(fmaf(fmaf(0.0009033629903569818f, g, -0.020859936252236366f), g, 0.2536691725254059f), g,
-                               -1.2337005138397217f), g, 1.0f);
+    const float u = fmaf(fmaf(fmaf(fmaf(0.012307871133089066f, g, -0.1175343245267868f), g, 0.667619526386261f), g,
+                              -2.029355764389038f),
+                         g, 2.4674010276794434f);
+    const float lo = g * u, hi = 1.0f - lo;
     const bool odd = ((int)q) & 1;
-    as = odd ? cp : sp;
-    ac = odd ? sp : cp;
+    s2 = odd ? hi : lo;
+    c2 = odd ? lo : hi;
 }
-__device__ __forceinline__ void abs_sincos_2pi(double x, double& as, double& ac) {
+__device__ __forceinline__ void sincos2_2pi(double x, double& s2, double& c2) {
     const double r = 4.0 * x;
     const double q = rint(r);
     const double f = r - q;
     const double g = f * f;
-    double sp = 6.0669357311061955e-12;
-    sp = fma(sp, g, -6.688035109811468e-10);
-    sp = fma(sp, g, 5.692172921967927e-08);
-    sp = fma(sp, g, -3.598843235212085e-06);
-    sp = fma(sp, g, 0.00016044118478735983);
-    sp = fma(sp, g, -0.004681754135318688);
-    sp = fma(sp, g, 0.07969262624616705);
-    sp = fma(sp, g, -0.6459640975062463);
-    sp = fma(sp, g, 1.5707963267948966);
-    sp *= f;
-    double cp = -5.294400200734623e-13;
-    cp = fma(cp, g, 6.565963114979473e-11);
-    cp = fma(cp, g, -6.386603083791852e-09);
-    cp = fma(cp, g, 4.710874778818172e-07);
-    cp = fma(cp, g, -2.5202042373060607e-05);
-    cp = fma(cp, g, 0.0009192602748394266);
-    cp = fma(cp, g, -0.02086348076335296);
-    cp = fma(cp, g, 0.25366950790104803);
-    cp = fma(cp, g, -1.2337005501361697);
-    cp = fma(cp, g, 1.0);
+    double u = 3.850353565300677e-11;
+    u = fma(u, g, -1.8023653987312506e-09);
+    u = fma(u, g, 6.939476231106886e-08);
+    u = fma(u, g, -2.1515347935164736e-06);
+    u = fma(u, g, 5.231905246242285e-05);
+    u = fma(u, g, -0.0009647871547019616);
+    u = fma(u, g, 0.01290344569500703);
+    u = fma(u, g, -0.1176653151794466);
+    u = fma(u, g, 0.6676313844272947);
+    u = fma(u, g, -2.0293560632083842);
+    u = fma(u, g, 2.4674011002723395);
+    const double lo = g * u, hi = 1.0 - lo;
     const bool odd = ((long long)q) & 1;
-    as = odd ? cp : sp;
-    ac = odd ? sp : cp;
+    s2 = odd ? hi : lo;
+    c2 = odd ? lo : hi;
 }
 
 // ---------------------------------------------------------------------------
-// Ellipse evaluator. Ctx holds 2 pi a' and 2 pi b' with a', b' the optionally
-// normalised semi-axes, so speed(x) = sqrt((A sin 2pi x)^2 + (B cos 2pi x)^2).
+// Ellipse evaluator. Ctx holds A^2 and B^2 with A = 2 pi a', B = 2 pi b' and
+// a', b' the optionally normalised semi-axes, so
+// speed(x) = sqrt(A^2 sin^2(2 pi x) + B^2 cos^2(2 pi x)).
 // ---------------------------------------------------------------------------
 template <typename Dt>
 struct EllipseEval {
@@ -193,7 +186,7 @@ struct EllipseEval {
     int normalize;
 
     struct Ctx {
-        Dt A, B;
+        Dt A2, B2;
     };
 
     __device__ __forceinline__ double scale(int i) const {
@@ -204,19 +197,18 @@ struct EllipseEval {
     __device__ __forceinline__ Ctx load(int i) const { return load(i, 1.0 / scale(i)); }
     // with the reciprocal of scale(i) already at hand
     __device__ __forceinline__ Ctx load(int i, double inv_scale) const {
-        return Ctx{Dt(kTwoPi * a[i] * inv_scale), Dt(kTwoPi * b[i] * inv_scale)};
+        const double A = kTwoPi * a[i] * inv_scale, B = kTwoPi * b[i] * inv_scale;
+        return Ctx{Dt(A * A), Dt(B * B)};
     }
     __device__ __forceinline__ float speed(const Ctx& c, float x) const {
-        float sn, cs;
-        abs_sincos_2pi(x, sn, cs);
-        const float u = float(c.A) * sn, v = float(c.B) * cs;
-        return __builtin_amdgcn_sqrtf(fmaf(u, u, v * v));
+        float s2, c2;
+        sincos2_2pi(x, s2, c2);
+        return __builtin_amdgcn_sqrtf(fmaf(float(c.A2), s2, float(c.B2) * c2));
     }
     __device__ __forceinline__ double speed(const Ctx& c, double x) const {
-        double sn, cs;
-        abs_sincos_2pi(x, sn, cs);
-        const double u = double(c.A) * sn, v = double(c.B) * cs;
-        return sqrt(fma(u, u, v * v));
+        double s2, c2;
+        sincos2_2pi(x, s2, c2);
+        return sqrt(fma(double(c.A2), s2, double(c.B2) * c2));
     }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }

@@ -141,11 +141,39 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0;
     int max_stack = 0, overflow = 0;
 
+    // Software pipeline over this wave's tiles: the interval id of tile t + 2
+    // and the evaluator context of tile t + 1 are in flight while tile t
+    // computes, so the dependent global loads (list -> coefficients) overlap
+    // the quadratures instead of stalling every tile.
+    auto interval_of = [&](int64_t tl, bool& ok) -> int {
+        const int64_t slot = (tl << ipw_log2) + jslot;
+        ok = slot < n_slots;
+        return ok ? (list ? list[slot] : (int)slot) : (list ? list[0] : 0);
+    };
+    bool v_cur = false, v_nxt = false;
+    int i_cur = 0, i_nxt = 0;
+    typename Ev::Ctx ctx_cur{};
+    if (gwave < n_tiles) {
+        i_cur = interval_of(gwave, v_cur);
+        ctx_cur = ev.load(i_cur);
+    }
+    if (gwave + nwaves < n_tiles) i_nxt = interval_of(gwave + nwaves, v_nxt);
+
     for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
-        const int64_t slot = (tile << ipw_log2) + jslot;
-        const bool valid = slot < n_slots;
-        const int i = valid ? (list ? list[slot] : (int)slot) : (list ? list[0] : 0);
-        const typename Ev::Ctx ctx = ev.load(i);
+        const bool valid = v_cur;
+        const int i = i_cur;
+        const typename Ev::Ctx ctx = ctx_cur;
+        typename Ev::Ctx ctx_nxt{};
+        if (tile + nwaves < n_tiles) ctx_nxt = ev.load(i_nxt);
+        bool v_nn = false;
+        int i_nn = 0;
+        if (tile + 2 * nwaves < n_tiles) i_nn = interval_of(tile + 2 * nwaves, v_nn);
+        const int64_t rbase = A.root_off ? A.root_off[i] : ((int64_t)i << ls);
+        i_cur = i_nxt;
+        v_cur = v_nxt;
+        ctx_cur = ctx_nxt;
+        i_nxt = i_nn;
+        v_nxt = v_nn;
 
         Dt x0, xm, x1;
         task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
@@ -223,7 +251,6 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         double cost = 0.0;
         if (A.int_cost) cost = seg_scan_incl(valid ? 3.0 + (fail ? 2.0 * s_cnt[lane] : 0.0) : 0.0, ls);
         if (valid) {
-            const int64_t rbase = A.root_off ? A.root_off[i] : ((int64_t)i << ls);
             A.root_prefix[rbase + k] = pre;
             if (k == nsub - 1) {
                 if (A.int_cost) A.int_cost[i] = (int32_t)cost;
@@ -576,7 +603,7 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         tau = tl;
     }
     Dt v_last;
-    const int pieces = (b - a) > Dt(1.0 / 64.0) ? 2 : 1;
+    const int pieces = 1;
     const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last);
     double dd = (double)d;
     if (sizeof(Dt) < sizeof(double) && A.fp64_step) {

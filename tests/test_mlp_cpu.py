@@ -73,13 +73,19 @@ def test_bf16_network_close_to_exact():
 
 
 def test_labels_from_costs():
-    # interval 0: cheapest k=1 (passes cleanly); 1: floor 4 forces k>=4; 2: k=6 cheapest, refined
-    cost = np.array([[30, 3, 9000], [6, 6, 8000], [12, 12, 7000], [24, 24, 6000],
-                     [48, 48, 5000], [96, 96, 4000], [192, 192, 1000]], np.int64)
-    floor = np.array([0, 4, 0], np.int8)
-    y = S.labels_from_costs(cost, floor)
-    np.testing.assert_array_equal(y[:, 0], [1, 4, 6])
-    np.testing.assert_array_equal(y[:, 1], [1, 1, 0])
+    # interval 0: k=1 passes cleanly; 1: floor 4 forces k>=4; 2: never clean, k=6 cheapest;
+    # 3: k=1 refined but cheapest (9 < 12), k=2 the first clean seed
+    cost = np.array([[30, 3, 9000, 40], [6, 6, 8000, 9], [12, 12, 7000, 12], [24, 24, 6000, 24],
+                     [48, 48, 5000, 48], [96, 96, 4000, 96], [192, 192, 1000, 192]], np.int64)
+    floor = np.array([0, 4, 0, 0], np.int8)
+    y = S.labels_from_costs(cost, floor, "clean")
+    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 2])
+    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 1])
+    y = S.labels_from_costs(cost, floor, "cheapest")
+    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 1])
+    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 0])
+    with pytest.raises(ValueError):
+        S.labels_from_costs(cost, floor, "nope")
 
 
 def test_weights_file_shapes():
