@@ -4,12 +4,17 @@
 A step is one pass of the hot path over the whole synthetic batch: the table
 build (setup, adaptive Gauss-Legendre subdivision, cumulative scan) followed by
 arc-length-uniform sampling of every curve, with all inputs resident in HBM.
-`value` is curve-intervals per second of step time; samples per second of the
-same steps are reported beside it.
+`value` is curve-intervals per second of step time with the learned-seed
+build (the MLP predicts each knot interval's initial root count; the adaptive
+error test still verifies every root); samples per second of the same steps
+are reported beside it, and the plain 64-way build is timed on the same
+inputs ("seed_comparison").
 
 Default workload: 1,000,000 ellipses, a ~ U[0.5, 10], b = a U[0.01, 1], one
 knot interval each, FP32 quadrature with FP64 scan, 64..4096 samples per curve
-(log-uniform), uniform positions j/(N-1).
+(log-uniform), uniform positions j/(N-1). The same line carries the learned vs
+plain comparison on ~4M knot intervals of cubic 3-D B-splines
+("c3_seed_comparison").
 """
 from __future__ import annotations
 
@@ -142,6 +147,117 @@ def copy_calibration(G, mib, steps):
                           "timed_ms": ms, "steps": steps}), flush=True)
 
 
+METRIC = "curve-intervals/sec and samples/sec at \u22641e-6 rel error: MLP-seeded vs plain, 1 GPU"
+
+
+def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
+    """Time `steps` hot-path steps (table build + uniform sampling of every
+    curve) of each build path after `warmup` untimed steps; per-kernel HIP
+    events on the library stream."""
+    eng = G.ArclengthCalculateGpu(ctx, batch, precision=G.GCV_FP32, gl_order=3, n_subdivided=64, normalize=True)
+    n_total = int(np.sum(counts))
+    off = np.zeros(batch.n_curve + 1, np.int64)
+    np.cumsum(counts, out=off[1:])
+    d_off = ctx.array(off)
+    d_out = ctx.empty(n_total, np.float64)
+    mlp = None
+    if "learned" in paths:
+        from gcurval_amd.seed import SeedMLP
+        mlp = SeedMLP.pretrained(ctx, weights) if weights else SeedMLP.pretrained(ctx)
+    out = {}
+    for path in paths:
+        def step():
+            if path == "learned":
+                eng.createArclengthTableLearned(mlp)
+            else:
+                eng.createArclengthTable()
+            eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
+
+        for _ in range(warmup):
+            step()
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            step()
+        ms = ctx.timer_stop()
+        ctx.sync()
+        kernels = {}
+        for name in KERNELS:
+            t, n = ctx.profile_query(name)
+            if n:
+                kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n, "ms_per_step": t / steps}
+        t_all, _ = ctx.profile_query("k_")
+        t_smp, _ = ctx.profile_query("k_sample")
+        ctx.profile(False)
+        st = eng.stats()
+        log(f"[bench] {path}: {steps} steps in {ms:.3f} ms; build stats {st}")
+        out[path] = {"ms": ms, "ms_per_step": ms / steps, "kernels": kernels, "stats": st,
+                     "build_ms": (t_all - t_smp) / steps, "sample_ms": t_smp / steps,
+                     "lengths": eng.getAllCurveArclengths()}
+    eng.close()
+    d_off.free()
+    d_out.free()
+    if mlp is not None:
+        mlp.close()
+    return out
+
+
+def comparison(runs, n_int, n_total):
+    pl, le = runs["plain"], runs["learned"]
+
+    def side(r):
+        return {"build_ms": r["build_ms"], "step_ms": r["ms_per_step"], "sample_ms": r["sample_ms"],
+                "quadratures": r["stats"]["quadratures"], "deep_roots": r["stats"]["deep_roots"],
+                "build_intervals_per_sec": n_int / (r["build_ms"] * 1e-3),
+                "step_intervals_per_sec": n_int / (r["ms_per_step"] * 1e-3),
+                "step_samples_per_sec": n_total / (r["ms_per_step"] * 1e-3),
+                "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r["kernels"].items()}}
+
+    return {"plain": side(pl), "learned": side(le),
+            "build_speedup": pl["build_ms"] / le["build_ms"],
+            "step_speedup": pl["ms_per_step"] / le["ms_per_step"],
+            "quadrature_ratio": pl["stats"]["quadratures"] / max(1, le["stats"]["quadratures"]),
+            "max_rel_length_diff": float(np.max(np.abs(le["lengths"] - pl["lengths"]) / pl["lengths"]))}
+
+
+def roofline_of(run, batch, n_total, cfg_name, steps):
+    """Dominant kernel (FP32 VALU bound): algorithmic FLOP per step from the
+    counted quadratures (subdivision) or samples (sampler), divided by that
+    kernel's event-timed duration per step."""
+    kernels, st = run["kernels"], run["stats"]
+    if batch.kind == "ellipse":
+        fnode = f_node_ellipse()
+    else:
+        ps = np.repeat(batch.degree, batch.intervals_per_curve())
+        ds = np.repeat(batch.dim, batch.intervals_per_curve())
+        fnode = float(np.mean([f_node_bspline(int(p), int(d)) for p, d in zip(ps[:100000], ds[:100000])]))
+    ngl, iters = 3, 2
+    sub_ms = kernels.get("k_subdivide", {"ms_per_step": 0.0})["ms_per_step"]
+    smp_ms = kernels.get("k_sample", {"ms_per_step": 0.0})["ms_per_step"]
+    quad_flop = st["quadratures"] * (ngl * fnode + 5)
+    # per sample: 2 speeds (Hermite start), one NGL quadrature and its end
+    # speed, (iters - 1) trapezoid speeds in FP32, one NGL quadrature in FP64
+    smp_flop = n_total * ((2 + ngl + iters + ngl) * fnode + 2 * 5)
+    if smp_ms >= sub_ms:
+        dom, flop, dur = "k_sample", smp_flop, smp_ms
+    else:
+        dom, flop, dur = "k_subdivide", quad_flop, sub_ms
+    achieved = flop / (dur * 1e-3) / 1e12 if dur > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        rec = json.load(open(tf)).get(cfg_name, {}).get("kernels", {}).get(dom)
+        if rec:
+            traffic = rec["read_bytes"] + rec["write_bytes"]
+    return {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "flop_per_step": flop,
+            "kernel_ms_per_step": dur,
+            "hbm_gbs_of_traffic": (traffic / (dur * 1e-3) / 1e9) if (traffic and dur > 0) else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,10 +266,11 @@ def main():
     ap.add_argument("--config", default=os.environ.get("GCV_BENCH_CONFIG", "c1"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--path", choices=("plain", "learned"), default="plain",
-                    help="table build: plain 64-way pre-split, or the learned per-interval seed")
-    ap.add_argument("--compare-seed", action="store_true",
-                    help="also time the other build path and report both")
+    ap.add_argument("--path", choices=("plain", "learned"), default="learned",
+                    help="build path whose throughput is `value` (the other is timed beside it)")
+    ap.add_argument("--no-compare", action="store_true", help="time only --path")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the learned-vs-plain check on the 4M-interval cubic B-spline workload")
     ap.add_argument("--seed-weights", default=None, help="learned-seed weights (.npz); default: the packaged ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time the library's streaming copy kernel over MIB mebibytes")
@@ -172,150 +289,56 @@ def main():
     t_gen = time.time()
     batch, counts, cfg = workload(args.config)
     n_total = int(np.sum(counts))
-    log(f"[bench] workload {cfg['workload']}: {batch.n_curve} curves, {batch.n_intervals} intervals, "
+    n_int = batch.n_intervals
+    log(f"[bench] workload {cfg['workload']}: {batch.n_curve} curves, {n_int} intervals, "
         f"{n_total} samples (generated in {time.time() - t_gen:.1f}s)")
 
     ctx = G.Context(0)
     n_cu, arch = ctx.device_info()
-    eng = G.ArclengthCalculateGpu(ctx, batch, precision=G.GCV_FP32, gl_order=3, n_subdivided=64, normalize=True)
-    off = np.zeros(batch.n_curve + 1, np.int64)
-    np.cumsum(counts, out=off[1:])
-    d_off = ctx.array(off)
-    d_out = ctx.empty(n_total, np.float64)
-    mlp = None
-    if args.path == "learned" or args.compare_seed:
-        from gcurval_amd.seed import SeedMLP
-        mlp = SeedMLP.pretrained(ctx, args.seed_weights) if args.seed_weights else SeedMLP.pretrained(ctx)
-
-    def run(path):
-        """Warm up, then time args.steps steps of one build path; per-kernel events on."""
-        def step():
-            if path == "learned":
-                eng.createArclengthTableLearned(mlp)
-            else:
-                eng.createArclengthTable()
-            eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
-
-        for _ in range(args.warmup):
-            step()
-        ctx.sync()
-        ctx.profile(True)
-        ctx.profile_reset()
-        ctx.sync()
-        ctx.timer_start()
-        for _ in range(args.steps):
-            step()
-        ms = ctx.timer_stop()
-        ctx.sync()
-        kernels = {}
-        for name in KERNELS:
-            t, n = ctx.profile_query(name)
-            if n:
-                kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n}
-        t_all, _ = ctx.profile_query("k_")
-        t_smp, _ = ctx.profile_query("k_sample")
-        ctx.profile(False)
-        st = eng.stats()
-        log(f"[bench] {path}: timed {args.steps} steps in {ms:.3f} ms; build stats {st}")
-        return {"ms": ms, "kernels": kernels, "stats": st, "build_ms": (t_all - t_smp) / args.steps,
-                "sample_ms": t_smp / args.steps, "lengths": eng.getAllCurveArclengths()}
-
-    main_run = run(args.path)
-    ms, kernels, st = main_run["ms"], main_run["kernels"], main_run["stats"]
-    ms_per_step = ms / args.steps
-    build_ms, sample_ms = main_run["build_ms"], main_run["sample_ms"]
-    n_int = batch.n_intervals
-    comparison = None
-    if args.compare_seed:
-        other = run("plain" if args.path == "learned" else "learned")
-        runs = {args.path: main_run, ("plain" if args.path == "learned" else "learned"): other}
-        pl, le = runs["plain"], runs["learned"]
-        comparison = {
-            "plain": {"build_ms": pl["build_ms"], "step_ms": pl["ms"] / args.steps,
-                      "quadratures": pl["stats"]["quadratures"],
-                      "build_intervals_per_sec": n_int / (pl["build_ms"] * 1e-3),
-                      "kernel_ms_per_step": {k: v["total_ms"] / args.steps for k, v in pl["kernels"].items()}},
-            "learned": {"build_ms": le["build_ms"], "step_ms": le["ms"] / args.steps,
-                        "quadratures": le["stats"]["quadratures"],
-                        "build_intervals_per_sec": n_int / (le["build_ms"] * 1e-3),
-                        "predict_ms": le["kernels"].get("k_mlp_predict", {"avg_ms": 0.0})["avg_ms"],
-                        "kernel_ms_per_step": {k: v["total_ms"] / args.steps for k, v in le["kernels"].items()}},
-            "build_speedup": pl["build_ms"] / le["build_ms"],
-            "step_speedup": pl["ms"] / le["ms"],
-            "quadrature_ratio": pl["stats"]["quadratures"] / max(1, le["stats"]["quadratures"]),
-            "max_rel_length_diff": float(np.max(np.abs(le["lengths"] - pl["lengths"]) / pl["lengths"])),
-        }
-        log(f"[bench] learned vs plain: build {comparison['build_speedup']:.2f}x, "
-            f"step {comparison['step_speedup']:.2f}x, quadratures {comparison['quadrature_ratio']:.2f}x")
-
-    # dominant kernel roofline (FP32 VALU, algorithmic FLOP from counted node evaluations)
-    if batch.kind == "ellipse":
-        fnode = f_node_ellipse()
-    else:
-        ps = np.repeat(batch.degree, batch.intervals_per_curve())
-        ds = np.repeat(batch.dim, batch.intervals_per_curve())
-        fnode = float(np.mean([f_node_bspline(int(p), int(d)) for p, d in zip(ps[:100000], ds[:100000])]))
-    ngl = 3
-    iters = 2
-    sub = kernels.get("k_subdivide", {"avg_ms": 0.0})
-    smp = kernels.get("k_sample", {"avg_ms": 0.0})
-    # subdivision launches per step (one per evaluator class / k bucket)
-    sub_step_ms = sub.get("total_ms", 0.0) / args.steps
-    smp_step_ms = smp.get("total_ms", 0.0) / args.steps
-    quad_flop = (st["quadratures"] * (ngl * fnode + 5))
-    # sampler per sample: 2 speeds (Hermite start) + one NGL quadrature and its
-    # speed + (iters - 1) trapezoid speeds in FP32, then one NGL quadrature in FP64
-    smp_flop = n_total * ((2 + ngl + iters + ngl) * fnode + 2 * 5)
-    if smp_step_ms >= sub_step_ms:
-        dom, flop, dur = "k_sample", smp_flop, smp_step_ms
-    else:
-        dom, flop, dur = "k_subdivide", quad_flop, sub_step_ms
-    achieved = flop / (dur * 1e-3) / 1e12 if dur > 0 else 0.0
-    # FP32 vector-ALU bound; the peak is MI355X's published FP32 figure.
-    # traffic: HBM bytes per step of the same kernel on the same workload from
-    # the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a
-    # streaming copy), when they exist.
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        rec = json.load(open(tf)).get(cfg["workload"], {}).get("kernels", {}).get(dom)
-        if rec:
-            traffic = rec["read_bytes"] + rec["write_bytes"]
-    roofline = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                "flop_per_step": flop, "kernel_ms_per_step": dur}
-    hbm_out_gbs = (n_total * 8) / (smp_step_ms * 1e-3) / 1e9 if smp_step_ms > 0 else 0.0
+    other = "plain" if args.path == "learned" else "learned"
+    paths = [args.path] if args.no_compare else [args.path, other]
+    runs = measure(G, ctx, batch, counts, args.steps, args.warmup, paths, args.seed_weights)
+    main_run = runs[args.path]
+    ms = main_run["ms"]
 
     result = {
-        "metric": "curve-intervals/sec and samples/sec at <=1e-6 rel error (plain GPU path, 1 GPU)",
+        "metric": METRIC,
         "value": n_int * args.steps / (ms * 1e-3),
         "unit": "curve-intervals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 quadrature + fp64 scan/polish",
+        "dtype": "f32 quadrature, f64 scan and final Newton step",
         "data": "synthetic (seeded)",
         "config": cfg,
+        "path": args.path,
         "samples_per_sec": n_total * args.steps / (ms * 1e-3),
         "n_intervals": n_int,
         "n_samples": n_total,
-        "build_ms": build_ms,
-        "sample_ms": sample_ms,
-        "build_intervals_per_sec": n_int / (build_ms * 1e-3) if build_ms > 0 else None,
-        "sample_only_samples_per_sec": n_total / (sample_ms * 1e-3) if sample_ms > 0 else None,
-        "sample_output_hbm_gbs": hbm_out_gbs,
-        "kernels": kernels,
-        "build_stats": st,
-        "roofline": roofline,
+        "build_ms": main_run["build_ms"],
+        "sample_ms": main_run["sample_ms"],
+        "build_intervals_per_sec": n_int / (main_run["build_ms"] * 1e-3),
+        "sample_only_samples_per_sec": n_total / (main_run["sample_ms"] * 1e-3),
+        "kernels": main_run["kernels"],
+        "build_stats": main_run["stats"],
+        "roofline": roofline_of(main_run, batch, n_total, cfg["workload"], args.steps),
         "device": {"arch": arch, "n_cu": n_cu},
-        "path": args.path,
     }
-    if comparison:
-        result["seed_comparison"] = comparison
+    if not args.no_compare:
+        result["seed_comparison"] = comparison(runs, n_int, n_total)
+    del runs, main_run
+
+    if not args.no_c3 and args.config != "c3":
+        b3, c3, cfg3 = workload("c3")
+        r3 = measure(G, ctx, b3, c3, 3, 1, ["plain", "learned"], args.seed_weights)
+        comp = comparison(r3, b3.n_intervals, int(np.sum(c3)))
+        comp["workload"] = cfg3
+        result["c3_seed_comparison"] = comp
+        log(f"[bench] c3 learned vs plain: build {comp['build_speedup']:.2f}x, step {comp['step_speedup']:.2f}x")
 
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -332,9 +355,6 @@ def main():
             result["cpu_baseline"] = {"value": None, "unit": "curve-intervals/s", "cores": 0, "kind": "port",
                                       "sample": f"unavailable: {e}"}
     print(json.dumps(result), flush=True)
-    eng.close()
-    d_off.free()
-    d_out.free()
     ctx.close()
 
 
