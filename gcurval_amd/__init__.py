@@ -108,6 +108,7 @@ _SIGS = [
     ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5),
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("gcv_ctx_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
     ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -66,6 +66,28 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     return LIB
 
 
+TEST_DRIVERS = [("tests/cpp/template_surface.hip", "tests/cpp/template_surface")]
+
+
+def build_tests(verbose: bool = True) -> list:
+    """Build the C++ drivers of the header-only template surface used by the
+    GPU tests (linked against the library, found through an rpath)."""
+    out = []
+    for src, exe in TEST_DRIVERS:
+        s, e = os.path.join(ROOT, src), os.path.join(ROOT, exe)
+        rpath = os.path.relpath(HERE, os.path.dirname(e))
+        cmd = [HIPCC, *HIP_FLAGS, s, "-o", e + ".tmp", "-L" + HERE, "-lgcurval_amd",
+               "-Wl,-rpath,$ORIGIN/" + rpath]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        os.replace(e + ".tmp", e)
+        out.append(e)
+        if verbose:
+            print(f"[gcurval_amd] built {e}", flush=True)
+    return out
+
+
 def build_oracle(verbose: bool = True) -> str:
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     odir = os.path.join(ROOT, "oracle")
@@ -80,5 +102,7 @@ def build_oracle(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    if "--no-tests" not in sys.argv:
+        build_tests()
     if "--no-oracle" not in sys.argv:
         build_oracle()

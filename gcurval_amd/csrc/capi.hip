@@ -175,6 +175,12 @@ int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
     return GCV_OK;
 }
 
+int gcv_ctx_stream(gcv_ctx* c, void** stream) {
+    if (!c || !stream) return GCV_ERR_INVALID;
+    *stream = (void*)c->stream;
+    return GCV_OK;
+}
+
 int gcv_ctx_destroy(gcv_ctx* c) {
     if (!c) return GCV_OK;
     hipSetDevice(c->device);

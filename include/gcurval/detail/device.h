@@ -40,10 +40,23 @@ template <> struct GL<5> {
                                     0.23692688505618908, 0.23692688505618908};
 };
 
+// An evaluator with `static constexpr bool kRule = true` supplies its own
+// length rule `length(ctx, x0, x1)` (user plug-ins: the traits'
+// CalculateFunction); the built-in curve families are integrated here.
+template <class Ev, class = void>
+struct HasLengthRule {
+    static constexpr bool value = false;
+};
+template <class Ev>
+struct HasLengthRule<Ev, decltype((void)Ev::kRule)> {
+    static constexpr bool value = Ev::kRule;
+};
+
 // n-point quadrature of the evaluator's speed over [x0, x1] (interval-local
 // parameter), in the arithmetic of Dt.
 template <int N, class Ev, typename Dt>
 __device__ __forceinline__ Dt quad(const Ev& ev, const typename Ev::Ctx& c, Dt x0, Dt x1) {
+    if constexpr (HasLengthRule<Ev>::value) return ev.length(c, x0, x1);
     const Dt k = (x1 - x0) * Dt(0.5);
     const Dt m = (x1 + x0) * Dt(0.5);
     Dt acc = Dt(0);

@@ -451,6 +451,7 @@ __device__ __forceinline__ double nrcp(double v) { return 1.0 / v; }
 // Quadrature of speed over [a, a + d], with the width d carried exactly.
 template <int N, class Ev, typename Dt>
 __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt d) {
+    if constexpr (HasLengthRule<Ev>::value) return ev.length(c, a, a + d);
     const Dt k = d * Dt(0.5);
     const Dt m = a + k;
     Dt acc = Dt(0);

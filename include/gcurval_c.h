@@ -73,6 +73,11 @@ int gcv_copy_d2h(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gcv_copy_d2d(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gcv_sync(gcv_ctx* ctx);
 int gcv_device_info(gcv_ctx* ctx, int* n_cu, char* name, int name_len);
+/* The context's HIP stream (a hipStream_t), for kernels a caller compiles
+ * itself, such as the header-only template surface (include/gcurval/gcurval.h)
+ * instantiated on user curve plug-ins. No counterpart in the reference, whose
+ * kernels run on the default stream. */
+int gcv_ctx_stream(gcv_ctx* ctx, void** stream);
 
 /* ---- timing: HIP events on the context stream ---- */
 int gcv_timer_start(gcv_ctx* ctx);
