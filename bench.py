@@ -193,10 +193,22 @@ def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
         t_smp, _ = ctx.profile_query("k_sample")
         ctx.profile(False)
         st = eng.stats()
-        log(f"[bench] {path}: {steps} steps in {ms:.3f} ms; build stats {st}")
-        out[path] = {"ms": ms, "ms_per_step": ms / steps, "kernels": kernels, "stats": st,
-                     "build_ms": (t_all - t_smp) / steps, "sample_ms": t_smp / steps,
-                     "lengths": eng.getAllCurveArclengths()}
+        lengths = eng.getAllCurveArclengths()
+        # the same step captured once and replayed as one graph launch
+        ctx.graph_begin()
+        step()
+        graph = ctx.graph_end()
+        graph.launch()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            graph.launch()
+        graph_ms = ctx.timer_stop()
+        graph.close()
+        log(f"[bench] {path}: {steps} steps in {ms:.3f} ms (graph replays {graph_ms:.3f} ms); build stats {st}")
+        out[path] = {"ms": ms, "ms_per_step": ms / steps, "graph_ms_per_step": graph_ms / steps,
+                     "kernels": kernels, "stats": st, "build_ms": (t_all - t_smp) / steps,
+                     "sample_ms": t_smp / steps, "lengths": lengths}
     eng.close()
     d_off.free()
     d_out.free()
@@ -210,6 +222,7 @@ def comparison(runs, n_int, n_total):
 
     def side(r):
         return {"build_ms": r["build_ms"], "step_ms": r["ms_per_step"], "sample_ms": r["sample_ms"],
+                "graph_step_ms": r["graph_ms_per_step"],
                 "quadratures": r["stats"]["quadratures"], "deep_roots": r["stats"]["deep_roots"],
                 "build_intervals_per_sec": n_int / (r["build_ms"] * 1e-3),
                 "step_intervals_per_sec": n_int / (r["ms_per_step"] * 1e-3),
@@ -309,6 +322,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms / args.steps,
+        "graph_ms_per_step": main_run["graph_ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

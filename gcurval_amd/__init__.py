@@ -109,6 +109,12 @@ _SIGS = [
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("gcv_ctx_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("gcv_graph_begin", C.c_int, [C.c_void_p]),
+    ("gcv_graph_end", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("gcv_graph_launch", C.c_int, [C.c_void_p]),
+    ("gcv_graph_destroy", C.c_int, [C.c_void_p]),
+    ("gcv_table_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    ("gcv_table_set_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
     ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -239,6 +245,15 @@ class Context:
         self._check(lib().gcv_profile_query(self.h, prefix.encode(), C.byref(t), C.byref(n)))
         return t.value, n.value
 
+    def graph_begin(self):
+        """Start capturing this context's stream into a graph."""
+        self._check(lib().gcv_graph_begin(self.h))
+
+    def graph_end(self) -> "Graph":
+        g = C.c_void_p()
+        self._check(lib().gcv_graph_end(self.h, C.byref(g)))
+        return Graph(self, g.value)
+
     def arena_stats(self):
         u, c = C.c_size_t(), C.c_size_t()
         self._check(lib().gcv_arena_stats(self.h, C.byref(u), C.byref(c)))
@@ -254,6 +269,33 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Graph:
+    """A captured sequence of library calls, replayed with one launch."""
+
+    def __init__(self, ctx: "Context", h: int):
+        self.ctx = ctx
+        self.h = h
+
+    def launch(self):
+        self.ctx._check(lib().gcv_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            lib().gcv_graph_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class _Tune(C.Structure):
+    _fields_ = [("sub_waves_per_block", C.c_int), ("sub_grid_pct", C.c_int), ("best_build_ms", C.c_double),
+                ("default_build_ms", C.c_double), ("n_tried", C.c_int)]
 
 
 def make_opts(precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0, tolerance=0.0,
@@ -393,6 +435,17 @@ class ArclengthCalculateGpu:
         if out.nbytes:
             self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(out), p.value, out.nbytes))
         return out
+
+    def autotune(self, mlp=None, reps: int = 3) -> dict:
+        """Pick the persistent subdivision kernel's launch shape for this table
+        by timing builds (learned-seed builds when `mlp` is given)."""
+        t = _Tune()
+        self.ctx._check(lib().gcv_table_autotune(self.h, mlp.h if mlp is not None else None, reps, C.byref(t)))
+        self._lengths = None
+        return {k: getattr(t, k) for k, _ in _Tune._fields_}
+
+    def set_launch(self, waves_per_block: int = 0, grid_pct: int = 100):
+        self.ctx._check(lib().gcv_table_set_launch(self.h, waves_per_block, grid_pct))
 
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)

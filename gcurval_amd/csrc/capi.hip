@@ -50,6 +50,7 @@ struct gcv_ctx {
     std::map<std::string, std::pair<double, int64_t>> prof;
     ProfRec pending;
     bool pending_on = false;
+    bool capturing = false;  // the stream is being captured into a graph
 };
 
 static int set_err(gcv_ctx* c, int code, const char* fmt, ...) {
@@ -63,6 +64,14 @@ static int set_err(gcv_ctx* c, int code, const char* fmt, ...) {
     }
     return code;
 }
+
+// Calls that synchronise the stream are refused while it is being captured.
+static int capture_guard(gcv_ctx* c, const char* what) {
+    if (c && c->capturing)
+        return set_err(c, GCV_ERR_STATE, "%s synchronises the stream: not allowed during graph capture", what);
+    return GCV_OK;
+}
+
 
 #define HIPCHK(ctx, expr)                                                                            \
     do {                                                                                             \
@@ -87,7 +96,7 @@ namespace gcurval {
 namespace rt {
 void prof_begin(void* owner, const char* name) {
     gcv_ctx* c = (gcv_ctx*)owner;
-    if (!c || !c->profiling) return;
+    if (!c || !c->profiling || c->capturing) return;
     c->pending.name = name;
     c->pending.s = take_event(c);
     c->pending.e = take_event(c);
@@ -120,7 +129,7 @@ static void harvest(gcv_ctx* c) {
     c->recs.clear();
 }
 
-static LaunchCfg cfg_of(gcv_ctx* c) { return LaunchCfg{c->stream, c->n_cu, c}; }
+static LaunchCfg cfg_of(gcv_ctx* c) { return LaunchCfg{c->stream, c->n_cu, c, 0, 100}; }
 
 extern "C" {
 
@@ -238,6 +247,11 @@ int gcv_free(gcv_ctx* c, void* p) {
     c->live.erase(it);
     c->used -= sz;
     if (!c->arena) {
+        if (c->capturing) {  // keep the bookkeeping consistent: refuse before forgetting the block
+            c->live[p] = sz;
+            c->used += sz;
+            return capture_guard(c, "gcv_free without an arena");
+        }
         hipStreamSynchronize(c->stream);
         HIPCHK(c, hipFree(p));
         return GCV_OK;
@@ -269,6 +283,7 @@ int gcv_arena_stats(const gcv_ctx* c, size_t* used, size_t* capacity) {
 
 int gcv_copy_h2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
+    if (int rc = capture_guard(c, "gcv_copy_h2d")) return rc;
     if (!bytes) return GCV_OK;
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -277,6 +292,7 @@ int gcv_copy_h2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
 
 int gcv_copy_d2h(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
+    if (int rc = capture_guard(c, "gcv_copy_d2h")) return rc;
     if (!bytes) return GCV_OK;
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -301,6 +317,7 @@ int gcv_copy_d2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
 
 int gcv_sync(gcv_ctx* c) {
     if (!c) return GCV_ERR_INVALID;
+    if (int rc = capture_guard(c, "gcv_sync")) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GCV_OK;
 }
@@ -405,6 +422,8 @@ struct gcv_table {
     int8_t* seed_k = nullptr;             // [n_int] k used by uniform builds
     int8_t* seed_floor = nullptr;         // [n_int] safety floor of seeded builds
     uint16_t* feat_x = nullptr;           // [n_int][16] learned-seed input rows (allocated on first use)
+    int tune_wpb = 0;                     // subdivision launch shape picked by gcv_table_autotune
+    int tune_grid_pct = 100;
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
     int64_t* root_off = nullptr;          // [n_int + 1]
     int32_t* bucket_list = nullptr;       // [n_int]
@@ -615,6 +634,13 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
     return GCV_OK;
 }
 
+static LaunchCfg cfg_of_table(const gcv_table* t) {
+    LaunchCfg L = cfg_of(t->ctx);
+    L.sub_wpb = t->tune_wpb;
+    L.sub_grid_pct = t->tune_grid_pct;
+    return L;
+}
+
 template <typename Dt>
 static dev::SubdivArgs<Dt> subdiv_args(gcv_table* t) {
     dev::SubdivArgs<Dt> A{};
@@ -720,7 +746,7 @@ template <typename Dt, int NGL>
 static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
     hipError_t e = build_setup<Dt, NGL>(t);
     if (e != hipSuccess) return e;
-    const LaunchCfg L = cfg_of(t->ctx);
+    const LaunchCfg L = cfg_of_table(t);
     dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
     A.log2_nsub = ls;
     A.eps0 = eps_of<Dt>(t);
@@ -788,7 +814,7 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         LAUNCH_OR_RETURN("k_root_apply", dev::k_root_apply, dim3(nb), dim3(256), 0, c->stream, n, kk,
                          t->seed_floor, t->scan_blocks, t->root_off);
     }
-    const LaunchCfg L = cfg_of(c);
+    const LaunchCfg L = cfg_of_table(t);
     const BSplineTableDev T = bspline_dev(t);
     for (int ci = 0; ci < n_cls; ci++) {
         for (int k = 0; k <= dev::kMaxSeedK; k++) {
@@ -918,9 +944,123 @@ extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, cons
     return GCV_OK;
 }
 
+// ===========================================================================
+// Graphs: capture the context stream (builds, sampling, predictor) once and
+// replay it with one launch.
+// ===========================================================================
+struct gcv_graph {
+    gcv_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int gcv_graph_begin(gcv_ctx* c) {
+    if (!c) return GCV_ERR_INVALID;
+    if (c->capturing) return set_err(c, GCV_ERR_STATE, "graph capture already in progress");
+    if (c->pending_on) return set_err(c, GCV_ERR_STATE, "a profiled launch is open");
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    return GCV_OK;
+}
+
+extern "C" int gcv_graph_end(gcv_ctx* c, gcv_graph** out) {
+    if (!c || !out) return GCV_ERR_INVALID;
+    if (!c->capturing) return set_err(c, GCV_ERR_STATE, "no graph capture in progress");
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->capturing = false;
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return set_err(c, GCV_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    *out = new gcv_graph{c, g, x};
+    return GCV_OK;
+}
+
+extern "C" int gcv_graph_launch(gcv_graph* g) {
+    if (!g) return GCV_ERR_INVALID;
+    if (g->ctx->capturing) return set_err(g->ctx, GCV_ERR_STATE, "graph launch during capture");
+    HIPCHK(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+    return GCV_OK;
+}
+
+extern "C" int gcv_graph_destroy(gcv_graph* g) {
+    if (!g) return GCV_OK;
+    hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return GCV_OK;
+}
+
+// ===========================================================================
+// Autotuner: times the table build over the persistent subdivision grid's
+// shape (waves per block x resident-grid fraction) and keeps the fastest.
+// ===========================================================================
+extern "C" int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* out) {
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (int rc = capture_guard(c, "gcv_table_autotune")) return rc;
+    if (reps < 1) reps = 3;
+    static const int kWpb[3] = {4, 2, 1};
+    static const int kGrid[3] = {100, 50, 25};
+    const int keep_wpb = t->tune_wpb, keep_grid = t->tune_grid_pct;
+    double best = 1e300, dflt = 0.0;
+    int best_wpb = 4, best_grid = 100, tried = 0;
+    for (int wi = 0; wi < 3; wi++) {
+        for (int gi = 0; gi < 3; gi++) {
+            t->tune_wpb = kWpb[wi];
+            t->tune_grid_pct = kGrid[gi];
+            auto build = [&]() { return m ? gcv_table_build_mlp(t, m, 0, nullptr) : gcv_table_build(t); };
+            int rc = build();  // warm-up
+            if (!rc) rc = gcv_timer_start(c);
+            for (int r = 0; r < reps && !rc; r++) rc = build();
+            double ms = 0.0;
+            if (!rc) rc = gcv_timer_stop(c, &ms);
+            if (rc) {
+                t->tune_wpb = keep_wpb;
+                t->tune_grid_pct = keep_grid;
+                return rc;
+            }
+            ms /= reps;
+            tried++;
+            if (wi == 0 && gi == 0) dflt = ms;
+            if (ms < best) {
+                best = ms;
+                best_wpb = kWpb[wi];
+                best_grid = kGrid[gi];
+            }
+        }
+    }
+    t->tune_wpb = best_wpb;
+    t->tune_grid_pct = best_grid;
+    if (out) {
+        out->sub_waves_per_block = best_wpb;
+        out->sub_grid_pct = best_grid;
+        out->best_build_ms = best;
+        out->default_build_ms = dflt;
+        out->n_tried = tried;
+    }
+    return m ? gcv_table_build_mlp(t, m, 0, nullptr) : gcv_table_build(t);
+}
+
+extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct) {
+    if (!t) return GCV_ERR_INVALID;
+    if (sub_waves_per_block != 0 && sub_waves_per_block != 1 && sub_waves_per_block != 2 && sub_waves_per_block != 4)
+        return set_err(t->ctx, GCV_ERR_INVALID, "waves per block must be 0 (default), 1, 2 or 4");
+    if (sub_grid_pct < 1 || sub_grid_pct > 100) return set_err(t->ctx, GCV_ERR_INVALID, "grid percent in [1, 100]");
+    t->tune_wpb = sub_waves_per_block;
+    t->tune_grid_pct = sub_grid_pct;
+    return GCV_OK;
+}
+
 extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
                                uint64_t* overflow, uint64_t* tasks) {
     if (!t) return GCV_ERR_INVALID;
+    if (int rc = capture_guard(t->ctx, "gcv_table_stats")) return rc;
     if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
     std::vector<unsigned long long> rows(dev::kStatWords);
     int rc = gcv_copy_d2h(t->ctx, rows.data(), t->stats, sizeof(unsigned long long) * dev::kStatWords);
@@ -978,6 +1118,9 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     // block -> curve table (grown on demand; reused while large enough)
     const int64_t n_blk = (n_total + dev::kSampleBlock - 1) / dev::kSampleBlock;
     if (n_blk > t->blk_cap) {
+        if (c->capturing)
+            return set_err(c, GCV_ERR_STATE, "sampling %lld points needs a larger block map: run the call once "
+                           "before capturing it", (long long)n_total);
         if (t->blk_curve) {
             hipStreamSynchronize(c->stream);
             gcv_free(c, t->blk_curve);
@@ -1162,7 +1305,10 @@ static hipError_t setup_and_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t*
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8));
     hipError_t e;
     if (t->kind == KIND_BSPLINE) {
-        if (!t->feat_x && n > 0 && talloc(t, &t->feat_x, n * dev::kFeat)) return hipErrorOutOfMemory;
+        if (!t->feat_x && n > 0) {
+            if (c->capturing) return hipErrorStreamCaptureUnsupported;  // first learned build must run eagerly
+            if (talloc(t, &t->feat_x, n * dev::kFeat)) return hipErrorOutOfMemory;
+        }
         const dev::FeatSetup fs{t->feat_x, t->seed_floor, m->mean, m->istd,
                                 (float)(t->o.tolerance / (double)(1 << t->log2_nsub)), t->log2_nsub};
         e = dispatch_build<SetupWithFeatures>(t, reset_stats, &fs);

@@ -17,7 +17,9 @@ namespace rt {
 struct LaunchCfg {
     hipStream_t stream;
     int n_cu;
-    void* owner;  // gcv_ctx*
+    void* owner;           // gcv_ctx*
+    int sub_wpb = 0;       // subdivision waves per block (0: as many as LDS allows, up to 4)
+    int sub_grid_pct = 100; // persistent subdivision grid, percent of the resident-block limit
 };
 
 // Profiling hooks implemented by the context (events around a launch).
@@ -53,7 +55,7 @@ struct Launch {
     static hipError_t sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
                                      const dev::SampleArgs& A, const int32_t* blk_curve);
     // LDS per block and waves per block of the subdivision kernel.
-    static void subdivide_shape(int stack_cap, int* waves_per_block, size_t* lds_bytes);
+    static void subdivide_shape(int stack_cap, int want_wpb, int* waves_per_block, size_t* lds_bytes);
 };
 
 }  // namespace rt

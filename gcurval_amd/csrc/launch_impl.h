@@ -9,9 +9,9 @@ namespace rt {
 using namespace gcurval::dev;
 
 template <typename Dt, int NGL>
-void Launch<Dt, NGL>::subdivide_shape(int stack_cap, int* waves_per_block, size_t* lds_bytes) {
+void Launch<Dt, NGL>::subdivide_shape(int stack_cap, int want_wpb, int* waves_per_block, size_t* lds_bytes) {
     const int per_wave = subdiv_lds_per_wave<Dt>(stack_cap);
-    int w = 4;
+    int w = (want_wpb == 1 || want_wpb == 2 || want_wpb == 4) ? want_wpb : 4;
     while (w > 1 && w * per_wave > 64 * 1024) w >>= 1;
     *waves_per_block = w;
     *lds_bytes = (size_t)w * per_wave;
@@ -24,7 +24,7 @@ static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivAr
     if (A.n_slots <= 0) return hipSuccess;
     int wpb;
     size_t lds;
-    Launch<Dt, NGL>::subdivide_shape(A.stack_cap, &wpb, &lds);
+    Launch<Dt, NGL>::subdivide_shape(A.stack_cap, c.sub_wpb, &wpb, &lds);
     const int threads = 64 * wpb;
     const int ipw = 64 >> A.log2_nsub;
     const int64_t n_tiles = (A.n_slots + ipw - 1) / ipw;
@@ -39,7 +39,8 @@ static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivAr
         if (per_cu < 1) per_cu = 1;
         if (wpb <= 4 && lds_key < kMaxLdsKey) cached_per_cu[wpb - 1][lds_key] = per_cu;
     }
-    int64_t blocks = (int64_t)c.n_cu * per_cu;
+    int64_t blocks = (int64_t)c.n_cu * per_cu * (c.sub_grid_pct > 0 ? c.sub_grid_pct : 100) / 100;
+    if (blocks < 1) blocks = 1;
     const int64_t need = (n_tiles + wpb - 1) / wpb;
     if (blocks > need) blocks = need;
     prof_begin(c.owner, name);

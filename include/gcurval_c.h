@@ -216,6 +216,38 @@ int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, f
 int gcv_mlp_train_step(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float lr, float w_pass,
                        float* host_loss);
 
+/* ---- graphs: gcv_graph_begin starts capturing the context stream; the
+ * library calls made until gcv_graph_end (builds, learned builds, sampling)
+ * are recorded, not run, and gcv_graph_launch replays them with one launch.
+ * Calls that synchronise (copies to the host, lengths, statistics, gcv_sync)
+ * fail with GCV_ERR_STATE while capturing; run a call once before capturing
+ * it so that its lazily sized buffers exist. The graph keeps the tables and
+ * buffers it was recorded with: they must outlive it. Replaces the
+ * reference's per-level host loop with its synchronisations
+ * (gcurval/detail/cuda/arclength_calculate.h:259-323). ---- */
+typedef struct gcv_graph gcv_graph;
+int gcv_graph_begin(gcv_ctx* ctx);
+int gcv_graph_end(gcv_ctx* ctx, gcv_graph** out);
+int gcv_graph_launch(gcv_graph* g);
+int gcv_graph_destroy(gcv_graph* g);
+
+/* ---- autotuner: builds the table with each launch shape of the persistent
+ * subdivision kernel (1, 2 or 4 waves per block x 100, 50 or 25 % of the
+ * resident-block limit), `reps` timed builds each, keeps the fastest for
+ * this table and rebuilds with it. m: time learned-seed builds with this
+ * predictor (NULL: plain builds). No counterpart in the reference (fixed
+ * launch shapes). ---- */
+typedef struct gcv_tune {
+    int sub_waves_per_block;
+    int sub_grid_pct;
+    double best_build_ms;
+    double default_build_ms; /* 4 waves per block, full grid */
+    int n_tried;
+} gcv_tune;
+int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* out);
+/* Set the launch shape directly (0 waves per block: the default). */
+int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct);
+
 /* Host-side helpers that run without a GPU (used by the CPU test suite). */
 /* Power-basis derivative coefficients of one B-spline span, as computed by
  * the setup kernel (out[d*p + j] multiplies x^j, x in [0,1] across the span). */
