@@ -109,6 +109,7 @@ _SIGS = [
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("gcv_ctx_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("gcv_points", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("gcv_graph_begin", C.c_int, [C.c_void_p]),
     ("gcv_graph_end", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_graph_launch", C.c_int, [C.c_void_p]),
@@ -510,8 +511,7 @@ class ArclengthCalculateGpu:
         where=gcurvalHost returns numpy arrays; gcurvalDevice returns the
         DeviceArray (owned by the caller, free with .free()).
         """
-        if what != gcurvalReverseReturnUs:
-            raise GcvError(4, "gcurvalReverseReturnPoints is not built")
+        points = what == gcurvalReverseReturnPoints
         n = np.ascontiguousarray(n_ptrs_for_curves, dtype=np.int64)
         assert n.size == self.n_curve
         off = np.zeros(self.n_curve + 1, dtype=np.int64)
@@ -520,9 +520,20 @@ class ArclengthCalculateGpu:
         d_off = self.ctx.array(off)
         d_pos = self.ctx.array(np.asarray(sample_position, dtype=np.float64)) if sample_position is not None else None
         d_out = self.ctx.empty(max(total, 1), np.float64)
-        d_seg = self.ctx.empty(max(total, 1), np.int32) if return_segments else None
+        d_seg = self.ctx.empty(max(total, 1), np.int32) if (return_segments or points) else None
         try:
             self.sample_device(d_off, total, d_pos, is_relative, d_out, d_seg, uniform_closed)
+            if points:
+                dim = 2 if self.batch.kind == "ellipse" else int(self.batch.ctrl_stride)
+                d_pts = self.ctx.empty(max(total, 1) * dim, np.float64)
+                self.ctx._check(lib().gcv_points(self.h, d_off.ptr, total, d_out.ptr, d_seg.ptr, dim, d_pts.ptr))
+                d_out.free()
+                d_seg.free()
+                if where == gcurvalDevice:
+                    return d_pts
+                pts = d_pts.download()[:total * dim].reshape(total, dim)
+                d_pts.free()
+                return pts
             if where == gcurvalDevice:
                 return (d_out, d_seg) if return_segments else d_out
             u = d_out.download()[:total]

@@ -662,6 +662,16 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
     return BSplineTableDev{t->coef64, t->coef32, t->cstride, t->int_scale, t->u0, t->h};
 }
 
+#define LAUNCH_OR_RETURN_STATUS(name, ...)                                                              \
+    do {                                                                                               \
+        prof_begin(c, name);                                                                           \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                               \
+        hipError_t _le = hipGetLastError();                                                            \
+        prof_end(c);                                                                                   \
+        if (_le != hipSuccess) return set_err(c, GCV_ERR_HIP, "%s launch failed: %s", name,            \
+                                              hipGetErrorString(_le));                                 \
+    } while (0)
+
 #define LAUNCH_OR_RETURN(name, ...)            \
     do {                                       \
         prof_begin(c, name);                   \
@@ -941,6 +951,29 @@ extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, cons
     if (deep_mask) *deep_mask = t->deep;
     if (curve_len) *curve_len = t->curve_len;
     if (int_off) *int_off = t->ioff;
+    return GCV_OK;
+}
+
+extern "C" int gcv_points(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_u,
+                          const int32_t* d_seg, int out_dim, double* d_out) {
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (n_total < 0) return set_err(c, GCV_ERR_INVALID, "n_total < 0");
+    if (n_total == 0) return GCV_OK;
+    if (!d_sample_off || !d_u || !d_out) return set_err(c, GCV_ERR_INVALID, "null sample offsets, parameters or output");
+    if (t->n_curve == 0) return set_err(c, GCV_ERR_INVALID, "points requested from an empty batch");
+    const int need = t->kind == KIND_ELLIPSE ? 2 : t->bat.ctrl_stride;
+    if (out_dim < need || out_dim > 3) return set_err(c, GCV_ERR_INVALID, "out_dim must be in [%d, 3]", need);
+    const unsigned blocks = (unsigned)((n_total + 255) / 256);
+    if (t->kind == KIND_ELLIPSE) {
+        LAUNCH_OR_RETURN_STATUS("k_points", dev::k_points_ellipse, dim3(blocks), dim3(256), 0, c->stream, t->n_curve,
+                                n_total, d_sample_off, t->a, t->b, d_u, out_dim, d_out);
+    } else {
+        if (!d_seg) return set_err(c, GCV_ERR_INVALID, "B-spline points need the sampler's interval indices");
+        LAUNCH_OR_RETURN_STATUS("k_points", dev::k_points_bspline, dim3(blocks), dim3(256), 0, c->stream, t->n_curve,
+                                n_total, d_sample_off, t->bat.degree, t->bat.dim, t->bat.knot_off, t->bat.ctrl_off,
+                                t->bat.knots, t->bat.ctrl, t->bat.ctrl_stride, d_u, d_seg, out_dim, d_out);
+    }
     return GCV_OK;
 }
 

@@ -9,6 +9,7 @@
 #include "../../include/gcurval/detail/kernels.h"
 #include "../../include/gcurval/detail/seeded.h"
 #include "../../include/gcurval/detail/mlp.h"
+#include "../../include/gcurval/detail/points.h"
 
 namespace gcurval {
 namespace rt {

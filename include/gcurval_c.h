@@ -179,6 +179,15 @@ int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, u
 int gcv_sample(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_pos,
                int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg);
 
+/* Curve points at sampled parameters: the gcurvalReverseReturnPoints variant
+ * of arclengthReverse (kn_toPoints, gcurval/detail/cuda/kernel/reverse.inl.cuh:344-363;
+ * cuda/arclength_calculate.h:458-500). d_u and d_seg are gcv_sample's outputs
+ * for the same d_sample_off (d_seg may be NULL for ellipses); d_out: device
+ * double [n_total][out_dim], raw coordinates, out_dim 2..3 (at least the
+ * batch's ctrl_stride for B-splines; missing components are 0). */
+int gcv_points(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_u,
+               const int32_t* d_seg, int out_dim, double* d_out);
+
 /* ---- learned seed: a 16-64-64-2 MLP on BF16 matrix cores (FP32 accumulate)
  * predicting, per knot interval, log2 of the initial root count and whether
  * every root of that seed passes the level-0 error test. No counterpart in the
