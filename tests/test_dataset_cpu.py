@@ -1,0 +1,60 @@
+"""CPU tests of the B-spline dataset loader on the committed fixture (the first
+300 curves of dataset/BSpline/Bundle_0.txt, verbatim) and the whole-dataset
+totals recorded when the fixture was made (tests/golden/make_dataset_fixture.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gcurval_amd import dataset as D
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "dataset_bundle0_head.txt")
+
+
+def test_parse_fixture_records():
+    with open(FIX) as f:
+        curves = D.parse_curves(f.read())
+    assert len(curves) == 300
+    for p, x, y, t in curves:
+        assert p == 3 and x.size == y.size and t.size == x.size + p + 1
+        assert np.all(np.diff(t) >= 0) and t[0] == 0.0 and t[-1] == 1.0
+    # first record of the file, read back exactly
+    p, x, y, t = curves[0]
+    np.testing.assert_array_equal(x, [9.6512, 12.3463, 13.0744, 12.4825])
+    np.testing.assert_array_equal(y, [14.5618, 8.82678, 8.5968, 12.4963])
+    np.testing.assert_array_equal(t, [0, 0, 0, 0, 1, 1, 1, 1])
+
+
+@pytest.mark.parametrize("f32", [True, False])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_loader_normalisation(f32, dim):
+    b = D.load_bspline_dataset([FIX], dim=dim, f32=f32)
+    assert b.n_curve == 300 and b.ctrl_stride == dim
+    assert b.n_intervals == int(np.sum(np.diff(b.knot_off) - 7))
+    for c in range(b.n_curve):
+        P = b.ctrl[b.ctrl_off[c]:b.ctrl_off[c + 1]]
+        assert np.max(np.abs(P[:, :2])) == 1.0          # divided by its own largest |x|, |y|
+        assert np.all(np.abs(P[:, :2].mean(axis=0)) <= (1e-6 if f32 else 1e-14))
+        if dim == 3:
+            assert np.all(P[:, 2] == 0.0)
+        if f32:
+            np.testing.assert_array_equal(P, P.astype(np.float32).astype(np.float64))
+
+
+def test_loader_limits_and_errors(tmp_path):
+    assert D.load_bspline_dataset([FIX], n_curve=17).n_curve == 17
+    bad = tmp_path / "bad.txt"
+    bad.write_text("3 4 7\n1 2 3 4\n1 2 3 4\n0 0 0 0 1 1 1\n")
+    with pytest.raises(ValueError):
+        D.load_bspline_dataset([str(bad)])
+    bad.write_text("3 4 8\n1 2 3 4\n1 2\n")
+    with pytest.raises(ValueError):
+        D.load_bspline_dataset([str(bad)])
+
+
+def test_whole_dataset_totals_recorded():
+    meta = json.load(open(os.path.join(HERE, "golden", "dataset_meta.json")))
+    # the reference's dataset: 100 files x 1000 curves, cubic, 499,339 knot intervals
+    assert meta["n_curve"] == 100000 and meta["n_intervals"] == 499339 and meta["degrees"] == [3]
