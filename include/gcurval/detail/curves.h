@@ -129,6 +129,32 @@ GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, co
 
 #if defined(__HIPCC__)
 
+// 1 / v for finite nonzero normal v, within 1 ulp: v_rcp_f64 refined by two
+// Newton steps (the IEEE-exact division adds scaling and fix-up steps that
+// start values, scales and Newton updates do not need).
+__device__ __forceinline__ double rcp_f64(double v) {
+    double r = __builtin_amdgcn_rcp(v);
+    double e = fma(-v, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-v, r, 1.0);
+    return fma(r, e, r);
+}
+
+// sqrt(v) for v >= 0, within 1 ulp for v >= 2^-1000 (0 below): v_rsq_f64
+// refined by one coupled Goldschmidt step on (g, h) ~ (sqrt v, 1/(2 sqrt v))
+// and one Newton correction g += (v - g^2) h with the residual from an FMA.
+// Speeds squared below 2^-1000 contribute nothing to an arc length, so the
+// denormal scaling of a general square root is left out.
+__device__ __forceinline__ double sqrt_pos(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    double g = v * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, v), h, g);
+    return v >= 0x1p-1000 ? g : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // sin^2(2 pi x) and cos^2(2 pi x) for x in [0, 1] from one polynomial: r = 4x
 // and its nearest integer q are exact, f = r - q lies in [-1/2, 1/2], and with
@@ -169,7 +195,7 @@ __device__ __forceinline__ void sincos2_2pi(double x, double& s2, double& c2) {
     u = fma(u, g, -2.0293560632083842);
     u = fma(u, g, 2.4674011002723395);
     const double lo = g * u, hi = 1.0 - lo;
-    const bool odd = ((long long)q) & 1;
+    const bool odd = ((int)q) & 1;  // |q| <= 4 on the domain
     s2 = odd ? hi : lo;
     c2 = odd ? lo : hi;
 }
@@ -194,7 +220,7 @@ struct EllipseEval {
         double s = fmax(fabs(a[i]), fabs(b[i]));
         return s > 0.0 ? s : 1.0;
     }
-    __device__ __forceinline__ Ctx load(int i) const { return load(i, 1.0 / scale(i)); }
+    __device__ __forceinline__ Ctx load(int i) const { return load(i, rcp_f64(scale(i))); }
     // with the reciprocal of scale(i) already at hand
     __device__ __forceinline__ Ctx load(int i, double inv_scale) const {
         const double A = kTwoPi * a[i] * inv_scale, B = kTwoPi * b[i] * inv_scale;
@@ -208,7 +234,7 @@ struct EllipseEval {
     __device__ __forceinline__ double speed(const Ctx& c, double x) const {
         double s2, c2;
         sincos2_2pi(x, s2, c2);
-        return sqrt(fma(double(c.A2), s2, double(c.B2) * c2));
+        return sqrt_pos(fma(double(c.A2), s2, double(c.B2) * c2));
     }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }
@@ -254,7 +280,7 @@ struct BSplineEval {
         return sq(s);
     }
     __device__ __forceinline__ static float sq(float v) { return __builtin_amdgcn_sqrtf(v); }
-    __device__ __forceinline__ static double sq(double v) { return sqrt(v); }
+    __device__ __forceinline__ static double sq(double v) { return sqrt_pos(v); }
     __device__ __forceinline__ double to_u(int i, double x) const { return u0[i] + h[i] * x; }
 };
 

@@ -99,6 +99,15 @@ __host__ __device__ inline int effective_depth(int max_depth, int ls_plain, int 
 // Bounds of task (root kr, level l, index idx) in interval-local x; exact.
 template <typename Dt>
 __device__ __forceinline__ void task_bounds(int kr, int l, uint32_t idx, int ls, Dt& x0, Dt& xm, Dt& x1) {
+    if constexpr (sizeof(Dt) == 4) {
+        // FP32 depths are capped at ls + l <= 24 (effective_depth), so the
+        // numerators fit 32 bits and convert in one instruction
+        const uint32_t base = ((uint32_t)kr << l) + idx;
+        x0 = Dt(base) * pow2m(Dt(0), ls + l);
+        x1 = Dt(base + 1u) * pow2m(Dt(0), ls + l);
+        xm = Dt(2u * base + 1u) * pow2m(Dt(0), ls + l + 1);
+        return;
+    }
     const uint64_t base = ((uint64_t)kr << l) + idx;
     x0 = dyadic<Dt>(base, ls + l);
     x1 = dyadic<Dt>(base + 1, ls + l);
@@ -433,20 +442,30 @@ __device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t lo,
     }
     return lo;
 }
-// first index in [lo, hi) with a[idx] >= v (hi if none)
-__device__ __forceinline__ int64_t lower_bound_f64(const double* a, int64_t lo, int64_t hi, double v) {
-    int64_t cnt = hi - lo;
-    while (cnt > 0) {
-        const int64_t h = cnt >> 1;
-        if (a[lo + h] < v) { lo += h + 1; cnt -= h + 1; } else cnt = h;
+
+// Number of entries of the non-decreasing a[0, n) below v, i.e. the first
+// index with a[idx] >= v (n if none); 32-bit offsets from a 64-bit base.
+__device__ __forceinline__ uint32_t count_below(const double* a, uint32_t n, double v) {
+    uint32_t lo = 0;
+    while (n > 0) {
+        const uint32_t h = n >> 1;
+        if (a[lo + h] < v) { lo += h + 1; n -= h + 1; } else n = h;
     }
     return lo;
 }
+// The same over a power-of-two count 2^ls, clamped to 2^ls - 1: one probe per
+// bit of the answer, no data-dependent trip count.
+__device__ __forceinline__ uint32_t count_below_pow2(const double* a, int ls, double v) {
+    uint32_t k = 0;
+    for (uint32_t st = (1u << ls) >> 1; st > 0; st >>= 1)
+        if (a[k + st - 1] < v) k += st;
+    return k;
+}
 
 // Reciprocal for Newton updates and start values: one v_rcp (1 ulp) in float,
-// exact division in double.
+// v_rcp_f64 with two Newton steps (1 ulp) in double.
 __device__ __forceinline__ float nrcp(float v) { return __builtin_amdgcn_rcpf(v); }
-__device__ __forceinline__ double nrcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ double nrcp(double v) { return rcp_f64(v); }
 
 // Quadrature of speed over [a, a + d], with the width d carried exactly.
 template <int N, class Ev, typename Dt>
@@ -494,7 +513,7 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
     } else {
         d = w * t;
     }
-    d = d < Dt(0) ? Dt(0) : (d > w ? w : d);
+    d = d > Dt(0) ? (d < w ? d : w) : Dt(0);  // also maps a NaN start to 0
     *v_out = va;
     if (iters <= 0) return d;
     Dt lo = Dt(0), hi = w;
@@ -567,15 +586,18 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         pos = A.pos[g];
     } else {
         const int64_t N = A.soff[c + 1] - s0, j = g - s0;
-        pos = A.uniform_closed ? (N > 1 ? (double)j * (1.0 / (double)(N - 1)) : 0.0) : (double)j * (1.0 / (double)N);
+        // j / (N - 1) (closed) or j / N to within 1.5 ulp; a position a rounding
+        // above 1 is clamped to the curve's end by the searches below
+        pos = A.uniform_closed ? (N > 1 ? (double)j * rcp_f64((double)(N - 1)) : 0.0) : (double)j * rcp_f64((double)N);
     }
     const double s = A.is_relative ? pos * A.curve_len[c] : pos;
     const int64_t ib = A.ioff[c], ie = A.ioff[c + 1];
-    int64_t i = lower_bound_f64(A.int_cum, ib, ie, s);
-    if (i >= ie) i = ie - 1;
+    // interval: first cumulative length >= s among the curve's first
+    // n_int - 1, so a target past the last boundary lands in the last interval
+    const int64_t i = ib + count_below(A.int_cum + ib, (uint32_t)(ie - ib - 1), s);
     if (seg_out) *seg_out = (int32_t)(i - ib);
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
-    const double inv_sc = 1.0 / ev.scale((int)i);
+    const double inv_sc = rcp_f64(ev.scale((int)i));
     const double sn = sl * inv_sc;
     int ls = A.log2_nsub;
     int64_t ro = (int64_t)i << ls;
@@ -584,12 +606,10 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         ro = A.root_off[i];
         ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
     }
-    const int nsub = 1 << ls;
-    int64_t kk = lower_bound_f64(A.root_prefix, ro, ro + nsub, sn) - ro;
-    if (kk >= nsub) kk = nsub - 1;
-    const int k = (int)kk;
-    const double pre = k > 0 ? A.root_prefix[ro + k - 1] : 0.0;
-    const double rlen = A.root_prefix[ro + k] - pre;
+    const double* rp = A.root_prefix + ro;
+    const int k = (int)count_below_pow2(rp, ls, sn);
+    const double pre = k > 0 ? rp[k - 1] : 0.0;
+    const double rlen = rp[k] - pre;
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
 
