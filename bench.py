@@ -4,11 +4,12 @@
 A step is one pass of the hot path over the whole synthetic batch: the table
 build (setup, adaptive Gauss-Legendre subdivision, cumulative scan) followed by
 arc-length-uniform sampling of every curve, with all inputs resident in HBM.
-`value` is curve-intervals per second of step time with the learned-seed
-build (the MLP predicts each knot interval's initial root count; the adaptive
-error test still verifies every root); samples per second of the same steps
-are reported beside it, and the plain 64-way build is timed on the same
-inputs ("seed_comparison").
+`value` is curve-intervals per second of step time with the build path the
+workload is defined on: the plain 64-way build for the 1M-ellipse workload,
+the learned-seed build (the MLP predicts each knot interval's initial root
+count; the adaptive error test still verifies every root) for the B-spline
+workloads. Samples per second of the same steps are reported beside it, and
+the other build path is timed on the same inputs ("seed_comparison").
 
 Default workload: 1,000,000 ellipses, a ~ U[0.5, 10], b = a U[0.01, 1], one
 knot interval each, FP32 quadrature with FP64 scan, 64..4096 samples per curve
@@ -279,8 +280,10 @@ def main():
     ap.add_argument("--config", default=os.environ.get("GCV_BENCH_CONFIG", "c1"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--path", choices=("plain", "learned"), default="learned",
-                    help="build path whose throughput is `value` (the other is timed beside it)")
+    ap.add_argument("--path", choices=("plain", "learned"), default=None,
+                    help="build path whose throughput is `value` (the other is timed beside it); default: "
+                         "plain for the 1M-ellipse workload, which is defined on the plain method, "
+                         "learned for the B-spline workloads")
     ap.add_argument("--no-compare", action="store_true", help="time only --path")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the learned-vs-plain check on the 4M-interval cubic B-spline workload")
@@ -306,6 +309,8 @@ def main():
     log(f"[bench] workload {cfg['workload']}: {batch.n_curve} curves, {n_int} intervals, "
         f"{n_total} samples (generated in {time.time() - t_gen:.1f}s)")
 
+    if args.path is None:
+        args.path = "plain" if args.config == "c1" else "learned"
     ctx = G.Context(0)
     n_cu, arch = ctx.device_info()
     other = "plain" if args.path == "learned" else "learned"

@@ -475,7 +475,9 @@ static int resolve_opts(gcv_ctx* c, const gcv_opts* in, gcv_opts* o) {
         return set_err(c, GCV_ERR_INVALID, "max_depth must be in [0, %d]", maxd);
     if (o->tolerance == 0.0) o->tolerance = o->precision == GCV_FP32 ? 1e-6 : 1e-14;
     if (!(o->tolerance > 0.0)) return set_err(c, GCV_ERR_INVALID, "tolerance must be positive");
-    if (o->newton_iters == 0) o->newton_iters = o->precision == GCV_FP32 ? 2 : 3;
+    // FP32 with the FP64 step: one FP32 Newton step brings the lane into the
+    // quadratic regime the FP64 step finishes; lanes not there yet keep going
+    if (o->newton_iters == 0) o->newton_iters = o->precision == GCV_FP32 ? (o->skip_fp64_step ? 2 : 1) : 3;
     if (o->newton_iters < 0 || o->newton_iters > 32) return set_err(c, GCV_ERR_INVALID, "newton_iters out of range");
     return GCV_OK;
 }

@@ -492,12 +492,17 @@ __device__ __forceinline__ Dt quad_from_n(const Ev& ev, const typename Ev::Ctx& 
 // interpolant of the inverse map x(s) (x' = 1 / speed at both ends); the first
 // Newton step evaluates the integral by quadrature, later steps update it by
 // the trapezoid rule over the (quadratically shrinking) step.
+// After the `iters` steps a lane whose last step still exceeded 2^-14 (FP32)
+// or 2^-26 (FP64) of the leaf width is not yet in Newton's quadratic regime
+// (a poor start next to a near-zero speed): it keeps iterating, with the
+// residual recomputed by quadrature, up to kExtraNewton more steps.
 // Returns d; *v_out receives the speed at the last evaluated point (a usable
 // Newton derivative for a final step at d).
 // `pieces` = 2 for leaves wider than 1/64 of their knot interval: the error
 // test bounds a leaf's quadrature error by tol x (leaf width / interval
 // width), so halving keeps the partial integrals within tol / 64 as for the
 // plain method's 64-way pre-split.
+constexpr int kExtraNewton = 16;
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
                                           int iters, int pieces, Dt* v_out) {
@@ -520,18 +525,22 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
     Dt F = quad_from_n<NGL>(ev, c, a, d, pieces) - tau;
     Dt v = ev.speed(c, a + d);
     *v_out = v;
-    for (int it = 0; it < iters; it++) {
+    const Dt conv = w * (sizeof(Dt) == 4 ? Dt(0x1p-14) : Dt(0x1p-26));
+    for (int it = 0;; it++) {
         if (F > Dt(0)) hi = d; else lo = d;
         Dt dn = v > Dt(0) ? d - F * nrcp(v) : Dt(0.5) * (lo + hi);
         if (!(dn >= lo && dn <= hi)) dn = Dt(0.5) * (lo + hi);
-        if (it + 1 == iters) return dn;
+        const bool past = it + 1 >= iters;
+        if (past && (dabs(dn - d) <= conv || it + 1 >= iters + kExtraNewton)) return dn;
         const Dt vn = ev.speed(c, a + dn);
-        F += (dn - d) * (v + vn) * Dt(0.5);
+        if (past)
+            F = quad_from_n<NGL>(ev, c, a, dn, pieces) - tau;
+        else
+            F += (dn - d) * (v + vn) * Dt(0.5);
         d = dn;
         v = vn;
         *v_out = v;
     }
-    return d;
 }
 
 // Re-walk the adaptive tree of a refined root in the build's arithmetic and

@@ -50,7 +50,8 @@ typedef struct gcv_opts {
     int max_depth;     /* 0 = default; <= 18 (FP32), <= 31 (FP64) */
     double tolerance;  /* 0 = default; per-interval absolute tolerance */
     int normalize;     /* 1: test each curve at unit max-abs coordinate (lengths rescaled) */
-    int newton_iters;  /* 0 = default: Newton steps after the Hermite start (2 FP32 + 1 FP64; 3 in FP64 mode) */
+    int newton_iters;  /* 0 = default: Newton steps after the Hermite start (FP32 mode: 1, or 2 without the FP64
+                          step, then 1 FP64 step; FP64 mode: 3); lanes whose last step is still large go on */
     int skip_fp64_step; /* FP32 mode: 1 = no final FP64 Newton step (default 0: the step is taken) */
     int reserved[5];
 } gcv_opts;
