@@ -12,6 +12,8 @@
 
 #include <stdint.h>
 #include <math.h>
+#include <type_traits>
+#include <utility>
 
 #ifndef GCURVAL_HD
 #if defined(__HIPCC__)
@@ -267,6 +269,17 @@ struct BSplineEval {
     }
     // coefficients are stored pre-normalised
     __device__ __forceinline__ Ctx load(int i, double) const { return load(i); }
+    // the context of an evaluator over the same float coefficients, widened
+    // (exact: each value is the stored float); only for Ct = float
+    template <class C32, typename C = Ct,
+              typename = typename std::enable_if<std::is_same<C, float>::value &&
+                                                 sizeof(((C32*)nullptr)->c) == sizeof(float) * P * D>::type>
+    __device__ __forceinline__ static Ctx widen(const C32& c32) {
+        Ctx k;
+#pragma unroll
+        for (int j = 0; j < P * D; j++) k.c[j] = Dt(c32.c[j]);
+        return k;
+    }
     template <typename T>
     __device__ __forceinline__ T speed(const Ctx& k, T x) const {
         T s = T(0);

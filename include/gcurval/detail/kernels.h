@@ -151,20 +151,35 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     int max_stack = 0, overflow = 0;
 
     // Software pipeline over this wave's tiles: the interval id of tile t + 2
-    // and the evaluator context of tile t + 1 are in flight while tile t
-    // computes, so the dependent global loads (list -> coefficients) overlap
-    // the quadratures instead of stalling every tile.
+    // and the evaluator context, root base and scale of tile t + 1 are in
+    // flight while tile t computes, so the dependent global loads (list ->
+    // coefficients) overlap the quadratures instead of stalling every tile.
+    // (one unconditional-address load per form: a conditional load whose
+    // register another path writes directly makes the compiler wait for every
+    // load in flight, the prefetches included)
     auto interval_of = [&](int64_t tl, bool& ok) -> int {
         const int64_t slot = (tl << ipw_log2) + jslot;
         ok = slot < n_slots;
-        return ok ? (list ? list[slot] : (int)slot) : (list ? list[0] : 0);
+        const int64_t s = ok ? slot : 0;
+        int r = (int)s;
+        if (list) r = list[s];
+        return r;
+    };
+    auto root_base = [&](int iv) -> int64_t {
+        int64_t r = (int64_t)iv << ls;
+        if (A.root_off) r = A.root_off[iv];
+        return r;
     };
     bool v_cur = false, v_nxt = false;
     int i_cur = 0, i_nxt = 0;
     typename Ev::Ctx ctx_cur{};
+    int64_t rb_cur = 0;
+    double sc_cur = 1.0;
     if (gwave < n_tiles) {
         i_cur = interval_of(gwave, v_cur);
         ctx_cur = ev.load(i_cur);
+        rb_cur = root_base(i_cur);
+        sc_cur = ev.scale(i_cur);
     }
     if (gwave + nwaves < n_tiles) i_nxt = interval_of(gwave + nwaves, v_nxt);
 
@@ -172,15 +187,24 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         const bool valid = v_cur;
         const int i = i_cur;
         const typename Ev::Ctx ctx = ctx_cur;
+        const int64_t rbase = rb_cur;
+        const double scale_i = sc_cur;
         typename Ev::Ctx ctx_nxt{};
-        if (tile + nwaves < n_tiles) ctx_nxt = ev.load(i_nxt);
+        int64_t rb_nxt = 0;
+        double sc_nxt = 1.0;
+        if (tile + nwaves < n_tiles) {
+            ctx_nxt = ev.load(i_nxt);
+            rb_nxt = root_base(i_nxt);
+            sc_nxt = ev.scale(i_nxt);
+        }
         bool v_nn = false;
         int i_nn = 0;
         if (tile + 2 * nwaves < n_tiles) i_nn = interval_of(tile + 2 * nwaves, v_nn);
-        const int64_t rbase = A.root_off ? A.root_off[i] : ((int64_t)i << ls);
         i_cur = i_nxt;
         v_cur = v_nxt;
         ctx_cur = ctx_nxt;
+        rb_cur = rb_nxt;
+        sc_cur = sc_nxt;
         i_nxt = i_nn;
         v_nxt = v_nn;
 
@@ -263,7 +287,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
             A.root_prefix[rbase + k] = pre;
             if (k == nsub - 1) {
                 if (A.int_cost) A.int_cost[i] = (int32_t)cost;
-                const double L_int = pre * ev.scale(i);
+                const double L_int = pre * scale_i;
                 A.int_len[i] = L_int;
                 if (A.curve_len) A.curve_len[i] = L_int;
                 const uint64_t bits = (ls == 6) ? fmask : ((fmask >> (jslot << ls)) & ((1ull << nsub) - 1ull));
@@ -586,6 +610,20 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
     a = x0; b = x1; tau_leaf = 0.0; leaf_len = Dt(0);
 }
 
+// Context of the FP64 Newton step: widened from the FP32 context where that
+// is exact (B-spline coefficients stored as float), else loaded.
+template <class Ev64, class Ctx32, class = void>
+struct HasWiden : std::false_type {};
+template <class Ev64, class Ctx32>
+struct HasWiden<Ev64, Ctx32, decltype((void)Ev64::widen(std::declval<const Ctx32&>()))> : std::true_type {};
+template <class Ev64, class Ctx32>
+__device__ __forceinline__ typename Ev64::Ctx widen_or_load(const Ev64& ev64, const Ctx32& c32, int i, double inv_sc) {
+    if constexpr (HasWiden<Ev64, Ctx32>::value)
+        return Ev64::widen(c32);
+    else
+        return ev64.load(i, inv_sc);
+}
+
 template <typename Dt, int NGL, class Ev, class Ev64>
 __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
                                              int64_t g, int32_t* seg_out) {
@@ -608,6 +646,9 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
     const double inv_sc = rcp_f64(ev.scale((int)i));
     const double sn = sl * inv_sc;
+    // the evaluator context depends on the interval only: its loads are in
+    // flight during the root search
+    const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
     int ls = A.log2_nsub;
     int64_t ro = (int64_t)i << ls;
     const Dt eps0 = (Dt)A.eps0;
@@ -622,7 +663,6 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
 
-    const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
     Dt a, b, xm;
     task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
     Dt len = (Dt)rlen;
@@ -640,7 +680,7 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         // one FP64 Newton step on the same leaf: the residual is evaluated in
         // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
         // only scales a residual that is already at FP32 rounding level)
-        const typename Ev64::Ctx c64 = ev64.load((int)i, inv_sc);
+        const typename Ev64::Ctx c64 = widen_or_load(ev64, ctx, (int)i, inv_sc);
         const double ad = (double)a, wd = (double)b - (double)a;
         const double F = quad_from_n<NGL>(ev64, c64, ad, dd, pieces) - tau;
         if (v_last > Dt(0)) {
