@@ -427,7 +427,9 @@ struct gcv_table {
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
     int64_t* root_off = nullptr;          // [n_int + 1]
     int32_t* bucket_list = nullptr;       // [n_int]
-    int32_t* bucket_hist = nullptr;               // [kMaxBins * kBucketBlocks] per-block bin counts
+    int32_t* bucket_hist = nullptr;       // [kMaxBins * kBucketMaxWaves] per-wave bin counts
+    int8_t* bucket_bin = nullptr;         // [n_int] bin of each interval
+    int32_t* bucket_totals = nullptr;     // [kMaxBins] intervals per bin
     int64_t* bucket_ranges = nullptr;     // [2 kMaxBins]
     int64_t* scan_blocks = nullptr;       // block sums of the root-offset scan
     int64_t n_scan_blocks = 0;
@@ -503,7 +505,8 @@ static int table_alloc_common(gcv_table* t) {
     t->n_scan_blocks = (t->n_int + 256 * dev::kScanItems - 1) / (256 * dev::kScanItems);
     if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
         (rc = talloc(t, &t->root_off, t->n_int + 1)) ||
-        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketBlocks)) ||
+        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketMaxWaves)) ||
+        (rc = talloc(t, &t->bucket_bin, t->n_int)) || (rc = talloc(t, &t->bucket_totals, dev::kMaxBins)) ||
         (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) ||
         (rc = talloc(t, &t->scan_blocks, t->n_scan_blocks + 1)))
         return rc;
@@ -810,18 +813,23 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
                              t->seed_floor);
         }
         dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve};
-        const int nblk = (int)std::min<int64_t>(dev::kBucketBlocks, (n + 2047) / 2048);
-        const int64_t chunk = (n + nblk - 1) / nblk;
-        LAUNCH_OR_RETURN("k_bucket_hist", dev::k_bucket_hist, dim3(nblk), dim3(256), 0, c->stream, n, BA, n_bins,
-                         chunk, t->bucket_hist);
-        LAUNCH_OR_RETURN("k_bucket_scan", dev::k_bucket_scan, dim3(1), dim3(1024), 0, c->stream, n_bins, nblk,
-                         t->bucket_hist, t->bucket_ranges);
-        LAUNCH_OR_RETURN("k_bucket_place", dev::k_bucket_place, dim3(nblk), dim3(256), 0, c->stream, n, BA, n_bins,
-                         chunk, (const int32_t*)t->bucket_hist, t->bucket_list);
+        // wave chunks of kBucketChunk intervals, longer when that would exceed kBucketMaxWaves
+        const int64_t wchunk = std::max<int64_t>(
+            dev::kBucketChunk, ((n + dev::kBucketMaxWaves - 1) / dev::kBucketMaxWaves + 63) / 64 * 64);
+        const int nw = (int)((n + wchunk - 1) / wchunk);
+        const unsigned gb = (unsigned)((nw + 3) / 4);
+        LAUNCH_OR_RETURN("k_bucket_hist", dev::k_bucket_hist, dim3(gb), dim3(256), 0, c->stream, n, BA, n_bins,
+                         wchunk, nw, t->bucket_bin, t->bucket_hist);
+        LAUNCH_OR_RETURN("k_bucket_totals", dev::k_bucket_totals, dim3((unsigned)n_bins), dim3(1024), 0, c->stream,
+                         nw, (const int32_t*)t->bucket_hist, t->bucket_totals);
+        LAUNCH_OR_RETURN("k_bucket_rows", dev::k_bucket_rows, dim3((unsigned)n_bins), dim3(1024), 0, c->stream, nw,
+                         t->bucket_hist, (const int32_t*)t->bucket_totals, t->bucket_ranges);
+        LAUNCH_OR_RETURN("k_bucket_place", dev::k_bucket_place, dim3(gb), dim3(256), 0, c->stream, n, n_bins,
+                         wchunk, nw, (const int8_t*)t->bucket_bin, (const int32_t*)t->bucket_hist, t->bucket_list);
         const unsigned nb = (unsigned)t->n_scan_blocks;
         LAUNCH_OR_RETURN("k_root_block_sums", dev::k_root_block_sums, dim3(nb), dim3(256), 0, c->stream, n, kk,
                          t->seed_floor, t->scan_blocks);
-        LAUNCH_OR_RETURN("k_scan_block_sums", dev::k_scan_block_sums, dim3(1), dim3(256), 0, c->stream,
+        LAUNCH_OR_RETURN("k_scan_block_sums", dev::k_scan_block_sums, dim3(1), dim3(1024), 0, c->stream,
                          (int64_t)nb, t->scan_blocks);
         LAUNCH_OR_RETURN("k_root_apply", dev::k_root_apply, dim3(nb), dim3(256), 0, c->stream, n, kk,
                          t->seed_floor, t->scan_blocks, t->root_off);

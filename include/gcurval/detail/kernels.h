@@ -363,19 +363,48 @@ static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B
     for (int64_t c = gwave; c < B.n_curve; c += nwaves) {
         double sc = 1.0;
         if (normalize) {
+            // two passes (centroid, then max |coordinate - centroid|), each
+            // with 4 x 64 control points in flight per wave iteration: a
+            // 2048-point curve is then a few latency rounds, not 32
             const int D = B.dim[c];
             const int64_t p0 = B.ctrl_off[c], p1 = B.ctrl_off[c + 1];
+            const int64_t st = B.ctrl_stride;
             double cen[3] = {0, 0, 0};
-            for (int64_t j = p0 + lane; j < p1; j += 64)
-                for (int d = 0; d < D; d++) cen[d] += B.ctrl[j * B.ctrl_stride + d];
-            for (int d = 0; d < D; d++) {
+            for (int64_t j0 = p0 + lane; j0 < p1; j0 += 256) {
+                double x[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const int64_t j = j0 + 64 * u;
+                        x[u][d] = (j < p1 && d < D) ? B.ctrl[j * st + d] : 0.0;
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) cen[d] += x[u][d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
                 double v = cen[d];
                 for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
                 cen[d] = v / (double)(p1 - p0);
             }
             double m = 0.0;
-            for (int64_t j = p0 + lane; j < p1; j += 64)
-                for (int d = 0; d < D; d++) m = fmax(m, fabs(B.ctrl[j * B.ctrl_stride + d] - cen[d]));
+            for (int64_t j0 = p0 + lane; j0 < p1; j0 += 256) {
+                double x[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const int64_t j = j0 + 64 * u;
+                        x[u][d] = (j < p1 && d < D) ? B.ctrl[j * st + d] : cen[d];
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) m = fmax(m, fabs(x[u][d] - cen[d]));
+            }
             for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
             sc = m > 0.0 ? m : 1.0;
         }

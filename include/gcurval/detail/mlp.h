@@ -58,6 +58,18 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// K = 32 products (v_mfma_f32_16x16x32_bf16): each lane supplies 8 values of
+// A's row / B's column. Operands built as the concatenation of two K = 16
+// fragments (cat8) contract over the same 32 k indices in A and B, so the
+// product equals the two K = 16 products summed.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 cat8(bf16x4 a, bf16x4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                  0, 0, 0);
+}
+
 // A operand from an LDS tile T[16][ld] (bf16), columns k0..k0+15.
 __device__ __forceinline__ bf16x4 load_a(const uint16_t* T, int ld, int k0) {
     const int l = lane_id();
@@ -139,7 +151,8 @@ __device__ __forceinline__ void store_d(f32x4 d, const float* bias, bool relu, u
 // consecutive bytes of a row-major feature row. The weight A operands
 // W^T[m][k] = W[k][m] are loaded once per wave (48 VGPRs).
 struct MlpFrags {
-    bf16x4 w1[4], w2[16], w3[4];
+    bf16x4 w1[4];
+    bf16x8 w2[8], w3[2];  // w2[c * 4 + mt]: k blocks 2c, 2c + 1 of output tile mt
     f32x4 b1[4], b2[4];
     float b3[kOut];
 };
@@ -149,9 +162,9 @@ __device__ __forceinline__ MlpFrags load_frags(const MlpParams& P) {
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
         F.w1[mt] = load_b(P.w1b, kHid, 0, mt * 16);
-        F.w3[mt] = load_b(P.w3b, 16, mt * 16, 0);
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) F.w2[kt * 4 + mt] = load_b(P.w2b, kHid, kt * 16, mt * 16);
+        for (int c = 0; c < 2; c++)
+            F.w2[c * 4 + mt] = cat8(load_b(P.w2b, kHid, 2 * c * 16, mt * 16), load_b(P.w2b, kHid, (2 * c + 1) * 16, mt * 16));
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             F.b1[mt][j] = P.p[off_b1() + mt * 16 + h0 + j];
@@ -159,43 +172,53 @@ __device__ __forceinline__ MlpFrags load_frags(const MlpParams& P) {
         }
     }
 #pragma unroll
+    for (int c = 0; c < 2; c++) F.w3[c] = cat8(load_b(P.w3b, 16, 2 * c * 16, 0), load_b(P.w3b, 16, (2 * c + 1) * 16, 0));
+#pragma unroll
     for (int j = 0; j < kOut; j++) F.b3[j] = P.p[off_b3() + j];
     return F;
 }
 
-__device__ __forceinline__ bf16x4 bias_relu_bf16(f32x4 d, f32x4 b) {
-    bf16x4 v;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const float x = d[j] + b[j];
-        v[j] = (short)f2bf(x > 0.0f ? x : 0.0f);
-    }
-    return v;
+// ReLU as a signed-integer maximum of the bit pattern (negative floats and -0
+// become +0; no canonicalising of the MFMA results), then round to BF16
+// (nearest even) two values per v_cvt_pk_bf16_f32.
+__device__ __forceinline__ float relu_bits(float x) {
+    return __builtin_bit_cast(float, __builtin_elementwise_max(__builtin_bit_cast(int, x), 0));
+}
+__device__ __forceinline__ uint32_t relu_pk_bf16(float a, float b) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    const f32x2_t v = {relu_bits(a), relu_bits(b)};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 d) {
+    typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+    const u32x2_t p = {relu_pk_bf16(d[0], d[1]), relu_pk_bf16(d[2], d[3])};
+    return __builtin_bit_cast(bf16x4, p);
 }
 
 // Forward pass of 16 rows: x is the B operand X^T (lane l: row l % 16,
 // features 4 (l / 16) .. +3). Lanes 0..15 receive outputs 0 and 1 of row l.
+// Each layer's accumulator starts from its bias.
 __device__ __forceinline__ void mlp_forward16(const MlpFrags& W, bf16x4 x, float& o0, float& o1) {
     bf16x4 a1[4], a2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-        a1[mt] = bias_relu_bf16(mfma16(W.w1[mt], x, z), W.b1[mt]);
-    }
+    for (int mt = 0; mt < 4; mt++) a1[mt] = relu_bf16(mfma16(W.w1[mt], x, W.b1[mt]));
     f32x4 h2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) h2[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int mt = 0; mt < 4; mt++) h2[mt] = W.b2[mt];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++)
+    for (int c = 0; c < 2; c++) {
+        const bf16x8 b = cat8(a1[2 * c], a1[2 * c + 1]);
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) h2[mt] = mfma16(W.w2[kt * 4 + mt], a1[kt], h2[mt]);
+        for (int mt = 0; mt < 4; mt++) h2[mt] = mfma32(W.w2[c * 4 + mt], b, h2[mt]);
+    }
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) a2[mt] = bias_relu_bf16(h2[mt], W.b2[mt]);
-    f32x4 o = {0.f, 0.f, 0.f, 0.f};
+    for (int mt = 0; mt < 4; mt++) a2[mt] = relu_bf16(h2[mt]);
+    f32x4 o = {W.b3[0], W.b3[1], 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) o = mfma16(W.w3[kt], a2[kt], o);
-    o0 = o[0] + W.b3[0];
-    o1 = o[1] + W.b3[1];
+    for (int c = 0; c < 2; c++) o = mfma32(W.w3[c], cat8(a2[2 * c], a2[2 * c + 1]), o);
+    o0 = o[0];
+    o1 = o[1];
 }
 
 // B operand X^T from a row-major BF16 row block (rows r0.., 16 features).
@@ -203,13 +226,7 @@ __device__ __forceinline__ bf16x4 load_xt(const uint16_t* rows, int64_t r0, int6
     const int l = lane_id();
     const int64_t r = r0 + (l & 15);
     bf16x4 v = {0, 0, 0, 0};
-    if (r < n) {
-        const uint2 q = *(const uint2*)(rows + r * kFeat + 4 * (l >> 4));
-        v[0] = (short)(q.x & 0xFFFFu);
-        v[1] = (short)(q.x >> 16);
-        v[2] = (short)(q.y & 0xFFFFu);
-        v[3] = (short)(q.y >> 16);
-    }
+    if (r < n) v = *(const bf16x4*)(rows + r * kFeat + 4 * (l >> 4));
     return v;
 }
 
@@ -349,7 +366,7 @@ static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParam
     }
 }
 
-static __global__ __launch_bounds__(256) void k_mlp_predict_rows(const uint16_t* xb, int64_t n, MlpParams P,
+static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_predict_rows(const uint16_t* xb, int64_t n, MlpParams P,
                                                                  int hedge, int8_t* kk, float* raw_out) {
     const int lane = lane_id();
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);

@@ -136,106 +136,8 @@ static __global__ __launch_bounds__(256) void k_seed_floor_ellipse(int64_t n, co
     kmin[i] = floor_from_ratio(ellipse_speed2_ratio(a[i], b[i]), ls_plain);
 }
 
-// Stable counting sort of intervals by bin (one radix pass). Block j owns
-// the contiguous chunk [j chunk, (j + 1) chunk) of intervals.
-//   k_bucket_hist   per-block bin counts, bin-major: hist[b * nblk + j]
-//   k_bucket_scan   one block: exclusive scan of hist in place, and
-//                   ranges[2b] = first slot, ranges[2b + 1] = count of bin b
-//   k_bucket_place  list[hist[b * nblk + j] + rank] = i, rank = order of i
-//                   among the chunk's intervals of bin b
-// The list is ordered by (bin, interval), so a wave tile takes neighbouring
-// intervals of one curve.
-constexpr int kBucketBlocks = 1024;  // most blocks of the bucketing passes
-
-static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B, int n_bins, int64_t chunk,
-                                                            int32_t* hist) {
-    __shared__ int h[kMaxBins];
-    for (int j = threadIdx.x; j < kMaxBins; j += blockDim.x) h[j] = 0;
-    __syncthreads();
-    const int64_t b0 = (int64_t)blockIdx.x * chunk;
-    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&h[bin_of(B, i)], 1);
-    __syncthreads();
-    for (int j = threadIdx.x; j < n_bins; j += blockDim.x) hist[(int64_t)j * gridDim.x + blockIdx.x] = h[j];
-}
-
-static __global__ __launch_bounds__(1024) void k_bucket_scan(int n_bins, int nblk, int32_t* hist, int64_t* ranges) {
-    __shared__ int32_t s_wave[16];
-    const int m = n_bins * nblk;
-    const int per = (m + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int b0 = (int)threadIdx.x * per;
-    const int b1 = b0 + per < m ? b0 + per : m;
-    int32_t sum = 0;
-    for (int j = b0; j < b1; j++) sum += hist[j];
-    // block exclusive scan of the per-thread sums
-    const int lane = lane_id(), w = (int)threadIdx.x >> 6;
-    int32_t x = sum;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int32_t t = __shfl_up(x, (unsigned)d, 64);
-        if (lane >= d) x += t;
-    }
-    if (lane == 63) s_wave[w] = x;
-    __syncthreads();
-    int32_t run = x - sum, total = 0;
-    for (int j = 0; j < (int)(blockDim.x >> 6); j++) {
-        if (j < w) run += s_wave[j];
-        total += s_wave[j];
-    }
-    for (int j = b0; j < b1; j++) {
-        const int32_t v = hist[j];
-        hist[j] = run;
-        run += v;
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < n_bins; b += blockDim.x) {
-        const int32_t start = hist[b * nblk];
-        const int32_t end = b + 1 < n_bins ? hist[(b + 1) * nblk] : total;
-        ranges[2 * b] = start;
-        ranges[2 * b + 1] = end - start;
-    }
-}
-
-static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, BinArgs B, int n_bins, int64_t chunk,
-                                                             const int32_t* hist, int32_t* list) {
-    __shared__ int32_t s_base[kMaxBins];
-    __shared__ int32_t s_wcnt[4][kMaxBins];
-    const int lane = lane_id(), w = (int)threadIdx.x >> 6;
-    for (int j = threadIdx.x; j < n_bins; j += blockDim.x) s_base[j] = hist[(int64_t)j * gridDim.x + blockIdx.x];
-    const int64_t b0 = (int64_t)blockIdx.x * chunk;
-    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
-    for (int64_t base = b0; base < b1; base += blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const bool valid = i < b1;
-        const int b = valid ? bin_of(B, i) : 0;
-        for (int j = lane; j < n_bins; j += 64) s_wcnt[w][j] = 0;
-        __syncthreads();
-        // rank within the wave, one ballot per distinct bin
-        int rank = 0;
-        uint64_t todo = ballot(valid);
-        while (todo) {
-            const int leader = __ffsll((unsigned long long)todo) - 1;
-            const int bl = __shfl(b, leader, 64);
-            const uint64_t mm = ballot(valid && b == bl);
-            if (valid && b == bl) rank = mbcnt(mm);
-            if (lane == leader) s_wcnt[w][bl] = popc64(mm);
-            todo &= ~mm;
-        }
-        __syncthreads();
-        if (valid) {
-            int32_t pos = s_base[b] + rank;
-            for (int v = 0; v < w; v++) pos += s_wcnt[v][b];
-            list[pos] = (int32_t)i;
-        }
-        __syncthreads();
-        for (int j = threadIdx.x; j < n_bins; j += blockDim.x)
-            s_base[j] += s_wcnt[0][j] + s_wcnt[1][j] + s_wcnt[2][j] + s_wcnt[3][j];
-        __syncthreads();
-    }
-}
-
-// Exclusive scan of 2^k_i into root_off[0..n] in three passes: per-block
-// sums of kScanItems * 256 elements, a single-block scan of those sums, and
-// the per-block apply.
+// Block-wide exclusive scan (blockDim.x / 64 <= 16 waves); *total gets the
+// block sum.
 __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* s_wave, int64_t* total) {
     // wave inclusive scan
     const int lane = lane_id();
@@ -258,6 +160,159 @@ __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* s_wave, i
     return wbase + x - v;
 }
 
+// Stable counting sort of intervals by bin (one radix pass). Wave w owns the
+// contiguous chunk [w wchunk, (w + 1) wchunk) of intervals (wchunk a multiple
+// of 64); bin counters live in registers, lane j holding bin j (bins 64..127
+// in a second register), and each 64-interval group is ranked with one
+// ballot per distinct bin, so there are no LDS atomics and no block barriers.
+//   k_bucket_hist   bins8[i] = bin of i; per-wave counts, bin-major:
+//                   hist[b * nw + w]
+//   k_bucket_totals block per bin: total count of the bin
+//   k_bucket_rows   block per bin: exclusive scan of the bin's row in place,
+//                   offset by the totals of the bins before it, and
+//                   ranges[2b] = first slot, ranges[2b + 1] = count of bin b
+//   k_bucket_place  list[hist[b * nw + w] + rank] = i, rank = order of i
+//                   among the wave chunk's intervals of bin b
+// The list is ordered by (bin, interval), so a wave tile takes neighbouring
+// intervals of one curve.
+constexpr int kBucketChunk = 1024;      // intervals per wave chunk (more when capped)
+constexpr int kBucketMaxWaves = 16384; // most wave chunks of the bucketing passes
+constexpr int kBucketUnroll = 4;       // 64-interval groups in flight per wave
+
+struct BinCounts {
+    int lo = 0, hi = 0;  // lane j: count of bin j, and of bin 64 + j
+    __device__ __forceinline__ int get(int bin) const {  // bin wave-uniform
+        return bin < 64 ? __builtin_amdgcn_readlane(lo, bin) : __builtin_amdgcn_readlane(hi, bin - 64);
+    }
+    __device__ __forceinline__ void add(int bin, int v) {
+        const int lane = lane_id();
+        if (bin < 64) {
+            if (lane == bin) lo += v;
+        } else if (lane == bin - 64) {
+            hi += v;
+        }
+    }
+};
+
+// Rank of each valid lane among the group's lanes of its bin, offset by the
+// running count of that bin; the counts then include the group.
+__device__ __forceinline__ int group_rank(bool valid, int b, BinCounts& cnt) {
+    int rank = 0;
+    uint64_t todo = ballot(valid);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int bl = __builtin_amdgcn_readlane(b, leader);
+        const uint64_t mm = ballot(valid && b == bl);
+        if (valid && b == bl) rank = cnt.get(bl) + mbcnt(mm);
+        cnt.add(bl, popc64(mm));
+        todo &= ~mm;
+    }
+    return rank;
+}
+
+static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B, int n_bins, int64_t wchunk, int nw,
+                                                            int8_t* bins8, int32_t* hist) {
+    const int lane = lane_id();
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const int64_t b0 = w * wchunk;
+    const int64_t b1 = b0 + wchunk < n ? b0 + wchunk : n;
+    BinCounts cnt;
+    for (int64_t g0 = b0; g0 < b1; g0 += 64 * kBucketUnroll) {
+        int bb[kBucketUnroll];
+#pragma unroll
+        for (int u = 0; u < kBucketUnroll; u++) {
+            const int64_t i = g0 + 64 * u + lane;
+            bb[u] = i < b1 ? bin_of(B, i) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBucketUnroll; u++) {
+            const int64_t i = g0 + 64 * u + lane;
+            const bool valid = i < b1;
+            if (valid) bins8[i] = (int8_t)bb[u];
+            group_rank(valid, bb[u], cnt);
+        }
+    }
+    if (lane < n_bins) hist[(int64_t)lane * nw + w] = cnt.lo;
+    if (lane + 64 < n_bins) hist[(int64_t)(lane + 64) * nw + w] = cnt.hi;
+}
+
+static __global__ __launch_bounds__(1024) void k_bucket_totals(int nw, const int32_t* hist, int32_t* totals) {
+    __shared__ int32_t s_wave[16];
+    const int32_t* row = hist + (int64_t)blockIdx.x * nw;
+    int32_t v = 0;
+    for (int j = threadIdx.x; j < nw; j += blockDim.x) v += row[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane_id() == 0) s_wave[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t t = 0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); j++) t += s_wave[j];
+        totals[blockIdx.x] = t;
+    }
+}
+
+static __global__ __launch_bounds__(1024) void k_bucket_rows(int nw, int32_t* hist, const int32_t* totals,
+                                                             int64_t* ranges) {
+    __shared__ int64_t s_wave[16];
+    __shared__ int64_t s_start;
+    const int b = blockIdx.x;
+    if (threadIdx.x < 64) {  // start of bin b: totals of bins 0 .. b-1 (b < kMaxBins <= 128)
+        const int l = (int)threadIdx.x;
+        int64_t v = (l < b ? totals[l] : 0) + (l + 64 < b ? totals[l + 64] : 0);
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (l == 0) s_start = v;
+    }
+    __syncthreads();
+    const int64_t start = s_start;
+    int32_t* row = hist + (int64_t)b * nw;
+    int64_t carry = start;
+    for (int base = 0; base < nw; base += blockDim.x) {
+        const int j = base + (int)threadIdx.x;
+        const int64_t v = j < nw ? row[j] : 0;
+        int64_t tot;
+        const int64_t ex = block_scan_excl(v, s_wave, &tot);
+        if (j < nw) row[j] = (int32_t)(carry + ex);
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        ranges[2 * b] = start;
+        ranges[2 * b + 1] = totals[b];
+    }
+}
+
+static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, int n_bins, int64_t wchunk, int nw,
+                                                             const int8_t* bins8, const int32_t* hist,
+                                                             int32_t* list) {
+    const int lane = lane_id();
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const int64_t b0 = w * wchunk;
+    const int64_t b1 = b0 + wchunk < n ? b0 + wchunk : n;
+    BinCounts base;
+    base.lo = lane < n_bins ? hist[(int64_t)lane * nw + w] : 0;
+    base.hi = lane + 64 < n_bins ? hist[(int64_t)(lane + 64) * nw + w] : 0;
+    for (int64_t g0 = b0; g0 < b1; g0 += 64 * kBucketUnroll) {
+        int bb[kBucketUnroll];
+#pragma unroll
+        for (int u = 0; u < kBucketUnroll; u++) {
+            const int64_t i = g0 + 64 * u + lane;
+            bb[u] = i < b1 ? (int)bins8[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBucketUnroll; u++) {
+            const int64_t i = g0 + 64 * u + lane;
+            const bool valid = i < b1;
+            const int pos = group_rank(valid, bb[u], base);
+            if (valid) list[pos] = (int32_t)i;
+        }
+    }
+}
+
+// Exclusive scan of 2^k_i into root_off[0..n] in three passes: per-block
+// sums of kScanItems * 256 elements, a single-block scan of those sums, and
+// the per-block apply.
+
 static __global__ __launch_bounds__(256) void k_root_block_sums(int64_t n, const int8_t* kk, const int8_t* kmin,
                                                               int64_t* block_sums) {
     __shared__ int64_t s_wave[4];
@@ -270,9 +325,9 @@ static __global__ __launch_bounds__(256) void k_root_block_sums(int64_t n, const
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// In place exclusive scan of nb block sums by one block.
-static __global__ __launch_bounds__(256) void k_scan_block_sums(int64_t nb, int64_t* block_sums) {
-    __shared__ int64_t s_wave[4];
+// In place exclusive scan of nb block sums by one block of 1024 threads.
+static __global__ __launch_bounds__(1024) void k_scan_block_sums(int64_t nb, int64_t* block_sums) {
+    __shared__ int64_t s_wave[16];
     int64_t carry = 0;
     for (int64_t base = 0; base < nb; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
