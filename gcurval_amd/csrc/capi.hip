@@ -439,6 +439,7 @@ struct gcv_table {
     std::vector<void*> owned;
     // sampling scratch: curve of the first sample of each sample block
     int32_t* blk_curve = nullptr;
+    int32_t* blk_int = nullptr;           // interval of each sample block's first sample (uniform positions)
     int64_t blk_cap = 0;
 };
 
@@ -458,6 +459,7 @@ static void table_free(gcv_table* t) {
         hipStreamSynchronize(t->ctx->stream);
         for (void* p : t->owned) gcv_free(t->ctx, p);
         if (t->blk_curve) gcv_free(t->ctx, t->blk_curve);
+        if (t->blk_int) gcv_free(t->ctx, t->blk_int);
     }
     delete t;
 }
@@ -1167,13 +1169,20 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
         if (t->blk_curve) {
             hipStreamSynchronize(c->stream);
             gcv_free(c, t->blk_curve);
-            t->blk_curve = nullptr;
+            gcv_free(c, t->blk_int);
+            t->blk_curve = t->blk_int = nullptr;
             t->blk_cap = 0;
         }
-        void* p = nullptr;
+        void *p = nullptr, *q = nullptr;
         int rc = gcv_alloc(c, sizeof(int32_t) * (size_t)n_blk, &p);
         if (rc) return rc;
+        rc = gcv_alloc(c, sizeof(int32_t) * (size_t)n_blk, &q);
+        if (rc) {
+            gcv_free(c, p);
+            return rc;
+        }
         t->blk_curve = (int32_t*)p;
+        t->blk_int = (int32_t*)q;
         t->blk_cap = n_blk;
     }
     prof_begin(c, "k_block_curve");
@@ -1182,6 +1191,18 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     e = hipGetLastError();
     prof_end(c);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "block-curve launch failed: %s", hipGetErrorString(e));
+    if (!d_pos && t->n_int > t->n_curve) {
+        // uniform targets are monotone along each curve: per-block interval
+        // bounds shorten every sample's interval search
+        prof_begin(c, "k_block_interval");
+        hipLaunchKernelGGL(dev::k_block_interval, dim3((unsigned)((n_blk + 255) / 256)), dim3(256), 0, c->stream, A,
+                           (const int32_t*)t->blk_curve, t->blk_int);
+        e = hipGetLastError();
+        prof_end(c);
+        if (e != hipSuccess)
+            return set_err(c, GCV_ERR_HIP, "block-interval launch failed: %s", hipGetErrorString(e));
+        A.blk_int = t->blk_int;
+    }
     const bool f32 = t->o.precision == GCV_FP32;
     const bool g3 = t->o.gl_order == 3;
     const int32_t* bc = t->blk_curve;

@@ -133,14 +133,15 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
                                                         const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
-    const int c = sample_curve(A, blk_curve, blockIdx.x, g);
+    const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
+    const int c = L.c;
     const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
     int32_t seg = 0;
     double u = 0.0;
 #define GCV_SAMPLE_CASE(PP, DD)                                                                     \
     case 10 * PP + DD:                                                                             \
         u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv64<Dt, PP, DD>::get(T), A, c, g,   \
-                                &seg);                                                             \
+                                &seg, L.lo, L.hi);                                                 \
         break;
     switch (cls) {
         GCV_SAMPLE_CASE(1, 2)

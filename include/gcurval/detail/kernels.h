@@ -485,6 +485,7 @@ struct SampleArgs {
     double* out_u;              // [n_total]
     int32_t* out_seg;           // optional local interval index per sample
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
+    const int32_t* blk_int;     // optional (uniform positions): interval of each sample block's first sample
 };
 
 // first index in [lo, hi) with a[idx] > v (hi if none)
@@ -653,24 +654,55 @@ __device__ __forceinline__ typename Ev64::Ctx widen_or_load(const Ev64& ev64, co
         return ev64.load(i, inv_sc);
 }
 
-template <typename Dt, int NGL, class Ev, class Ev64>
-__device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
-                                             int64_t g, int32_t* seg_out) {
+// Samples per block of the sampling kernels (blockDim.x).
+constexpr int kSampleBlock = 256;
+
+// Target arc length of sample g (curve c) in curve units.
+__device__ __forceinline__ double sample_target(const SampleArgs& A, int c, int64_t g) {
     double pos;
-    const int64_t s0 = A.soff[c];
     if (A.pos) {
         pos = A.pos[g];
     } else {
+        const int64_t s0 = A.soff[c];
         const int64_t N = A.soff[c + 1] - s0, j = g - s0;
         // j / (N - 1) (closed) or j / N to within 1.5 ulp; a position a rounding
         // above 1 is clamped to the curve's end by the searches below
         pos = A.uniform_closed ? (N > 1 ? (double)j * rcp_f64((double)(N - 1)) : 0.0) : (double)j * rcp_f64((double)N);
     }
-    const double s = A.is_relative ? pos * A.curve_len[c] : pos;
+    return A.is_relative ? pos * A.curve_len[c] : pos;
+}
+
+// Interval of target s on curve c: the first cumulative length >= s among the
+// curve's first n_int - 1, so a target past the last boundary lands in the
+// last interval; the search is limited to [lo, hi] (clamped to the curve).
+__device__ __forceinline__ int64_t interval_of_target(const SampleArgs& A, int c, double s, int64_t lo, int64_t hi) {
     const int64_t ib = A.ioff[c], ie = A.ioff[c + 1];
-    // interval: first cumulative length >= s among the curve's first
-    // n_int - 1, so a target past the last boundary lands in the last interval
-    const int64_t i = ib + count_below(A.int_cum + ib, (uint32_t)(ie - ib - 1), s);
+    lo = lo > ib ? lo : ib;
+    hi = hi < ie - 1 ? hi : ie - 1;
+    if (lo > hi) lo = hi;  // a stale hint: still a valid interval of the curve
+    return lo + count_below(A.int_cum + lo, (uint32_t)(hi - lo), s);
+}
+
+// Intervals of sample block blk (uniform positions): the sampler of a block's
+// samples on curve blk_curve[blk] searches only between this block's first
+// interval and the next block's (one interval of slack on each side).
+static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, const int32_t* blk_curve,
+                                                               int32_t* blk_int) {
+    const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_blk) return;
+    const int c = blk_curve[b];
+    const double s = sample_target(A, c, b * kSampleBlock);
+    blk_int[b] = (int32_t)interval_of_target(A, c, s, 0, INT64_MAX);
+}
+
+template <typename Dt, int NGL, class Ev, class Ev64>
+__device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
+                                             int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
+                                             int64_t hint_hi = INT64_MAX) {
+    const double s = sample_target(A, c, g);
+    const int64_t ib = A.ioff[c];
+    const int64_t i = interval_of_target(A, c, s, hint_lo, hint_hi);
     if (seg_out) *seg_out = (int32_t)(i - ib);
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
     const double inv_sc = rcp_f64(ev.scale((int)i));
@@ -724,7 +756,6 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
 // curve writes the blocks whose first sample falls in its range, so the
 // sampler finds a lane's curve with one load and a search over the few
 // curves one block spans.
-constexpr int kSampleBlock = 256;
 
 static __global__ __launch_bounds__(256) void k_block_curve(int n_curve, const int64_t* soff, int32_t* blk_curve) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -742,13 +773,35 @@ __device__ __forceinline__ int sample_curve(const SampleArgs& A, const int32_t* 
     return (int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1);
 }
 
+// Curve of sample g and, with the block interval map (uniform positions), the
+// interval range its search may be limited to: targets are non-decreasing
+// along a curve, so a sample lies between the intervals of its block's first
+// sample and the next block's first sample when those are on its curve.
+struct SampleLoc {
+    int c;
+    int64_t lo, hi;
+};
+__device__ __forceinline__ SampleLoc locate_sample(const SampleArgs& A, const int32_t* blk_curve, int64_t blk,
+                                                   int64_t g) {
+    const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
+    const int64_t c0 = blk_curve[blk];
+    const bool has_next = blk + 1 < n_blk;
+    const int64_t c1 = has_next ? blk_curve[blk + 1] : A.n_curve - 1;
+    SampleLoc L{(int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1), 0, INT64_MAX};
+    if (A.blk_int) {
+        if (L.c == c0) L.lo = (int64_t)A.blk_int[blk] - 1;
+        if (has_next && L.c == c1) L.hi = (int64_t)A.blk_int[blk + 1] + 1;
+    }
+    return L;
+}
+
 template <typename Dt, int NGL, class Ev, class Ev64>
 __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
-    const int c = sample_curve(A, blk_curve, blockIdx.x, g);
+    const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
     int32_t seg;
-    const double u = sample_one<Dt, NGL>(ev, ev64, A, c, g, &seg);
+    const double u = sample_one<Dt, NGL>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi);
     A.out_u[g] = u;
     if (A.out_seg) A.out_seg[g] = seg;
 }
