@@ -549,7 +549,7 @@ class ArclengthCalculateGpu:
 
     def close(self):
         if getattr(self, "h", None) is not None:
-            lib().gcv_table_destroy(self.h)
+            self.ctx._check(lib().gcv_table_destroy(self.h))  # refused during graph capture
             self.h = None
         for b in self._bufs:
             b.free()

@@ -73,6 +73,13 @@ static int capture_guard(gcv_ctx* c, const char* what) {
 }
 
 
+// Every entry point runs on its context's device, whatever device the calling
+// thread has current (hipSetDevice is per host thread).
+static void use_device(const gcv_ctx* c) {
+    int d = -1;
+    if (c && hipGetDevice(&d) == hipSuccess && d != c->device) hipSetDevice(c->device);
+}
+
 #define HIPCHK(ctx, expr)                                                                            \
     do {                                                                                             \
         hipError_t _e = (expr);                                                                      \
@@ -185,14 +192,21 @@ int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
 }
 
 int gcv_ctx_stream(gcv_ctx* c, void** stream) {
+    use_device(c);
     if (!c || !stream) return GCV_ERR_INVALID;
     *stream = (void*)c->stream;
     return GCV_OK;
 }
 
 int gcv_ctx_destroy(gcv_ctx* c) {
+    use_device(c);
     if (!c) return GCV_OK;
-    hipSetDevice(c->device);
+    if (c->capturing) {  // an unfinished capture: end it and drop the graph
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c->stream, &g);
+        if (g) hipGraphDestroy(g);
+        c->capturing = false;
+    }
     hipStreamSynchronize(c->stream);
     harvest(c);
     for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
@@ -209,6 +223,7 @@ int gcv_ctx_destroy(gcv_ctx* c) {
 const char* gcv_last_error(const gcv_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int gcv_alloc(gcv_ctx* c, size_t bytes, void** dptr) {
+    use_device(c);
     if (!c || !dptr) return GCV_ERR_INVALID;
     *dptr = nullptr;
     if (bytes == 0) bytes = 1;
@@ -239,6 +254,7 @@ int gcv_alloc(gcv_ctx* c, size_t bytes, void** dptr) {
 }
 
 int gcv_free(gcv_ctx* c, void* p) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     if (!p) return GCV_OK;
     auto it = c->live.find(p);
@@ -275,6 +291,7 @@ int gcv_free(gcv_ctx* c, void* p) {
 }
 
 int gcv_arena_stats(const gcv_ctx* c, size_t* used, size_t* capacity) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     if (used) *used = c->used;
     if (capacity) *capacity = c->cap;
@@ -282,6 +299,7 @@ int gcv_arena_stats(const gcv_ctx* c, size_t* used, size_t* capacity) {
 }
 
 int gcv_copy_h2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    use_device(c);
     if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
     if (int rc = capture_guard(c, "gcv_copy_h2d")) return rc;
     if (!bytes) return GCV_OK;
@@ -291,6 +309,7 @@ int gcv_copy_h2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int gcv_copy_d2h(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    use_device(c);
     if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
     if (int rc = capture_guard(c, "gcv_copy_d2h")) return rc;
     if (!bytes) return GCV_OK;
@@ -300,6 +319,7 @@ int gcv_copy_d2h(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int gcv_copy_d2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
+    use_device(c);
     if (!c || (bytes && (!dst || !src))) return GCV_ERR_INVALID;
     if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
         return set_err(c, GCV_ERR_INVALID, "gcv_copy_d2d needs 16-byte aligned pointers and sizes");
@@ -316,6 +336,7 @@ int gcv_copy_d2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int gcv_sync(gcv_ctx* c) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     if (int rc = capture_guard(c, "gcv_sync")) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -323,6 +344,7 @@ int gcv_sync(gcv_ctx* c) {
 }
 
 int gcv_device_info(gcv_ctx* c, int* n_cu, char* name, int name_len) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     if (n_cu) *n_cu = c->n_cu;
     if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->dev_name);
@@ -330,13 +352,16 @@ int gcv_device_info(gcv_ctx* c, int* n_cu, char* name, int name_len) {
 }
 
 int gcv_timer_start(gcv_ctx* c) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     HIPCHK(c, hipEventRecord(c->t0, c->stream));
     return GCV_OK;
 }
 
 int gcv_timer_stop(gcv_ctx* c, double* ms) {
+    use_device(c);
     if (!c || !ms) return GCV_ERR_INVALID;
+    if (int rc = capture_guard(c, "gcv_timer_stop")) return rc;
     HIPCHK(c, hipEventRecord(c->t1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->t1));
     float f = 0.f;
@@ -346,12 +371,14 @@ int gcv_timer_stop(gcv_ctx* c, double* ms) {
 }
 
 int gcv_profile_enable(gcv_ctx* c, int on) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     c->profiling = on != 0;
     return GCV_OK;
 }
 
 int gcv_profile_reset(gcv_ctx* c) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     harvest(c);
     c->prof.clear();
@@ -359,6 +386,7 @@ int gcv_profile_reset(gcv_ctx* c) {
 }
 
 int gcv_profile_query(gcv_ctx* c, const char* prefix, double* total_ms, int64_t* count) {
+    use_device(c);
     if (!c || !prefix) return GCV_ERR_INVALID;
     harvest(c);
     double t = 0;
@@ -522,6 +550,7 @@ static int table_alloc_common(gcv_table* t) {
 
 extern "C" int gcv_table_create_ellipse(gcv_ctx* c, int n_curve, const double* d_a, const double* d_b,
                                         const gcv_opts* opts, gcv_table** out) {
+    use_device(c);
     if (!c || !out) return GCV_ERR_INVALID;
     *out = nullptr;
     if (n_curve < 0 || (n_curve > 0 && (!d_a || !d_b))) return set_err(c, GCV_ERR_INVALID, "bad ellipse batch");
@@ -545,6 +574,7 @@ extern "C" int gcv_table_create_ellipse(gcv_ctx* c, int n_curve, const double* d
 
 extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, const gcv_opts* opts,
                                         gcv_table** out) {
+    use_device(c);
     if (!c || !B || !out) return GCV_ERR_INVALID;
     *out = nullptr;
     const int n = B->n_curve;
@@ -895,6 +925,7 @@ struct SeededAfterSetup {
 };
 
 extern "C" int gcv_table_build(gcv_table* t) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     hipError_t e = dispatch_build<UniformBuild>(t, t->log2_nsub, (int32_t*)nullptr);
     if (e != hipSuccess) return set_err(t->ctx, GCV_ERR_HIP, "table build launch failed: %s", hipGetErrorString(e));
@@ -903,6 +934,7 @@ extern "C" int gcv_table_build(gcv_table* t) {
 }
 
 extern "C" int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_cost) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (log2_roots < 0 || log2_roots > dev::kMaxSeedK)
         return set_err(t->ctx, GCV_ERR_INVALID, "log2_roots must be in [0, %d]", dev::kMaxSeedK);
@@ -913,6 +945,7 @@ extern "C" int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_
 }
 
 extern "C" int gcv_table_build_seeded(gcv_table* t, const int8_t* d_k, int32_t* d_cost) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (!d_k && t->n_int > 0) return set_err(t->ctx, GCV_ERR_INVALID, "null per-interval k array");
     hipError_t e = dispatch_build<SeededBuild>(t, d_k, d_cost);
@@ -922,11 +955,15 @@ extern "C" int gcv_table_build_seeded(gcv_table* t, const int8_t* d_k, int32_t* 
 }
 
 extern "C" int gcv_table_destroy(gcv_table* t) {
+    if (t) use_device(t->ctx);
+    if (t && t->ctx && t->ctx->capturing)
+        return set_err(t->ctx, GCV_ERR_STATE, "gcv_table_destroy synchronises the stream: not allowed during graph capture");
     table_free(t);
     return GCV_OK;
 }
 
 extern "C" int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_intervals, int* n_subdivided) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (n_curve) *n_curve = t->n_curve;
     if (n_intervals) *n_intervals = t->n_int;
@@ -935,12 +972,14 @@ extern "C" int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_inter
 }
 
 extern "C" int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor) {
+    if (t) use_device(t->ctx);
     if (!t || !d_floor) return GCV_ERR_INVALID;
     *d_floor = t->seed_floor;
     return GCV_OK;
 }
 
 extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int64_t** d_root_off) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (seeded) *seeded = t->seeded ? 1 : 0;
     if (log2_roots) *log2_roots = t->built_ls;
@@ -949,6 +988,7 @@ extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots,
 }
 
 extern "C" int gcv_table_lengths(gcv_table* t, double* host_out) {
+    if (t) use_device(t->ctx);
     if (!t || (!host_out && t->n_curve > 0)) return GCV_ERR_INVALID;
     if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
     return gcv_copy_d2h(t->ctx, host_out, t->curve_len, sizeof(double) * t->n_curve);
@@ -957,6 +997,7 @@ extern "C" int gcv_table_lengths(gcv_table* t, double* host_out) {
 extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** root_prefix,
                                       const uint64_t** deep_mask, const double** curve_len,
                                       const int64_t** int_off) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (int_cum) *int_cum = t->int_cum;
     if (root_prefix) *root_prefix = t->root_prefix;
@@ -968,6 +1009,7 @@ extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, cons
 
 extern "C" int gcv_points(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_u,
                           const int32_t* d_seg, int out_dim, double* d_out) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     if (n_total < 0) return set_err(c, GCV_ERR_INVALID, "n_total < 0");
@@ -1000,6 +1042,7 @@ struct gcv_graph {
 };
 
 extern "C" int gcv_graph_begin(gcv_ctx* c) {
+    use_device(c);
     if (!c) return GCV_ERR_INVALID;
     if (c->capturing) return set_err(c, GCV_ERR_STATE, "graph capture already in progress");
     if (c->pending_on) return set_err(c, GCV_ERR_STATE, "a profiled launch is open");
@@ -1009,6 +1052,7 @@ extern "C" int gcv_graph_begin(gcv_ctx* c) {
 }
 
 extern "C" int gcv_graph_end(gcv_ctx* c, gcv_graph** out) {
+    use_device(c);
     if (!c || !out) return GCV_ERR_INVALID;
     if (!c->capturing) return set_err(c, GCV_ERR_STATE, "no graph capture in progress");
     hipGraph_t g = nullptr;
@@ -1026,6 +1070,7 @@ extern "C" int gcv_graph_end(gcv_ctx* c, gcv_graph** out) {
 }
 
 extern "C" int gcv_graph_launch(gcv_graph* g) {
+    if (g) use_device(g->ctx);
     if (!g) return GCV_ERR_INVALID;
     if (g->ctx->capturing) return set_err(g->ctx, GCV_ERR_STATE, "graph launch during capture");
     HIPCHK(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
@@ -1033,7 +1078,9 @@ extern "C" int gcv_graph_launch(gcv_graph* g) {
 }
 
 extern "C" int gcv_graph_destroy(gcv_graph* g) {
+    if (g) use_device(g->ctx);
     if (!g) return GCV_OK;
+    if (int rc = capture_guard(g->ctx, "gcv_graph_destroy")) return rc;
     hipStreamSynchronize(g->ctx->stream);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
@@ -1046,6 +1093,7 @@ extern "C" int gcv_graph_destroy(gcv_graph* g) {
 // shape (waves per block x resident-grid fraction) and keeps the fastest.
 // ===========================================================================
 extern "C" int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* out) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     if (int rc = capture_guard(c, "gcv_table_autotune")) return rc;
@@ -1093,6 +1141,7 @@ extern "C" int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* 
 }
 
 extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (sub_waves_per_block != 0 && sub_waves_per_block != 1 && sub_waves_per_block != 2 && sub_waves_per_block != 4)
         return set_err(t->ctx, GCV_ERR_INVALID, "waves per block must be 0 (default), 1, 2 or 4");
@@ -1104,6 +1153,7 @@ extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int s
 
 extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
                                uint64_t* overflow, uint64_t* tasks) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (int rc = capture_guard(t->ctx, "gcv_table_stats")) return rc;
     if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
@@ -1129,6 +1179,7 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
 
 extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
                           int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg) {
+    if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     if (!t->built) return set_err(c, GCV_ERR_STATE, "arclengthReverse before createArclengthTable");
@@ -1266,6 +1317,7 @@ static int malloc_m(gcv_mlp* m, T** p, size_t count) {
 }
 
 extern "C" int gcv_mlp_create(gcv_ctx* c, gcv_mlp** out) {
+    use_device(c);
     if (!c || !out) return GCV_ERR_INVALID;
     gcv_mlp* m = new gcv_mlp;
     m->ctx = c;
@@ -1284,7 +1336,9 @@ extern "C" int gcv_mlp_create(gcv_ctx* c, gcv_mlp** out) {
 }
 
 extern "C" int gcv_mlp_destroy(gcv_mlp* m) {
+    if (m) use_device(m->ctx);
     if (!m) return GCV_OK;
+    if (int rc = capture_guard(m->ctx, "gcv_mlp_destroy")) return rc;
     hipStreamSynchronize(m->ctx->stream);
     for (void* p : m->owned) gcv_free(m->ctx, p);
     delete m;
@@ -1292,6 +1346,7 @@ extern "C" int gcv_mlp_destroy(gcv_mlp* m) {
 }
 
 extern "C" int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* mean, const float* istd) {
+    if (m) use_device(m->ctx);
     if (!m || !params || !mean || !istd) return GCV_ERR_INVALID;
     gcv_ctx* c = m->ctx;
     int rc;
@@ -1311,6 +1366,7 @@ extern "C" int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* 
 }
 
 extern "C" int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd) {
+    if (m) use_device(m->ctx);
     if (!m) return GCV_ERR_INVALID;
     gcv_ctx* c = m->ctx;
     int rc;
@@ -1344,6 +1400,7 @@ static dev::FeatArgs feat_args(gcv_table* t) {
 }
 
 extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
+    if (t) use_device(t->ctx);
     if (!t || (!d_out && t->n_int > 0)) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     hipError_t e = dispatch_build<SetupOnly>(t, false);
@@ -1393,6 +1450,7 @@ static hipError_t setup_and_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t*
 }
 
 extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw) {
+    if (m) use_device(m->ctx);
     if (!m || !t || (!d_k && t->n_int > 0)) return GCV_ERR_INVALID;
     if (m->ctx != t->ctx) return set_err(t->ctx, GCV_ERR_INVALID, "predictor and table belong to different contexts");
     hipError_t e = setup_and_predict(m, t, hedge, d_k, d_raw, false);
@@ -1401,6 +1459,7 @@ extern "C" int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_
 }
 
 extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost) {
+    if (t) use_device(t->ctx);
     if (!t || !m) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     if (m->ctx != c) return set_err(c, GCV_ERR_INVALID, "predictor and table belong to different contexts");
@@ -1412,12 +1471,14 @@ extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t*
 }
 
 extern "C" int gcv_table_seed_k(gcv_table* t, const int8_t** d_k) {
+    if (t) use_device(t->ctx);
     if (!t || !d_k) return GCV_ERR_INVALID;
     *d_k = t->seed_k;
     return GCV_OK;
 }
 
 extern "C" int gcv_mlp_forward(gcv_mlp* m, const uint16_t* d_x, int64_t n, float* d_out) {
+    if (m) use_device(m->ctx);
     if (!m || n < 0 || (n > 0 && (!d_x || !d_out))) return GCV_ERR_INVALID;
     gcv_ctx* c = m->ctx;
     if (n == 0) return GCV_OK;
@@ -1447,6 +1508,7 @@ static hipError_t launch_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y,
 
 extern "C" int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float w_pass,
                             float* host_grad, float* host_loss) {
+    if (m) use_device(m->ctx);
     if (!m || n <= 0 || !d_x || !d_y) return GCV_ERR_INVALID;
     hipError_t e = launch_grad(m, d_x, d_y, n, w_pass);
     if (e != hipSuccess) return set_err(m->ctx, GCV_ERR_HIP, "grad launch failed: %s", hipGetErrorString(e));
@@ -1458,6 +1520,7 @@ extern "C" int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, i
 
 extern "C" int gcv_mlp_train_step(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float lr,
                                   float w_pass, float* host_loss) {
+    if (m) use_device(m->ctx);
     if (!m || n <= 0 || !d_x || !d_y) return GCV_ERR_INVALID;
     gcv_ctx* c = m->ctx;
     hipError_t e = launch_grad(m, d_x, d_y, n, w_pass);

@@ -75,6 +75,8 @@ def test_capture_refuses_synchronising_calls(ctx):
             ctx.sync()
         with pytest.raises(G.GcvError):
             eng.stats()
+        with pytest.raises(G.GcvError):
+            eng.close()  # destroying a table synchronises: refused, the table stays usable
         g = ctx.graph_end()
         g.launch()
         ctx.sync()
