@@ -167,11 +167,14 @@ def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
         mlp = SeedMLP.pretrained(ctx, weights) if weights else SeedMLP.pretrained(ctx)
     out = {}
     for path in paths:
-        def step():
+        def build():
             if path == "learned":
                 eng.createArclengthTableLearned(mlp)
             else:
                 eng.createArclengthTable()
+
+        def step():
+            build()
             eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
 
         for _ in range(warmup):
@@ -190,9 +193,15 @@ def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
             t, n = ctx.profile_query(name)
             if n:
                 kernels[name] = {"total_ms": t, "launches": n, "avg_ms": t / n, "ms_per_step": t / steps}
-        t_all, _ = ctx.profile_query("k_")
         t_smp, _ = ctx.profile_query("k_sample")
         ctx.profile(False)
+        # build wall time on its own (a learned build may run its per-bin
+        # launches concurrently, so kernel times need not add up to it)
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            build()
+        build_ms = ctx.timer_stop() / steps
         st = eng.stats()
         lengths = eng.getAllCurveArclengths()
         # the same step captured once and replayed as one graph launch
@@ -208,7 +217,7 @@ def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
         graph.close()
         log(f"[bench] {path}: {steps} steps in {ms:.3f} ms (graph replays {graph_ms:.3f} ms); build stats {st}")
         out[path] = {"ms": ms, "ms_per_step": ms / steps, "graph_ms_per_step": graph_ms / steps,
-                     "kernels": kernels, "stats": st, "build_ms": (t_all - t_smp) / steps,
+                     "kernels": kernels, "stats": st, "build_ms": build_ms,
                      "sample_ms": t_smp / steps, "lengths": lengths}
     eng.close()
     d_off.free()

@@ -115,7 +115,7 @@ _SIGS = [
     ("gcv_graph_launch", C.c_int, [C.c_void_p]),
     ("gcv_graph_destroy", C.c_int, [C.c_void_p]),
     ("gcv_table_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
-    ("gcv_table_set_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gcv_table_set_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
     ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -296,7 +296,7 @@ class Graph:
 
 class _Tune(C.Structure):
     _fields_ = [("sub_waves_per_block", C.c_int), ("sub_grid_pct", C.c_int), ("best_build_ms", C.c_double),
-                ("default_build_ms", C.c_double), ("n_tried", C.c_int)]
+                ("default_build_ms", C.c_double), ("n_tried", C.c_int), ("bin_streams", C.c_int)]
 
 
 def make_opts(precision=GCV_FP32, gl_order=3, n_subdivided=64, max_depth=0, tolerance=0.0,
@@ -445,8 +445,8 @@ class ArclengthCalculateGpu:
         self._lengths = None
         return {k: getattr(t, k) for k, _ in _Tune._fields_}
 
-    def set_launch(self, waves_per_block: int = 0, grid_pct: int = 100):
-        self.ctx._check(lib().gcv_table_set_launch(self.h, waves_per_block, grid_pct))
+    def set_launch(self, waves_per_block: int = 0, grid_pct: int = 100, bin_streams: int = 0):
+        self.ctx._check(lib().gcv_table_set_launch(self.h, waves_per_block, grid_pct, bin_streams))
 
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)

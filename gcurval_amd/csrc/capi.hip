@@ -29,13 +29,21 @@ using namespace gcurval::rt;
 struct ProfRec {
     std::string name;
     hipEvent_t s, e;
+    hipStream_t stream;
 };
+
+// Auxiliary streams of a context: the learned build's per-bin subdivision
+// launches are spread over them so one bin's tail overlaps the next bin.
+constexpr int kAuxStreams = 3;
 
 struct gcv_ctx {
     int device = 0;
     int n_cu = 0;
     char dev_name[256] = {0};
     hipStream_t stream = nullptr;
+    hipStream_t aux[kAuxStreams] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kAuxStreams] = {};
     std::string err;
     // arena: one hipMalloc block, first-fit with coalescing (256-byte aligned)
     char* arena = nullptr;
@@ -101,19 +109,20 @@ static hipEvent_t take_event(gcv_ctx* c) {
 
 namespace gcurval {
 namespace rt {
-void prof_begin(void* owner, const char* name) {
+void prof_begin(void* owner, const char* name, hipStream_t stream) {
     gcv_ctx* c = (gcv_ctx*)owner;
     if (!c || !c->profiling || c->capturing) return;
     c->pending.name = name;
+    c->pending.stream = stream ? stream : c->stream;
     c->pending.s = take_event(c);
     c->pending.e = take_event(c);
-    if (c->pending.s) hipEventRecord(c->pending.s, c->stream);
+    if (c->pending.s) hipEventRecord(c->pending.s, c->pending.stream);
     c->pending_on = true;
 }
-void prof_end(void* owner) {
+void prof_end(void* owner, hipStream_t) {
     gcv_ctx* c = (gcv_ctx*)owner;
     if (!c || !c->pending_on) return;
-    if (c->pending.e) hipEventRecord(c->pending.e, c->stream);
+    if (c->pending.e) hipEventRecord(c->pending.e, c->pending.stream);
     c->recs.push_back(c->pending);
     c->pending_on = false;
 }
@@ -122,7 +131,7 @@ void prof_end(void* owner) {
 
 static void harvest(gcv_ctx* c) {
     if (c->recs.empty()) return;
-    hipStreamSynchronize(c->stream);
+    hipStreamSynchronize(c->stream);  // aux-stream work always joins the context stream
     for (ProfRec& r : c->recs) {
         float ms = 0.f;
         if (r.s && r.e && hipEventElapsedTime(&ms, r.s, r.e) == hipSuccess) {
@@ -137,6 +146,37 @@ static void harvest(gcv_ctx* c) {
 }
 
 static LaunchCfg cfg_of(gcv_ctx* c) { return LaunchCfg{c->stream, c->n_cu, c, 0, 100}; }
+
+// Streams and events of a context (each may be unset), then the object.
+static void ctx_release(gcv_ctx* c) {
+    for (int i = 0; i < kAuxStreams; i++) {
+        if (c->join_ev[i]) hipEventDestroy(c->join_ev[i]);
+        if (c->aux[i]) hipStreamDestroy(c->aux[i]);
+    }
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    if (c->t0) hipEventDestroy(c->t0);
+    if (c->t1) hipEventDestroy(c->t1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Fork: the auxiliary streams wait for the work queued so far on the context
+// stream; join: the context stream waits for everything queued on them. Both
+// are stream-ordered (no host synchronisation) and are recorded into a graph
+// being captured.
+static hipError_t aux_fork(gcv_ctx* c) {
+    hipError_t e = hipEventRecord(c->fork_ev, c->stream);
+    for (int i = 0; e == hipSuccess && i < kAuxStreams; i++) e = hipStreamWaitEvent(c->aux[i], c->fork_ev, 0);
+    return e;
+}
+static hipError_t aux_join(gcv_ctx* c) {
+    hipError_t e = hipSuccess;
+    for (int i = 0; e == hipSuccess && i < kAuxStreams; i++) {
+        e = hipEventRecord(c->join_ev[i], c->aux[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->join_ev[i], 0);
+    }
+    return e;
+}
 
 extern "C" {
 
@@ -169,19 +209,21 @@ int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
         snprintf(c->dev_name, sizeof c->dev_name, "%s", prop.gcnArchName);
     }
     if (c->n_cu <= 0) c->n_cu = 256;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return GCV_ERR_HIP; }
-    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
-        hipStreamDestroy(c->stream);
-        delete c;
+    // every stream at the default priority
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->t0) == hipSuccess && hipEventCreate(&c->t1) == hipSuccess &&
+              hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kAuxStreams; i++)
+        ok = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ctx_release(c);
         return GCV_ERR_HIP;
     }
     if (arena_bytes > 0) {
         arena_bytes = (arena_bytes + 255) & ~(size_t)255;
         if (hipMalloc((void**)&c->arena, arena_bytes) != hipSuccess) {
-            hipEventDestroy(c->t0);
-            hipEventDestroy(c->t1);
-            hipStreamDestroy(c->stream);
-            delete c;
+            ctx_release(c);
             return GCV_ERR_OOM;
         }
         c->cap = arena_bytes;
@@ -210,13 +252,12 @@ int gcv_ctx_destroy(gcv_ctx* c) {
     hipStreamSynchronize(c->stream);
     harvest(c);
     for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+    c->event_pool.clear();
     if (c->arena) hipFree(c->arena);
     else
         for (auto& kv : c->live) hipFree(kv.first);
-    hipEventDestroy(c->t0);
-    hipEventDestroy(c->t1);
-    hipStreamDestroy(c->stream);
-    delete c;
+    c->arena = nullptr;
+    ctx_release(c);
     return GCV_OK;
 }
 
@@ -452,6 +493,7 @@ struct gcv_table {
     uint16_t* feat_x = nullptr;           // [n_int][16] learned-seed input rows (allocated on first use)
     int tune_wpb = 0;                     // subdivision launch shape picked by gcv_table_autotune
     int tune_grid_pct = 100;
+    int tune_streams = 1;                 // streams of the seeded build's per-bin launches (default_streams)
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
     int64_t* root_off = nullptr;          // [n_int + 1]
     int32_t* bucket_list = nullptr;       // [n_int]
@@ -470,6 +512,12 @@ struct gcv_table {
     int32_t* blk_int = nullptr;           // interval of each sample block's first sample (uniform positions)
     int64_t blk_cap = 0;
 };
+
+// Default spread of the learned build's per-bin launches: over the auxiliary
+// streams for batches of several evaluator classes (many launches, 7 per
+// class, whose tails then overlap), on the context stream for one class
+// (where the fork and join cost more than the overlap gains).
+static int default_streams(const gcv_table* t) { return t->classes.size() > 1 ? kAuxStreams : 1; }
 
 template <typename T>
 static int talloc(gcv_table* t, T** p, size_t count) {
@@ -667,6 +715,7 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
             return rc;
         }
     }
+    t->tune_streams = default_streams(t);
     *out = t;
     return GCV_OK;
 }
@@ -866,10 +915,17 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         LAUNCH_OR_RETURN("k_root_apply", dev::k_root_apply, dim3(nb), dim3(256), 0, c->stream, n, kk,
                          t->seed_floor, t->scan_blocks, t->root_off);
     }
-    const LaunchCfg L = cfg_of_table(t);
     const BSplineTableDev T = bspline_dev(t);
+    // the (class, k) bins are independent: with tune_streams > 1 they are
+    // spread over the auxiliary streams so one launch's tail overlaps the next
+    const int n_streams = std::min(std::max(t->tune_streams, 1), kAuxStreams);
+    const bool multi = n_streams > 1;
+    if (multi && (e = aux_fork(c)) != hipSuccess) return e;
+    int launch_no = 0;
     for (int ci = 0; ci < n_cls; ci++) {
         for (int k = 0; k <= dev::kMaxSeedK; k++) {
+            LaunchCfg L = cfg_of_table(t);
+            if (multi) L.stream = c->aux[launch_no++ % n_streams];
             dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
             A.log2_nsub = k;
             A.eps0 = eps_of<Dt>(t);
@@ -886,9 +942,13 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
                 const ClassRange& r = t->classes[ci];
                 e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, A);
             }
-            if (e != hipSuccess) return e;
+            if (e != hipSuccess) {
+                if (multi) aux_join(c);
+                return e;
+            }
         }
     }
+    if (multi && (e = aux_join(c)) != hipSuccess) return e;
     t->seeded = true;
     return build_scan(t);
 }
@@ -1100,26 +1160,32 @@ extern "C" int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* 
     if (reps < 1) reps = 3;
     static const int kWpb[3] = {4, 2, 1};
     static const int kGrid[3] = {100, 50, 25};
-    const int keep_wpb = t->tune_wpb, keep_grid = t->tune_grid_pct;
+    const int keep_wpb = t->tune_wpb, keep_grid = t->tune_grid_pct, keep_streams = t->tune_streams;
     double best = 1e300, dflt = 0.0;
-    int best_wpb = 4, best_grid = 100, tried = 0;
+    int best_wpb = 4, best_grid = 100, best_streams = 1, tried = 0;
+    auto build = [&]() { return m ? gcv_table_build_mlp(t, m, 0, nullptr) : gcv_table_build(t); };
+    auto time_shape = [&](int wpb, int grid, int streams, double* ms) -> int {
+        t->tune_wpb = wpb;
+        t->tune_grid_pct = grid;
+        t->tune_streams = streams;
+        int rc = build();  // warm-up
+        if (!rc) rc = gcv_timer_start(c);
+        for (int r = 0; r < reps && !rc; r++) rc = build();
+        if (!rc) rc = gcv_timer_stop(c, ms);
+        if (rc) {
+            t->tune_wpb = keep_wpb;
+            t->tune_grid_pct = keep_grid;
+            t->tune_streams = keep_streams;
+            return rc;
+        }
+        *ms /= reps;
+        tried++;
+        return GCV_OK;
+    };
     for (int wi = 0; wi < 3; wi++) {
         for (int gi = 0; gi < 3; gi++) {
-            t->tune_wpb = kWpb[wi];
-            t->tune_grid_pct = kGrid[gi];
-            auto build = [&]() { return m ? gcv_table_build_mlp(t, m, 0, nullptr) : gcv_table_build(t); };
-            int rc = build();  // warm-up
-            if (!rc) rc = gcv_timer_start(c);
-            for (int r = 0; r < reps && !rc; r++) rc = build();
             double ms = 0.0;
-            if (!rc) rc = gcv_timer_stop(c, &ms);
-            if (rc) {
-                t->tune_wpb = keep_wpb;
-                t->tune_grid_pct = keep_grid;
-                return rc;
-            }
-            ms /= reps;
-            tried++;
+            if (int rc = time_shape(kWpb[wi], kGrid[gi], 1, &ms)) return rc;
             if (wi == 0 && gi == 0) dflt = ms;
             if (ms < best) {
                 best = ms;
@@ -1128,26 +1194,38 @@ extern "C" int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* 
             }
         }
     }
+    if (m) {  // learned builds: the best shape with the bins spread over the auxiliary streams too
+        double ms = 0.0;
+        if (int rc = time_shape(best_wpb, best_grid, kAuxStreams, &ms)) return rc;
+        if (ms < best) {
+            best = ms;
+            best_streams = kAuxStreams;
+        }
+    }
     t->tune_wpb = best_wpb;
     t->tune_grid_pct = best_grid;
+    t->tune_streams = best_streams;
     if (out) {
         out->sub_waves_per_block = best_wpb;
         out->sub_grid_pct = best_grid;
         out->best_build_ms = best;
         out->default_build_ms = dflt;
         out->n_tried = tried;
+        out->bin_streams = best_streams;
     }
-    return m ? gcv_table_build_mlp(t, m, 0, nullptr) : gcv_table_build(t);
+    return build();
 }
-
-extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct) {
+extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct, int bin_streams) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (sub_waves_per_block != 0 && sub_waves_per_block != 1 && sub_waves_per_block != 2 && sub_waves_per_block != 4)
         return set_err(t->ctx, GCV_ERR_INVALID, "waves per block must be 0 (default), 1, 2 or 4");
     if (sub_grid_pct < 1 || sub_grid_pct > 100) return set_err(t->ctx, GCV_ERR_INVALID, "grid percent in [1, 100]");
+    if (bin_streams < 0 || bin_streams > kAuxStreams)
+        return set_err(t->ctx, GCV_ERR_INVALID, "bin_streams must be in [0, %d]", kAuxStreams);
     t->tune_wpb = sub_waves_per_block;
     t->tune_grid_pct = sub_grid_pct;
+    t->tune_streams = bin_streams > 0 ? bin_streams : default_streams(t);
     return GCV_OK;
 }
 

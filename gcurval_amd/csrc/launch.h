@@ -23,9 +23,10 @@ struct LaunchCfg {
     int sub_grid_pct = 100; // persistent subdivision grid, percent of the resident-block limit
 };
 
-// Profiling hooks implemented by the context (events around a launch).
-void prof_begin(void* owner, const char* name);
-void prof_end(void* owner);
+// Profiling hooks implemented by the context (events around a launch on
+// `stream`, the context's own stream when null).
+void prof_begin(void* owner, const char* name, hipStream_t stream = nullptr);
+void prof_end(void* owner, hipStream_t stream = nullptr);
 
 // Device arrays of a B-spline table needed by the evaluators.
 struct BSplineTableDev {

@@ -43,10 +43,10 @@ static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivAr
     if (blocks < 1) blocks = 1;
     const int64_t need = (n_tiles + wpb - 1) / wpb;
     if (blocks > need) blocks = need;
-    prof_begin(c.owner, name);
+    prof_begin(c.owner, name, c.stream);
     hipLaunchKernelGGL((k_subdivide<Dt, NGL, Ev>), dim3((unsigned)blocks), dim3(threads), lds, c.stream, ev, A);
     e = hipGetLastError();
-    prof_end(c.owner);
+    prof_end(c.owner, c.stream);
     return e;
 }
 

@@ -101,3 +101,30 @@ def test_autotune_keeps_results(ctx):
             np.testing.assert_allclose(eng.getAllCurveArclengths(), L0, rtol=1e-13)
         with pytest.raises(G.GcvError):
             eng.set_launch(3, 100)
+
+
+def test_learned_bins_on_several_streams_match_one_stream(ctx):
+    """The learned build's per-bin launches spread over the auxiliary streams
+    (joined before the build returns) give the single-stream tables bit for
+    bit; the autotuner tries the spread for learned builds."""
+    b = W.bspline_batch(120, seed=85, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(16, 300), hairpin_frac=0.2)
+    counts = 8 * b.intervals_per_curve()
+    mlp = S.SeedMLP.pretrained(ctx)
+    try:
+        with G.ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
+            out = {}
+            for streams in (1, 3):
+                eng.set_launch(0, 100, streams)
+                eng.createArclengthTableLearned(mlp)
+                eng._lengths = None
+                out[streams] = (eng.getAllCurveArclengths().copy(), eng.arclengthReverse(counts, None, True))
+            np.testing.assert_array_equal(out[1][0], out[3][0])
+            np.testing.assert_array_equal(out[1][1], out[3][1])
+            rep = eng.autotune(mlp=mlp, reps=1)
+            assert rep["n_tried"] == 10 and rep["bin_streams"] in (1, 3)
+            eng._lengths = None
+            np.testing.assert_array_equal(eng.getAllCurveArclengths(), out[1][0])
+            with pytest.raises(G.GcvError):
+                eng.set_launch(0, 100, 4)
+    finally:
+        mlp.close()
