@@ -31,6 +31,29 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 // Coefficient block stride per interval (p*D <= 15 used).
 constexpr int kCoefStride = 16;
 
+#if defined(__HIPCC__)
+// 1 / v for finite nonzero normal v, within 1 ulp: v_rcp_f64 refined by two
+// Newton steps (the IEEE-exact division adds scaling and fix-up steps that
+// start values, scales and Newton updates do not need).
+__device__ __forceinline__ double rcp_f64(double v) {
+    double r = __builtin_amdgcn_rcp(v);
+    double e = fma(-v, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-v, r, 1.0);
+    return fma(r, e, r);
+}
+#endif
+
+// 1 / x for the span polynomials: the refined hardware reciprocal on the GPU,
+// IEEE division on the host (the two agree to 1 ulp).
+GCURVAL_HD inline double span_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rcp_f64(x);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // B-spline span -> power-basis derivative polynomial (double precision).
 //
@@ -61,7 +84,7 @@ GCURVAL_HD inline int span_poly(int s, const double* T, const double* Pc, int ps
     for (int j = 0; j <= q; j++) {
         const int m = s - P + j;
         const double den = T[m + P + 1] - T[m + 1];
-        fq[j] = den > 0.0 ? double(P) / den : 0.0;
+        fq[j] = den > 0.0 ? double(P) * span_rcp(den) : 0.0;
     }
     double a0[q + 1][q + 1], a1[q + 1][q + 1];
 #pragma unroll
@@ -70,8 +93,9 @@ GCURVAL_HD inline int span_poly(int s, const double* T, const double* Pc, int ps
         for (int j = q; j >= r; j--) {
             const int i = s - q + j;
             const double den = T[i + q - r + 1] - T[i];
-            a0[r][j] = den > 0.0 ? (ts - T[i]) / den : 0.0;
-            a1[r][j] = den > 0.0 ? h / den : 0.0;
+            const double rd = den > 0.0 ? span_rcp(den) : 0.0;
+            a0[r][j] = (ts - T[i]) * rd;
+            a1[r][j] = h * rd;
         }
     }
     const double f = h * scale_inv;
@@ -130,17 +154,6 @@ GCURVAL_HD inline int bspline_span_poly(int p, int D, int s, const double* T, co
 }
 
 #if defined(__HIPCC__)
-
-// 1 / v for finite nonzero normal v, within 1 ulp: v_rcp_f64 refined by two
-// Newton steps (the IEEE-exact division adds scaling and fix-up steps that
-// start values, scales and Newton updates do not need).
-__device__ __forceinline__ double rcp_f64(double v) {
-    double r = __builtin_amdgcn_rcp(v);
-    double e = fma(-v, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-v, r, 1.0);
-    return fma(r, e, r);
-}
 
 // sqrt(v) for v >= 0, within 1 ulp for v >= 2^-1000 (0 below): v_rsq_f64
 // refined by one coupled Goldschmidt step on (g, h) ~ (sqrt v, 1/(2 sqrt v))

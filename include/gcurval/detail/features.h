@@ -26,6 +26,14 @@ __host__ __device__ inline float bf2f(uint16_t b) {
     return f;
 }
 
+#if defined(__HIPCC__)
+// Hardware log2 / square root / reciprocal (about 1 ulp): the features are
+// rounded to BF16 (8 significant bits) before they reach the network.
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fdiv(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+#endif
+
 // ---------------------------------------------------------------------------
 // Per-interval features (16, FP32, before standardisation), all in the
 // normalised units the error test sees:
@@ -44,20 +52,20 @@ __device__ __forceinline__ void features_common(const float sp[5], const float d
     float mn = sp[0], mx = sp[0];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        f[q] = log2f(sp[q] + 1e-12f);
+        f[q] = flog2(sp[q] + 1e-12f);
         mn = fminf(mn, sp[q]);
         mx = fmaxf(mx, sp[q]);
     }
-    f[5] = log2f((mx + 1e-12f) / (mn + 1e-12f));
+    f[5] = flog2(fdiv(mx + 1e-12f, mn + 1e-12f));
     const int pa[3] = {0, 0, 2}, pb[3] = {4, 2, 4};
 #pragma unroll
     for (int t = 0; t < 3; t++) {
         const int a = pa[t], b = pb[t];
         const float dot = dv[a][0] * dv[b][0] + dv[a][1] * dv[b][1] + dv[a][2] * dv[b][2];
-        f[6 + t] = dot / (sp[a] * sp[b] + 1e-20f);
+        f[6 + t] = fdiv(dot, sp[a] * sp[b] + 1e-20f);
     }
     const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
-    f[9] = log2f(tol0 / Lest);
+    f[9] = flog2(fdiv(tol0, Lest));
 }
 
 template <int P, int D>
@@ -78,14 +86,14 @@ __device__ void features_bspline(const float* c, double h, int n_int_curve, cons
             dv[q][d] = v;
             s2 += v * v;
         }
-        sp[q] = sqrtf(s2);
+        sp[q] = fsqrt(s2);
     }
     features_common(sp, dv, tol0, f);
     const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
     float lead = 0.f;
 #pragma unroll
     for (int d = 0; d < D; d++) lead += c[d * P + P - 1] * c[d * P + P - 1];
-    f[10] = P > 1 ? log2f(sqrtf(lead) / Lest + 1e-12f) : -40.f;
+    f[10] = P > 1 ? flog2(fdiv(fsqrt(lead), Lest) + 1e-12f) : -40.f;
     // the span's control polygon P_{s-p} .. P_s
     float emin = 1e30f, emax = 0.f, cmin = 1.f;
     float pe[3] = {0.f, 0.f, 0.f};
@@ -97,25 +105,25 @@ __device__ void features_bspline(const float* c, double h, int n_int_curve, cons
             ev[d] = (float)(ctrl[(s - P + e + 1) * pstride + d] - ctrl[(s - P + e) * pstride + d]);
             el += ev[d] * ev[d];
         }
-        el = sqrtf(el);
+        el = fsqrt(el);
         emin = fminf(emin, el);
         emax = fmaxf(emax, el);
         if (e > 0) {
-            const float pl = sqrtf(pe[0] * pe[0] + pe[1] * pe[1] + pe[2] * pe[2]);
+            const float pl = fsqrt(pe[0] * pe[0] + pe[1] * pe[1] + pe[2] * pe[2]);
             const float dot = pe[0] * ev[0] + pe[1] * ev[1] + pe[2] * ev[2];
-            cmin = fminf(cmin, dot / (pl * el + 1e-20f));
+            cmin = fminf(cmin, fdiv(dot, pl * el + 1e-20f));
         }
         pe[0] = ev[0];
         pe[1] = ev[1];
         pe[2] = ev[2];
     }
-    f[11] = log2f((emax + 1e-12f) / (emin + 1e-12f));
+    f[11] = flog2(fdiv(emax + 1e-12f, emin + 1e-12f));
     f[12] = cmin;
-    f[13] = log2f((float)(h * n_int_curve) + 1e-12f);
+    f[13] = flog2((float)(h * n_int_curve) + 1e-12f);
     f[14] = P * 0.2f + D * 0.1f;
     // 17-sample min / max speed (the safety-floor quantity)
     const float r2 = speed2_ratio<P, D>(c);
-    f[15] = sqrtf(r2);
+    f[15] = fsqrt(r2);
     *r2_out = r2;
 }
 
@@ -129,16 +137,16 @@ __device__ void features_ellipse(double a, double b, float tol0, float* f) {
         dv[q][0] = -A * sn[q];
         dv[q][1] = B * cs[q];
         dv[q][2] = 0.f;
-        sp[q] = sqrtf(dv[q][0] * dv[q][0] + dv[q][1] * dv[q][1]);
+        sp[q] = fsqrt(dv[q][0] * dv[q][0] + dv[q][1] * dv[q][1]);
     }
     features_common(sp, dv, tol0, f);
     const float lo = fminf(fabsf(A), fabsf(B)), hi = fmaxf(fabsf(A), fabsf(B));
-    f[10] = log2f((hi + 1e-12f) / (lo + 1e-12f));
+    f[10] = flog2(fdiv(hi + 1e-12f, lo + 1e-12f));
     f[11] = 0.f;
     f[12] = 1.f;
     f[13] = 0.f;
     f[14] = 0.f;
-    f[15] = hi > 0.f ? lo / hi : 1.f;
+    f[15] = hi > 0.f ? fdiv(lo, hi) : 1.f;
 }
 
 // Optional outputs of the B-spline setup pass for a learned-seed build:

@@ -454,7 +454,7 @@ __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int32_
     const int nint = kFeatures ? (int)(ioff[c + 1] - i0) : 0;
     const double* T = B.knots + B.knot_off[c];
     const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
-    coef_one<P, D, kFeatures>(i, (int)(i - i0) + P, T, Pc, B.ctrl_stride, 1.0 / int_scale[i],
+    coef_one<P, D, kFeatures>(i, (int)(i - i0) + P, T, Pc, B.ctrl_stride, rcp_f64(int_scale[i]),
                               coef64 ? coef64 + i * cstride : nullptr, coef32 ? coef32 + i * cstride : nullptr,
                               &u0[i], &h[i],
                               F, nint);
