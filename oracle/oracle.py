@@ -140,7 +140,7 @@ def uniform_positions(counts, closed=True):
         if closed:
             out.append(j * (1.0 / (n - 1)) if n > 1 else np.zeros(n))
         else:
-            out.append(j * (1.0 / n))
+            out.append(j * (1.0 / n) if n > 0 else np.zeros(0))
     return np.concatenate(out) if out else np.zeros(0)
 
 

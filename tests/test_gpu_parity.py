@@ -222,6 +222,27 @@ def test_empty_batch_and_zero_samples(ctx):
         assert u.size == 5 and np.all(np.isfinite(u))
 
 
+@pytest.mark.parametrize("closed", [True, False])
+def test_ragged_sample_counts_bspline(ctx, closed):
+    """Per-curve counts 0, 1, 2, 3 and several hundred in one batch, so sample
+    blocks of 256 start mid-curve, span many short curves and skip empty
+    ones: the block interval bounds must leave every sample's interval and
+    parameter equal to the unbounded search's (the truth, within 1e-6)."""
+    b = W.bspline_batch(48, seed=34, degrees=(2, 3), dims=(2, 3), n_ctrl_range=(8, 300))
+    rng = np.random.default_rng(35)
+    counts = rng.choice([0, 1, 2, 3, 257, 700], size=b.n_curve).astype(np.int64)
+    counts[:4] = [0, 1, 2, 700]
+    tr = O.Truth(b)
+    Lt, cumt = tr.lengths()
+    with build(ctx, b, normalize=True) as eng:
+        u, seg = eng.arclengthReverse(counts, None, True, return_segments=True, uniform_closed=closed)
+    pos = O.uniform_positions(counts, closed=closed)
+    ut, segt = tr.reverse(counts, pos)
+    bad, ties = count_mismatches(b, counts, pos, seg, segt, Lt, cumt, 1e-6)
+    assert bad == 0, (bad, ties)
+    assert np.max(np.abs(u - ut)) <= 1e-6, np.max(np.abs(u - ut))
+
+
 def test_explicit_positions_absolute_and_clamped(ctx):
     b = W.ellipse_batch(10, seed=5)
     Lt, _ = O.Truth(b).lengths()
