@@ -316,12 +316,21 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
     for (int64_t c = gwave; c < n_curve; c += nwaves) {
         const int64_t b = ioff[c], e = ioff[c + 1];
         double carry = 0.0;
-        for (int64_t base = b; base < e; base += 64) {
-            const int64_t j = base + lane;
-            double v = j < e ? int_cum[j] : 0.0;
-            v = seg_scan_incl(v, 6) + carry;
-            if (j < e) int_cum[j] = v;
-            carry = shfl_d(v, 63);
+        // 4 x 64 lengths in flight per iteration, then scanned in order
+        for (int64_t base = b; base < e; base += 256) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t j = base + 64 * u + lane;
+                v[u] = j < e ? int_cum[j] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t j = base + 64 * u + lane;
+                const double w = seg_scan_incl(v[u], 6) + carry;
+                if (j < e) int_cum[j] = w;
+                carry = shfl_d(w, 63);
+            }
         }
         if (lane == 0) curve_len[c] = carry;
     }
