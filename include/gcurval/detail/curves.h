@@ -177,7 +177,8 @@ __device__ __forceinline__ double sqrt_pos(double v) {
 // sin^2(2 pi x) and cos^2(2 pi x) are g U(g) and 1 - g U(g), swapped when q
 // is odd, so both keep full relative accuracy near their zeros.
 // FP32: U is a degree-4 weighted least-squares fit (max relative error
-// 1.2e-7 in FP32 evaluation); FP64: 11 Taylor terms (2.2e-16). Derivation:
+// 1.2e-7 in FP32 evaluation); FP64: degree 7, the truncated Chebyshev
+// expansion in powers of g (2.2e-16, as the 11-term Taylor series). Derivation:
 // gcurval_amd/fit_sincos.py.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void sincos2_2pi(float x, float& s2, float& c2) {
@@ -198,16 +199,13 @@ __device__ __forceinline__ void sincos2_2pi(double x, double& s2, double& c2) {
     const double q = rint(r);
     const double f = r - q;
     const double g = f * f;
-    double u = 3.850353565300677e-11;
-    u = fma(u, g, -1.8023653987312506e-09);
-    u = fma(u, g, 6.939476231106886e-08);
-    u = fma(u, g, -2.1515347935164736e-06);
-    u = fma(u, g, 5.231905246242285e-05);
-    u = fma(u, g, -0.0009647871547019616);
-    u = fma(u, g, 0.01290344569500703);
-    u = fma(u, g, -0.1176653151794466);
-    u = fma(u, g, 0.6676313844272947);
-    u = fma(u, g, -2.0293560632083842);
+    double u = -2.0832065862542846e-06;
+    u = fma(u, g, 5.229143784023563e-05);
+    u = fma(u, g, -0.0009647813283021856);
+    u = fma(u, g, 0.012903445013289263);
+    u = fma(u, g, -0.11766531513610962);
+    u = fma(u, g, 0.6676313844259415);
+    u = fma(u, g, -2.029356063208368);
     u = fma(u, g, 2.4674011002723395);
     const double lo = g * u, hi = 1.0 - lo;
     const bool odd = ((int)q) & 1;  // |q| <= 4 on the domain
