@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
                                                         const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
+    __shared__ double s_win[kSampleBlock / 64][kRootWin];
     const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
     const int c = L.c;
     const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
 #define GCV_SAMPLE_CASE(PP, DD)                                                                     \
     case 10 * PP + DD:                                                                             \
         u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv64<Dt, PP, DD>::get(T), A, c, g,   \
-                                &seg, L.lo, L.hi);                                                 \
+                                &seg, L.lo, L.hi, (lds_double*)s_win[threadIdx.x >> 6]);          \
         break;
     switch (cls) {
         GCV_SAMPLE_CASE(1, 2)

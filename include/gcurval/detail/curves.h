@@ -171,52 +171,50 @@ __device__ __forceinline__ double sqrt_pos(double v) {
 }
 
 // ---------------------------------------------------------------------------
-// sin^2(2 pi x) and cos^2(2 pi x) for x in [0, 1] from one polynomial: r = 4x
-// and its nearest integer q are exact, f = r - q lies in [-1/2, 1/2], and with
-// g = f^2 the squared quarter-period sine sin^2(pi f / 2) = g U(g) <= 1/2;
-// sin^2(2 pi x) and cos^2(2 pi x) are g U(g) and 1 - g U(g), swapped when q
-// is odd, so both keep full relative accuracy near their zeros.
-// FP32: U is a degree-4 weighted least-squares fit (max relative error
-// 1.2e-7 in FP32 evaluation); FP64: degree 7, the truncated Chebyshev
-// expansion in powers of g (2.2e-16, as the 11-term Taylor series). Derivation:
-// gcurval_amd/fit_sincos.py.
+// sin^2(pi y) for y = 2x + shift, x in [0, 1] and shift 0 or -1/2, from one
+// polynomial: q = rint(y) and e = y - q in [-1/2, 1/2] are exact, and with
+// h = e^2 in [0, 1/4], sin^2(pi y) = sin^2(pi e) = h V(h) keeps full relative
+// accuracy near its zeros. shift 0 gives sin^2(2 pi x), shift -1/2 gives
+// cos^2(2 pi x). FP32: V is a degree-5 weighted least-squares fit (1.9e-7
+// relative for sin^2, 2.3e-7 for cos^2 in FP32 evaluation); FP64: degree 9,
+// the truncated Chebyshev expansion in powers of h (about 2 ulp).
+// Derivation: gcurval_amd/fit_sincos.py.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sincos2_2pi(float x, float& s2, float& c2) {
-    const float r = 4.0f * x;
-    const float q = rintf(r);
-    const float f = r - q;
-    const float g = f * f;
-    const float u = fmaf(fmaf(fmaf(fmaf(0.012307871133089066f, g, -0.1175343245267868f), g, 0.667619526386261f), g,
-                              -2.029355764389038f),
-                         g, 2.4674010276794434f);
-    const float lo = g * u, hi = 1.0f - lo;
-    const bool odd = ((int)q) & 1;
-    s2 = odd ? hi : lo;
-    c2 = odd ? lo : hi;
+__device__ __forceinline__ float sin2_2pi(float x, float shift) {
+    const float y = fmaf(2.0f, x, shift);
+    const float e = y - rintf(y);
+    const float h = e * e;
+    float v = -3.336219310760498f;
+    v = fmaf(v, h, 13.037117004394531f);
+    v = fmaf(v, h, -30.098554611206055f);
+    v = fmaf(v, h, 42.7269287109375f);
+    v = fmaf(v, h, -32.469661712646484f);
+    v = fmaf(v, h, 9.869604110717773f);
+    return h * v;
 }
-__device__ __forceinline__ void sincos2_2pi(double x, double& s2, double& c2) {
-    const double r = 4.0 * x;
-    const double q = rint(r);
-    const double f = r - q;
-    const double g = f * f;
-    double u = -2.0832065862542846e-06;
-    u = fma(u, g, 5.229143784023563e-05);
-    u = fma(u, g, -0.0009647813283021856);
-    u = fma(u, g, 0.012903445013289263);
-    u = fma(u, g, -0.11766531513610962);
-    u = fma(u, g, 0.6676313844259415);
-    u = fma(u, g, -2.029356063208368);
-    u = fma(u, g, 2.4674011002723395);
-    const double lo = g * u, hi = 1.0 - lo;
-    const bool odd = ((int)q) & 1;  // |q| <= 4 on the domain
-    s2 = odd ? hi : lo;
-    c2 = odd ? lo : hi;
+__device__ __forceinline__ double sin2_2pi(double x, double shift) {
+    const double y = fma(2.0, x, shift);
+    const double e = y - rint(y);
+    const double h = e * e;
+    double v = -0.0016981312977267715;
+    v = fma(v, h, 0.0180912904683608);
+    v = fma(v, h, -0.1409737898673586);
+    v = fma(v, h, 0.8571901937854678);
+    v = fma(v, h, -3.9517676198423173);
+    v = fma(v, h, 13.213128353891875);
+    v = fma(v, h, -30.122320684487374);
+    v = fma(v, h, 42.72840860331856);
+    v = fma(v, h, -32.469697011333935);
+    v = fma(v, h, 9.869604401089358);
+    return h * v;
 }
 
 // ---------------------------------------------------------------------------
-// Ellipse evaluator. Ctx holds A^2 and B^2 with A = 2 pi a', B = 2 pi b' and
-// a', b' the optionally normalised semi-axes, so
-// speed(x) = sqrt(A^2 sin^2(2 pi x) + B^2 cos^2(2 pi x)).
+// Ellipse evaluator. With A = 2 pi a', B = 2 pi b' and a', b' the optionally
+// normalised semi-axes, speed(x)^2 = A^2 sin^2(2 pi x) + B^2 cos^2(2 pi x)
+// = base + D S(x), base = min(A^2, B^2), D = |A^2 - B^2| and S the squared
+// sine (A >= B) or cosine (A < B) that vanishes where the speed is smallest:
+// both terms are non-negative, so the sum keeps full relative accuracy.
 // ---------------------------------------------------------------------------
 template <typename Dt>
 struct EllipseEval {
@@ -225,7 +223,7 @@ struct EllipseEval {
     int normalize;
 
     struct Ctx {
-        Dt A2, B2;
+        Dt base, D, shift;
     };
 
     __device__ __forceinline__ double scale(int i) const {
@@ -237,17 +235,15 @@ struct EllipseEval {
     // with the reciprocal of scale(i) already at hand
     __device__ __forceinline__ Ctx load(int i, double inv_scale) const {
         const double A = kTwoPi * a[i] * inv_scale, B = kTwoPi * b[i] * inv_scale;
-        return Ctx{Dt(A * A), Dt(B * B)};
+        const double A2 = A * A, B2 = B * B;
+        const bool sine = A2 >= B2;
+        return Ctx{Dt(sine ? B2 : A2), Dt(sine ? A2 - B2 : B2 - A2), Dt(sine ? 0.0 : -0.5)};
     }
     __device__ __forceinline__ float speed(const Ctx& c, float x) const {
-        float s2, c2;
-        sincos2_2pi(x, s2, c2);
-        return __builtin_amdgcn_sqrtf(fmaf(float(c.A2), s2, float(c.B2) * c2));
+        return __builtin_amdgcn_sqrtf(fmaf(float(c.D), sin2_2pi(x, float(c.shift)), float(c.base)));
     }
     __device__ __forceinline__ double speed(const Ctx& c, double x) const {
-        double s2, c2;
-        sincos2_2pi(x, s2, c2);
-        return sqrt_pos(fma(double(c.A2), s2, double(c.B2) * c2));
+        return sqrt_pos(fma(double(c.D), sin2_2pi(x, double(c.shift)), double(c.base)));
     }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }

@@ -52,6 +52,27 @@ struct HasLengthRule<Ev, decltype((void)Ev::kRule)> {
     static constexpr bool value = Ev::kRule;
 };
 
+// Node j of the N-point rule on [m - k, m + k]: k x_j + m, and m itself for
+// the centre node of odd rules (no multiply by zero).
+template <int N, int J, typename Dt>
+__device__ __forceinline__ Dt gl_node(Dt m, Dt k) {
+    if constexpr (GL<N>::x[J] == 0.0)
+        return m;
+    else
+        return k * Dt(GL<N>::x[J]) + m;
+}
+
+// sum_j w_j speed(node j), started from the first term rather than 0 + term
+template <int N, int J = 0, class Ev, typename Dt>
+__device__ __forceinline__ Dt gl_sum(const Ev& ev, const typename Ev::Ctx& c, Dt m, Dt k, Dt acc) {
+    if constexpr (J == N) {
+        return acc;
+    } else {
+        const Dt t = Dt(GL<N>::w[J]) * ev.speed(c, gl_node<N, J>(m, k));
+        return gl_sum<N, J + 1>(ev, c, m, k, J == 0 ? t : acc + t);
+    }
+}
+
 // n-point quadrature of the evaluator's speed over [x0, x1] (interval-local
 // parameter), in the arithmetic of Dt.
 template <int N, class Ev, typename Dt>
@@ -59,10 +80,7 @@ __device__ __forceinline__ Dt quad(const Ev& ev, const typename Ev::Ctx& c, Dt x
     if constexpr (HasLengthRule<Ev>::value) return ev.length(c, x0, x1);
     const Dt k = (x1 - x0) * Dt(0.5);
     const Dt m = (x1 + x0) * Dt(0.5);
-    Dt acc = Dt(0);
-#pragma unroll
-    for (int j = 0; j < N; j++) acc += Dt(GL<N>::w[j]) * ev.speed(c, k * Dt(GL<N>::x[j]) + m);
-    return acc * k;
+    return gl_sum<N>(ev, c, m, k, Dt(0)) * k;
 }
 
 // ---------------------------------------------------------------------------

@@ -495,6 +495,7 @@ struct SampleArgs {
     int32_t* out_seg;           // optional local interval index per sample
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
     const int32_t* blk_int;     // optional (uniform positions): interval of each sample block's first sample
+    int one_int;                // 1: interval c is curve c (ellipses; ioff is the identity)
 };
 
 // first index in [lo, hi) with a[idx] > v (hi if none)
@@ -517,12 +518,58 @@ __device__ __forceinline__ uint32_t count_below(const double* a, uint32_t n, dou
     return lo;
 }
 // The same over a power-of-two count 2^ls, clamped to 2^ls - 1: one probe per
-// bit of the answer, no data-dependent trip count.
-__device__ __forceinline__ uint32_t count_below_pow2(const double* a, int ls, double v) {
+// bit of the answer, no data-dependent trip count (global or LDS pointer).
+template <class P>
+__device__ __forceinline__ uint32_t count_below_pow2(P a, int ls, double v) {
     uint32_t k = 0;
     for (uint32_t st = (1u << ls) >> 1; st > 0; st >>= 1)
         if (a[k + st - 1] < v) k += st;
     return k;
+}
+
+// Root prefixes one wave stages in LDS for its root searches.
+constexpr int kRootWin = 256;
+typedef __attribute__((address_space(3))) double lds_double;
+
+// Root k of root-local target sn in the 2^ls-root block at root_prefix + ro,
+// with the prefix before it and its length. When the whole wave is active
+// and the blocks of its lanes lie in the kRootWin prefixes that start at the
+// first lane's block and end at the last lane's (targets grow along a wave
+// for uniform positions), the wave reads that window once, coalesced, into
+// its LDS slice `win` and every lane searches there; otherwise (or with no
+// slice) each lane searches global memory. Both give the same k.
+__device__ __forceinline__ void find_root(const double* root_prefix, lds_double* win, int64_t ro, int ls, double sn,
+                                          int& k, double& pre, double& rlen) {
+    if (win) {
+        const int64_t r0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)ro >> 32))
+                                      << 32) |
+                                     (uint32_t)__builtin_amdgcn_readfirstlane((int)ro));
+        const int64_t end = ro + (int64_t(1) << ls) - r0;  // this lane's block end in the window
+        const int span = __builtin_amdgcn_readlane((int)(end < kRootWin ? end : kRootWin + 1), 63);
+        const bool fits = ro >= r0 && end <= span && span <= kRootWin;
+        if (__ballot(1) == ~0ull && __ballot(fits) == ~0ull) {
+            const int lane = lane_id();
+            double v[kRootWin / 64];
+#pragma unroll
+            for (int q = 0; q < kRootWin / 64; q++)
+                if (lane + 64 * q < span) v[q] = root_prefix[r0 + lane + 64 * q];
+#pragma unroll
+            for (int q = 0; q < kRootWin / 64; q++)
+                if (lane + 64 * q < span) win[lane + 64 * q] = v[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const lds_double* w = win + (ro - r0);
+            k = (int)count_below_pow2(w, ls, sn);
+            pre = k > 0 ? w[k - 1] : 0.0;
+            rlen = w[k] - pre;
+            return;
+        }
+    }
+    const double* rp = root_prefix + ro;
+    k = (int)count_below_pow2(rp, ls, sn);
+    pre = k > 0 ? rp[k - 1] : 0.0;
+    rlen = rp[k] - pre;
 }
 
 // Reciprocal for Newton updates and start values: one v_rcp (1 ulp) in float,
@@ -536,10 +583,7 @@ __device__ __forceinline__ Dt quad_from(const Ev& ev, const typename Ev::Ctx& c,
     if constexpr (HasLengthRule<Ev>::value) return ev.length(c, a, a + d);
     const Dt k = d * Dt(0.5);
     const Dt m = a + k;
-    Dt acc = Dt(0);
-#pragma unroll
-    for (int j = 0; j < N; j++) acc += Dt(GL<N>::w[j]) * ev.speed(c, m + k * Dt(GL<N>::x[j]));
-    return acc * k;
+    return gl_sum<N>(ev, c, m, k, Dt(0)) * k;
 }
 
 // Composite rule: `pieces` (1 or 2) equal sub-ranges of [a, a + d].
@@ -708,10 +752,10 @@ static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, con
 template <typename Dt, int NGL, class Ev, class Ev64>
 __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
                                              int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
-                                             int64_t hint_hi = INT64_MAX) {
+                                             int64_t hint_hi = INT64_MAX, lds_double* win = nullptr) {
     const double s = sample_target(A, c, g);
-    const int64_t ib = A.ioff[c];
-    const int64_t i = interval_of_target(A, c, s, hint_lo, hint_hi);
+    const int64_t ib = A.one_int ? c : A.ioff[c];
+    const int64_t i = A.one_int ? c : interval_of_target(A, c, s, hint_lo, hint_hi);
     if (seg_out) *seg_out = (int32_t)(i - ib);
     const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
     const double inv_sc = rcp_f64(ev.scale((int)i));
@@ -726,10 +770,9 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         ro = A.root_off[i];
         ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
     }
-    const double* rp = A.root_prefix + ro;
-    const int k = (int)count_below_pow2(rp, ls, sn);
-    const double pre = k > 0 ? rp[k - 1] : 0.0;
-    const double rlen = rp[k] - pre;
+    int k;
+    double pre, rlen;
+    find_root(A.root_prefix, win, ro, ls, sn, k, pre, rlen);
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
 
@@ -796,7 +839,9 @@ __device__ __forceinline__ SampleLoc locate_sample(const SampleArgs& A, const in
     const int64_t c0 = blk_curve[blk];
     const bool has_next = blk + 1 < n_blk;
     const int64_t c1 = has_next ? blk_curve[blk + 1] : A.n_curve - 1;
-    SampleLoc L{(int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1), 0, INT64_MAX};
+    // a block whose first sample and the next block's are on one curve lies
+    // on that curve
+    SampleLoc L{c0 == c1 ? (int)c0 : (int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1), 0, INT64_MAX};
     if (A.blk_int) {
         if (L.c == c0) L.lo = (int64_t)A.blk_int[blk] - 1;
         if (has_next && L.c == c1) L.hi = (int64_t)A.blk_int[blk + 1] + 1;
@@ -808,9 +853,10 @@ template <typename Dt, int NGL, class Ev, class Ev64>
 __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
+    __shared__ double s_win[kSampleBlock / 64][kRootWin];
     const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
     int32_t seg;
-    const double u = sample_one<Dt, NGL>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi);
+    const double u = sample_one<Dt, NGL>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi, (lds_double*)s_win[threadIdx.x >> 6]);
     A.out_u[g] = u;
     if (A.out_seg) A.out_seg[g] = seg;
 }

@@ -99,6 +99,30 @@ def test_ellipse_batch_fp32(ctx):
         assert eng.stats()["overflow"] == 0
 
 
+@pytest.mark.parametrize("prec", [G.GCV_FP32, G.GCV_FP64])
+def test_tall_ellipses_b_above_a(ctx, prec):
+    """b > a: the speed's minimum moves to the quarter points (the squared
+    cosine branch of the evaluator). Lengths equal those of the swapped
+    ellipses (same curve, rotated); samples against the truth."""
+    wide = W.ellipse_batch(120, seed=23)
+    tall = G.EllipseBatch(wide.b, wide.a)
+    counts = W.ellipse_sample_counts(120, seed=24, lo=16, hi=256)
+    tr = O.Truth(tall)
+    Lt, _ = tr.lengths()
+    len_bar, u_floor = (1e-6, 1e-6) if prec == G.GCV_FP32 else (1e-10, 1e-9)
+    with build(ctx, tall, precision=prec, normalize=True) as eng, build(ctx, wide, precision=prec,
+                                                                            normalize=True) as ew:
+        Lg = eng.getAllCurveArclengths()
+        assert np.max(np.abs(Lg - Lt) / Lt) <= len_bar
+        assert np.max(np.abs(Lg - ew.getAllCurveArclengths()) / Lt) <= 2 * len_bar
+        pos = O.uniform_positions(counts)
+        u = eng.arclengthReverse(counts, None, True)
+        ut, _ = tr.reverse(counts, pos)
+        bound = np.repeat(np.maximum(u_floor, 2 * len_bar / 10 * Lt / np.minimum(tall.a, tall.b)), counts)
+        assert np.all(np.abs(u - ut) <= bound), float(np.max(np.abs(u - ut) - bound))
+        assert eng.stats()["overflow"] == 0
+
+
 def test_reference_sample1_dataset_lengths(ctx):
     b = W.reference_sample1_ellipses(1000)
     ref = O.RefGpuPlain(b, O.opts(0, 3))
