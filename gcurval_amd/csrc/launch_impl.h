@@ -119,8 +119,22 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
     EllipseEval<double> ev64{a, b, normalize};
     const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_ellipse");
-    hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>>), dim3((unsigned)blocks),
-                       dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve);
+    // the common modes (uniform or explicit positions, plain or seeded roots)
+    // are compiled in, so the kernel has no branch between its loads
+    const bool pos = A.pos != nullptr, seeded = A.root_off != nullptr;
+#define GCV_ELLIPSE_SAMPLE(P, S)                                                                          \
+    hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, StaticMode<P, S>>),       \
+                       dim3((unsigned)blocks), dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve)
+    if (pos && seeded) {
+        GCV_ELLIPSE_SAMPLE(true, true);
+    } else if (pos) {
+        GCV_ELLIPSE_SAMPLE(true, false);
+    } else if (seeded) {
+        GCV_ELLIPSE_SAMPLE(false, true);
+    } else {
+        GCV_ELLIPSE_SAMPLE(false, false);
+    }
+#undef GCV_ELLIPSE_SAMPLE
     hipError_t e = hipGetLastError();
     prof_end(c.owner);
     return e;

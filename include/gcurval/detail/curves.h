@@ -218,6 +218,7 @@ __device__ __forceinline__ double sin2_2pi(double x, double shift) {
 // ---------------------------------------------------------------------------
 template <typename Dt>
 struct EllipseEval {
+    static constexpr bool kOneInterval = true;  // one knot interval per curve
     const double* a;
     const double* b;
     int normalize;
@@ -227,9 +228,11 @@ struct EllipseEval {
     };
 
     __device__ __forceinline__ double scale(int i) const {
-        if (!normalize) return 1.0;
-        double s = fmax(fabs(a[i]), fabs(b[i]));
-        return s > 0.0 ? s : 1.0;
+        // the loads are unconditional (ev.load reads the same two values), so
+        // they issue with the caller's other per-interval loads
+        const double ai = a[i], bi = b[i];
+        const double s = fmax(fabs(ai), fabs(bi));
+        return normalize && s > 0.0 ? s : 1.0;
     }
     __device__ __forceinline__ Ctx load(int i) const { return load(i, rcp_f64(scale(i))); }
     // with the reciprocal of scale(i) already at hand

@@ -495,7 +495,6 @@ struct SampleArgs {
     int32_t* out_seg;           // optional local interval index per sample
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
     const int32_t* blk_int;     // optional (uniform positions): interval of each sample block's first sample
-    int one_int;                // 1: interval c is curve c (ellipses; ioff is the identity)
 };
 
 // first index in [lo, hi) with a[idx] > v (hi if none)
@@ -531,42 +530,40 @@ __device__ __forceinline__ uint32_t count_below_pow2(P a, int ls, double v) {
 constexpr int kRootWin = 256;
 typedef __attribute__((address_space(3))) double lds_double;
 
-// Root k of root-local target sn in the 2^ls-root block at root_prefix + ro,
-// with the prefix before it and its length. When the whole wave is active
-// and the blocks of its lanes lie in the kRootWin prefixes that start at the
+// LDS window of root prefixes for one wave: when the whole wave is active and
+// the root blocks of its lanes lie in the kRootWin prefixes that start at the
 // first lane's block and end at the last lane's (targets grow along a wave
 // for uniform positions), the wave reads that window once, coalesced, into
-// its LDS slice `win` and every lane searches there; otherwise (or with no
-// slice) each lane searches global memory. Both give the same k.
-__device__ __forceinline__ void find_root(const double* root_prefix, lds_double* win, int64_t ro, int ls, double sn,
-                                          int& k, double& pre, double& rlen) {
-    if (win) {
-        const int64_t r0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)ro >> 32))
-                                      << 32) |
-                                     (uint32_t)__builtin_amdgcn_readfirstlane((int)ro));
-        const int64_t end = ro + (int64_t(1) << ls) - r0;  // this lane's block end in the window
-        const int span = __builtin_amdgcn_readlane((int)(end < kRootWin ? end : kRootWin + 1), 63);
-        const bool fits = ro >= r0 && end <= span && span <= kRootWin;
-        if (__ballot(1) == ~0ull && __ballot(fits) == ~0ull) {
-            const int lane = lane_id();
-            double v[kRootWin / 64];
+// its LDS slice `win`, and each lane's block starts at win + off. Returns
+// false (nothing staged) otherwise.
+__device__ __forceinline__ bool stage_roots(const double* root_prefix, lds_double* win, int64_t ro, int ls,
+                                            int& off) {
+    const int64_t r0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)ro >> 32))
+                                  << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)ro));
+    const int64_t end = ro + (int64_t(1) << ls) - r0;  // this lane's block end in the window
+    const int span = __builtin_amdgcn_readlane((int)(end < kRootWin ? end : kRootWin + 1), 63);
+    const bool fits = ro >= r0 && end <= span && span <= kRootWin;
+    if (__ballot(1) != ~0ull || __ballot(fits) != ~0ull) return false;
+    const int lane = lane_id();
+    double v[kRootWin / 64];
 #pragma unroll
-            for (int q = 0; q < kRootWin / 64; q++)
-                if (lane + 64 * q < span) v[q] = root_prefix[r0 + lane + 64 * q];
+    for (int q = 0; q < kRootWin / 64; q++)
+        if (lane + 64 * q < span) v[q] = root_prefix[r0 + lane + 64 * q];
 #pragma unroll
-            for (int q = 0; q < kRootWin / 64; q++)
-                if (lane + 64 * q < span) win[lane + 64 * q] = v[q];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const lds_double* w = win + (ro - r0);
-            k = (int)count_below_pow2(w, ls, sn);
-            pre = k > 0 ? w[k - 1] : 0.0;
-            rlen = w[k] - pre;
-            return;
-        }
-    }
-    const double* rp = root_prefix + ro;
+    for (int q = 0; q < kRootWin / 64; q++)
+        if (lane + 64 * q < span) win[lane + 64 * q] = v[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    off = (int)(ro - r0);
+    return true;
+}
+
+// Root k of root-local target sn in a 2^ls-root block of inclusive prefixes
+// (global or LDS), with the prefix before it and its length.
+template <class P>
+__device__ __forceinline__ void find_root(P rp, int ls, double sn, int& k, double& pre, double& rlen) {
     k = (int)count_below_pow2(rp, ls, sn);
     pre = k > 0 ? rp[k - 1] : 0.0;
     rlen = rp[k] - pre;
@@ -710,20 +707,51 @@ __device__ __forceinline__ typename Ev64::Ctx widen_or_load(const Ev64& ev64, co
 // Samples per block of the sampling kernels (blockDim.x).
 constexpr int kSampleBlock = 256;
 
-// Target arc length of sample g (curve c) in curve units.
+// Sampling mode: explicit positions (pos) and seeded root offsets (root_off)
+// read from the arguments at run time, or fixed at compile time so that a
+// kernel has no branch between its loads.
+struct RuntimeMode {
+    static constexpr bool kStatic = false, kPos = false, kSeeded = false;
+};
+template <bool P, bool S>
+struct StaticMode {
+    static constexpr bool kStatic = true, kPos = P, kSeeded = S;
+};
+template <class M>
+__device__ __forceinline__ bool mode_pos(const SampleArgs& A) {
+    if constexpr (M::kStatic) return M::kPos; else return A.pos != nullptr;
+}
+template <class M>
+__device__ __forceinline__ bool mode_seeded(const SampleArgs& A) {
+    if constexpr (M::kStatic) return M::kSeeded; else return A.root_off != nullptr;
+}
+
+// Target arc length of sample g (curve c) in curve units. The curve's loads
+// come first and unconditionally (their addresses are always valid), so they
+// are in flight together.
+template <class M = RuntimeMode>
 __device__ __forceinline__ double sample_target(const SampleArgs& A, int c, int64_t g) {
+    const int64_t s0 = A.soff[c], s1 = A.soff[c + 1];
+    const double cl = A.curve_len[c];
     double pos;
-    if (A.pos) {
+    if (mode_pos<M>(A)) {
         pos = A.pos[g];
     } else {
-        const int64_t s0 = A.soff[c];
-        const int64_t N = A.soff[c + 1] - s0, j = g - s0;
-        // j / (N - 1) (closed) or j / N to within 1.5 ulp; a position a rounding
-        // above 1 is clamped to the curve's end by the searches below
-        pos = A.uniform_closed ? (N > 1 ? (double)j * rcp_f64((double)(N - 1)) : 0.0) : (double)j * rcp_f64((double)N);
+        // j / (N - 1) (closed; 0 for N = 1) or j / N to within 1.5 ulp; a
+        // position a rounding above 1 is clamped to the curve's end by the
+        // searches below
+        const int64_t N = s1 - s0, j = g - s0;
+        const double den = (double)(A.uniform_closed ? N - 1 : N);
+        pos = den > 0.0 ? (double)j * rcp_f64(den) : 0.0;
     }
-    return A.is_relative ? pos * A.curve_len[c] : pos;
+    return A.is_relative ? pos * cl : pos;
 }
+
+// Evaluators whose curves are one knot interval each (interval i = curve i).
+template <class Ev, class = void>
+struct OneIntervalPerCurve : std::false_type {};
+template <class Ev>
+struct OneIntervalPerCurve<Ev, std::enable_if_t<Ev::kOneInterval>> : std::true_type {};
 
 // Interval of target s on curve c: the first cumulative length >= s among the
 // curve's first n_int - 1, so a target past the last boundary lands in the
@@ -749,37 +777,45 @@ static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, con
     blk_int[b] = (int32_t)interval_of_target(A, c, s, 0, INT64_MAX);
 }
 
-template <typename Dt, int NGL, class Ev, class Ev64>
+template <typename Dt, int NGL, class M = RuntimeMode, class Ev, class Ev64>
 __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
                                              int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
                                              int64_t hint_hi = INT64_MAX, lds_double* win = nullptr) {
-    const double s = sample_target(A, c, g);
-    const int64_t ib = A.one_int ? c : A.ioff[c];
-    const int64_t i = A.one_int ? c : interval_of_target(A, c, s, hint_lo, hint_hi);
+    constexpr bool one = OneIntervalPerCurve<Ev>::value;
+    const double s = sample_target<M>(A, c, g);
+    const int64_t ib = one ? (int64_t)c : A.ioff[c];
+    const int64_t i = one ? (int64_t)c : interval_of_target(A, c, s, hint_lo, hint_hi);
     if (seg_out) *seg_out = (int32_t)(i - ib);
-    const double sl = s - (i > ib ? A.int_cum[i - 1] : 0.0);
-    const double inv_sc = rcp_f64(ev.scale((int)i));
-    const double sn = sl * inv_sc;
-    // the evaluator context depends on the interval only: its loads are in
-    // flight during the root search
-    const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
+    // every load that depends on the interval only is issued before the root
+    // window is staged, so one wait covers them all
+    const double prev = !one && i > ib ? A.int_cum[i - 1] : 0.0;
+    const double sc = ev.scale((int)i);
+    const uint64_t deep = A.deep[i];
     int ls = A.log2_nsub;
     int64_t ro = (int64_t)i << ls;
-    const Dt eps0 = (Dt)A.eps0;
-    if (A.root_off) {
+    if (mode_seeded<M>(A)) {
         ro = A.root_off[i];
         ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
     }
+    int off = 0;
+    const bool staged = win && stage_roots(A.root_prefix, win, ro, ls, off);
+    const double inv_sc = rcp_f64(sc);
+    const double sn = (s - prev) * inv_sc;
+    const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
+    const Dt eps0 = (Dt)A.eps0;
     int k;
     double pre, rlen;
-    find_root(A.root_prefix, win, ro, ls, sn, k, pre, rlen);
+    if (staged)
+        find_root((const lds_double*)(win + off), ls, sn, k, pre, rlen);
+    else
+        find_root(A.root_prefix + ro, ls, sn, k, pre, rlen);
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
 
     Dt a, b, xm;
     task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
     Dt len = (Dt)rlen;
-    if ((A.deep[i] >> k) & 1ull) {
+    if ((deep >> k) & 1ull) {
         double tl;
         const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
         walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
@@ -849,14 +885,24 @@ __device__ __forceinline__ SampleLoc locate_sample(const SampleArgs& A, const in
     return L;
 }
 
-template <typename Dt, int NGL, class Ev, class Ev64>
+template <typename Dt, int NGL, class Ev, class Ev64, class M = RuntimeMode>
 __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
     __shared__ double s_win[kSampleBlock / 64][kRootWin];
-    const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
+    lds_double* win = (lds_double*)s_win[threadIdx.x >> 6];
     int32_t seg;
-    const double u = sample_one<Dt, NGL>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi, (lds_double*)s_win[threadIdx.x >> 6]);
+    double u;
+    const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
+    const int c0 = blk_curve[blockIdx.x];
+    if (!A.blk_int && blockIdx.x + 1 < n_blk && blk_curve[blockIdx.x + 1] == c0) {
+        // the whole block lies on curve c0: the curve index is block-uniform,
+        // so its per-curve data comes through scalar loads issued together
+        u = sample_one<Dt, NGL, M>(ev, ev64, A, c0, g, &seg, 0, INT64_MAX, win);
+    } else {
+        const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
+        u = sample_one<Dt, NGL, M>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi, win);
+    }
     A.out_u[g] = u;
     if (A.out_seg) A.out_seg[g] = seg;
 }
