@@ -533,31 +533,34 @@ typedef __attribute__((address_space(3))) double lds_double;
 // LDS window of root prefixes for one wave: when the whole wave is active and
 // the root blocks of its lanes lie in the kRootWin prefixes that start at the
 // first lane's block and end at the last lane's (targets grow along a wave
-// for uniform positions), the wave reads that window once, coalesced, into
-// its LDS slice `win`, and each lane's block starts at win + off. Returns
-// false (nothing staged) otherwise.
-__device__ __forceinline__ bool stage_roots(const double* root_prefix, lds_double* win, int64_t ro, int ls,
-                                            int& off) {
+// for uniform positions), the wave reads that window once, coalesced
+// (fetch_roots, into registers v), then writes it to its LDS slice
+// (store_roots), where each lane's block starts at offset `off`.
+// fetch_roots returns false (nothing read) otherwise.
+__device__ __forceinline__ bool fetch_roots(const double* root_prefix, int64_t ro, int ls,
+                                            double (&v)[kRootWin / 64], int& off, int& span) {
     const int64_t r0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)ro >> 32))
                                   << 32) |
                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)ro));
     const int64_t end = ro + (int64_t(1) << ls) - r0;  // this lane's block end in the window
-    const int span = __builtin_amdgcn_readlane((int)(end < kRootWin ? end : kRootWin + 1), 63);
+    span = __builtin_amdgcn_readlane((int)(end < kRootWin ? end : kRootWin + 1), 63);
     const bool fits = ro >= r0 && end <= span && span <= kRootWin;
     if (__ballot(1) != ~0ull || __ballot(fits) != ~0ull) return false;
     const int lane = lane_id();
-    double v[kRootWin / 64];
 #pragma unroll
     for (int q = 0; q < kRootWin / 64; q++)
         if (lane + 64 * q < span) v[q] = root_prefix[r0 + lane + 64 * q];
+    off = (int)(ro - r0);
+    return true;
+}
+__device__ __forceinline__ void store_roots(lds_double* win, const double (&v)[kRootWin / 64], int span) {
+    const int lane = lane_id();
 #pragma unroll
     for (int q = 0; q < kRootWin / 64; q++)
         if (lane + 64 * q < span) win[lane + 64 * q] = v[q];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    off = (int)(ro - r0);
-    return true;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Root k of root-local target sn in a 2^ls-root block of inclusive prefixes
@@ -726,25 +729,41 @@ __device__ __forceinline__ bool mode_seeded(const SampleArgs& A) {
     if constexpr (M::kStatic) return M::kSeeded; else return A.root_off != nullptr;
 }
 
-// Target arc length of sample g (curve c) in curve units. The curve's loads
-// come first and unconditionally (their addresses are always valid), so they
-// are in flight together.
-template <class M = RuntimeMode>
-__device__ __forceinline__ double sample_target(const SampleArgs& A, int c, int64_t g) {
-    const int64_t s0 = A.soff[c], s1 = A.soff[c + 1];
-    const double cl = A.curve_len[c];
-    double pos;
-    if (mode_pos<M>(A)) {
-        pos = A.pos[g];
-    } else {
+// Target arc length of sample g (curve c) in curve units, in two parts: the
+// curve's loads (target_in; unconditional, their addresses are always valid)
+// and the arithmetic (target_of), so a caller can issue its other loads in
+// between.
+struct TargetIn {
+    int64_t s0, s1;  // the curve's sample offsets
+    double cl;       // its length
+    double pv;       // explicit position of the sample
+};
+template <class M>
+__device__ __forceinline__ TargetIn target_in(const SampleArgs& A, int c, int64_t g) {
+    TargetIn t;
+    t.s0 = A.soff[c];
+    t.s1 = A.soff[c + 1];
+    t.cl = A.curve_len[c];
+    t.pv = mode_pos<M>(A) ? A.pos[g] : 0.0;
+    return t;
+}
+template <class M>
+__device__ __forceinline__ double target_of(const SampleArgs& A, const TargetIn& t, int64_t g) {
+    double pos = t.pv;
+    if (!mode_pos<M>(A)) {
         // j / (N - 1) (closed; 0 for N = 1) or j / N to within 1.5 ulp; a
         // position a rounding above 1 is clamped to the curve's end by the
         // searches below
-        const int64_t N = s1 - s0, j = g - s0;
+        const int64_t N = t.s1 - t.s0, j = g - t.s0;
         const double den = (double)(A.uniform_closed ? N - 1 : N);
-        pos = den > 0.0 ? (double)j * rcp_f64(den) : 0.0;
+        const double r = rcp_f64(den > 0.0 ? den : 1.0);
+        pos = den > 0.0 ? (double)j * r : 0.0;
     }
-    return A.is_relative ? pos * cl : pos;
+    return A.is_relative ? pos * t.cl : pos;
+}
+template <class M = RuntimeMode>
+__device__ __forceinline__ double sample_target(const SampleArgs& A, int c, int64_t g) {
+    return target_of<M>(A, target_in<M>(A, c, g), g);
 }
 
 // Evaluators whose curves are one knot interval each (interval i = curve i).
@@ -782,7 +801,10 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
                                              int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
                                              int64_t hint_hi = INT64_MAX, lds_double* win = nullptr) {
     constexpr bool one = OneIntervalPerCurve<Ev>::value;
-    const double s = sample_target<M>(A, c, g);
+    const TargetIn tin = target_in<M>(A, c, g);
+    // one-interval evaluators know the interval before the target: their
+    // target arithmetic waits until the interval's loads below are issued
+    double s = one ? 0.0 : target_of<M>(A, tin, g);
     const int64_t ib = one ? (int64_t)c : A.ioff[c];
     const int64_t i = one ? (int64_t)c : interval_of_target(A, c, s, hint_lo, hint_hi);
     if (seg_out) *seg_out = (int32_t)(i - ib);
@@ -797,11 +819,20 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         ro = A.root_off[i];
         ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
     }
-    int off = 0;
-    const bool staged = win && stage_roots(A.root_prefix, win, ro, ls, off);
+    int off = 0, span = 0;
+    double rv[kRootWin / 64];
+    const bool staged = win && fetch_roots(A.root_prefix, ro, ls, rv, off, span);
+    // one-interval evaluators read their context from the loads above: the
+    // window is written after it (the wave barrier would force a reload);
+    // the others load their context after the window is in LDS
+    if constexpr (!one)
+        if (staged) store_roots(win, rv, span);
+    if constexpr (one) s = target_of<M>(A, tin, g);
     const double inv_sc = rcp_f64(sc);
     const double sn = (s - prev) * inv_sc;
     const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
+    if constexpr (one)
+        if (staged) store_roots(win, rv, span);
     const Dt eps0 = (Dt)A.eps0;
     int k;
     double pre, rlen;
