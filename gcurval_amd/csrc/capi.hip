@@ -1273,6 +1273,7 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     A.uniform_closed = uniform_closed;
     A.is_relative = is_relative;
     A.ioff = t->ioff;
+    A.n_int = t->n_int;
     A.int_cum = t->int_cum;
     A.curve_len = t->curve_len;
     A.root_prefix = t->root_prefix;

@@ -142,12 +142,13 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
 
 // Mixed-class B-spline sampler: the class is uniform within a curve, so a
 // wave diverges only where its samples straddle curves of different classes.
-template <typename Dt, int NGL>
+template <typename Dt, int NGL, class M>
 __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A,
                                                         const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
     __shared__ double s_win[kSampleBlock / 64][kRootWin];
+    __shared__ double s_iwin[kSampleBlock / 64][64];
     const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
     const int c = L.c;
     const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
@@ -155,8 +156,9 @@ __global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int u
     double u = 0.0;
 #define GCV_SAMPLE_CASE(PP, DD)                                                                     \
     case 10 * PP + DD:                                                                             \
-        u = sample_one<Dt, NGL>(MakeEv<Dt, PP, DD>::get(T), MakeEv64<Dt, PP, DD>::get(T), A, c, g,   \
-                                &seg, L.lo, L.hi, (lds_double*)s_win[threadIdx.x >> 6]);          \
+        u = sample_one<Dt, NGL, M>(MakeEv<Dt, PP, DD>::get(T), MakeEv64<Dt, PP, DD>::get(T), A, c, g, \
+                                   &seg, L.lo, L.hi, (lds_double*)s_win[threadIdx.x >> 6],          \
+                                   (lds_double*)s_iwin[threadIdx.x >> 6]);                          \
         break;
     switch (cls) {
         GCV_SAMPLE_CASE(1, 2)
@@ -183,8 +185,20 @@ hipError_t Launch<Dt, NGL>::sample_bspline(const LaunchCfg& c, int uniform_class
     if (A.n_total <= 0) return hipSuccess;
     const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_bspline");
-    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL>), dim3((unsigned)blocks), dim3(kSampleBlock), 0, c.stream, T,
-                       uniform_class, A, blk_curve);
+    const bool pos = A.pos != nullptr, seeded = A.root_off != nullptr;
+#define GCV_BSPLINE_SAMPLE(P, S)                                                                        \
+    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL, StaticMode<P, S>>), dim3((unsigned)blocks),          \
+                       dim3(kSampleBlock), 0, c.stream, T, uniform_class, A, blk_curve)
+    if (pos && seeded) {
+        GCV_BSPLINE_SAMPLE(true, true);
+    } else if (pos) {
+        GCV_BSPLINE_SAMPLE(true, false);
+    } else if (seeded) {
+        GCV_BSPLINE_SAMPLE(false, true);
+    } else {
+        GCV_BSPLINE_SAMPLE(false, false);
+    }
+#undef GCV_BSPLINE_SAMPLE
     hipError_t e = hipGetLastError();
     prof_end(c.owner);
     return e;

@@ -495,6 +495,7 @@ struct SampleArgs {
     int32_t* out_seg;           // optional local interval index per sample
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
     const int32_t* blk_int;     // optional (uniform positions): interval of each sample block's first sample
+    int64_t n_int;              // knot intervals of the batch (0: no interval windows)
 };
 
 // first index in [lo, hi) with a[idx] > v (hi if none)
@@ -508,7 +509,8 @@ __device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t lo,
 
 // Number of entries of the non-decreasing a[0, n) below v, i.e. the first
 // index with a[idx] >= v (n if none); 32-bit offsets from a 64-bit base.
-__device__ __forceinline__ uint32_t count_below(const double* a, uint32_t n, double v) {
+template <class P>
+__device__ __forceinline__ uint32_t count_below(P a, uint32_t n, double v) {
     uint32_t lo = 0;
     while (n > 0) {
         const uint32_t h = n >> 1;
@@ -796,21 +798,70 @@ static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, con
     blk_int[b] = (int32_t)interval_of_target(A, c, s, 0, INT64_MAX);
 }
 
+__device__ __forceinline__ int64_t readfirstlane64(int64_t v) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)v >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
+// Interval of sample g on curve c (several intervals per curve) and the
+// cumulative length before it. The search range [hint_lo, hint_hi] comes from
+// the block interval map; when it is the same for the whole (fully active)
+// wave and holds at most 63 intervals, the wave reads int_cum over it (plus
+// the entry before) once, coalesced, into its LDS slice `iwin` in the same
+// round trip as the curve's offsets, and searches there. Same result as the
+// global search.
+template <class M>
+__device__ __forceinline__ int64_t locate_interval(const SampleArgs& A, const TargetIn& tin, int c, int64_t g,
+                                                   int64_t hint_lo, int64_t hint_hi, lds_double* iwin, double& s,
+                                                   int64_t& ib, double& prev) {
+    const int64_t b0 = A.ioff[c], ie = A.ioff[c + 1];
+    int64_t w0 = 0;
+    bool ok = false;
+    double v = 0.0;
+    const int lane = lane_id();
+    if (iwin) {
+        const int64_t lo_u = readfirstlane64(hint_lo), hi_u = readfirstlane64(hint_hi);
+        w0 = lo_u - 1 > 0 ? lo_u - 1 : 0;
+        const int64_t w1 = hi_u < A.n_int - 1 ? hi_u : A.n_int - 1;
+        const int64_t wn = w1 - w0 + 1;
+        ok = __ballot(1) == ~0ull && __ballot(hint_lo == lo_u && hint_hi == hi_u) == ~0ull && wn >= 1 && wn <= 64;
+        if (ok) v = A.int_cum[w0 + (lane < wn ? lane : wn - 1)];
+    }
+    s = target_of<M>(A, tin, g);
+    int64_t lo = hint_lo > b0 ? hint_lo : b0;
+    const int64_t hi = hint_hi < ie - 1 ? hint_hi : ie - 1;
+    if (lo > hi) lo = hi;  // a stale hint: still a valid interval of the curve
+    int64_t i;
+    if (ok) {
+        iwin[lane] = v;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        i = lo + count_below((const lds_double*)(iwin + (lo - w0)), (uint32_t)(hi - lo), s);
+        prev = i > b0 ? iwin[i - 1 - w0] : 0.0;
+    } else {
+        i = lo + count_below(A.int_cum + lo, (uint32_t)(hi - lo), s);
+        prev = i > b0 ? A.int_cum[i - 1] : 0.0;
+    }
+    ib = b0;
+    return i;
+}
+
 template <typename Dt, int NGL, class M = RuntimeMode, class Ev, class Ev64>
 __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
                                              int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
-                                             int64_t hint_hi = INT64_MAX, lds_double* win = nullptr) {
+                                             int64_t hint_hi = INT64_MAX, lds_double* win = nullptr,
+                                             lds_double* iwin = nullptr) {
     constexpr bool one = OneIntervalPerCurve<Ev>::value;
     const TargetIn tin = target_in<M>(A, c, g);
     // one-interval evaluators know the interval before the target: their
     // target arithmetic waits until the interval's loads below are issued
-    double s = one ? 0.0 : target_of<M>(A, tin, g);
-    const int64_t ib = one ? (int64_t)c : A.ioff[c];
-    const int64_t i = one ? (int64_t)c : interval_of_target(A, c, s, hint_lo, hint_hi);
+    double s = 0.0, prev = 0.0;
+    int64_t ib = c, i = c;
+    if constexpr (!one) i = locate_interval<M>(A, tin, c, g, hint_lo, hint_hi, iwin, s, ib, prev);
     if (seg_out) *seg_out = (int32_t)(i - ib);
     // every load that depends on the interval only is issued before the root
     // window is staged, so one wait covers them all
-    const double prev = !one && i > ib ? A.int_cum[i - 1] : 0.0;
     const double sc = ev.scale((int)i);
     const uint64_t deep = A.deep[i];
     int ls = A.log2_nsub;
@@ -822,17 +873,14 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     int off = 0, span = 0;
     double rv[kRootWin / 64];
     const bool staged = win && fetch_roots(A.root_prefix, ro, ls, rv, off, span);
-    // one-interval evaluators read their context from the loads above: the
-    // window is written after it (the wave barrier would force a reload);
-    // the others load their context after the window is in LDS
-    if constexpr (!one)
-        if (staged) store_roots(win, rv, span);
     if constexpr (one) s = target_of<M>(A, tin, g);
     const double inv_sc = rcp_f64(sc);
     const double sn = (s - prev) * inv_sc;
+    // the context's loads go out with the root window's; the window is
+    // written to LDS after them (a wave barrier before them would hold them
+    // back, or force loads above to be repeated)
     const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
-    if constexpr (one)
-        if (staged) store_roots(win, rv, span);
+    if (staged) store_roots(win, rv, span);
     const Dt eps0 = (Dt)A.eps0;
     int k;
     double pre, rlen;
@@ -922,6 +970,11 @@ __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, 
     if (g >= A.n_total) return;
     __shared__ double s_win[kSampleBlock / 64][kRootWin];
     lds_double* win = (lds_double*)s_win[threadIdx.x >> 6];
+    lds_double* iwin = nullptr;
+    if constexpr (!OneIntervalPerCurve<Ev>::value) {
+        __shared__ double s_iwin[kSampleBlock / 64][64];
+        iwin = (lds_double*)s_iwin[threadIdx.x >> 6];
+    }
     int32_t seg;
     double u;
     const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
@@ -929,10 +982,10 @@ __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, 
     if (!A.blk_int && blockIdx.x + 1 < n_blk && blk_curve[blockIdx.x + 1] == c0) {
         // the whole block lies on curve c0: the curve index is block-uniform,
         // so its per-curve data comes through scalar loads issued together
-        u = sample_one<Dt, NGL, M>(ev, ev64, A, c0, g, &seg, 0, INT64_MAX, win);
+        u = sample_one<Dt, NGL, M>(ev, ev64, A, c0, g, &seg, 0, INT64_MAX, win, iwin);
     } else {
         const SampleLoc L = locate_sample(A, blk_curve, blockIdx.x, g);
-        u = sample_one<Dt, NGL, M>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi, win);
+        u = sample_one<Dt, NGL, M>(ev, ev64, A, L.c, g, &seg, L.lo, L.hi, win, iwin);
     }
     A.out_u[g] = u;
     if (A.out_seg) A.out_seg[g] = seg;
