@@ -148,6 +148,40 @@ def copy_calibration(G, mib, steps):
                           "timed_ms": ms, "steps": steps}), flush=True)
 
 
+def valu_calibration(G, steps):
+    """Sustained VALU rates of the library's chain kernels (FP32 FMA, FP64
+    FMA, FP32 sqrt): 8 independent chains per lane over a grid of 16384
+    blocks x 256 lanes (8 waves on every SIMD of 256 CUs, 8 rounds), timed
+    by HIP events on the library stream."""
+    n_blocks, iters = 16384, 2048
+    with G.Context(0) as ctx:
+        n_cu = ctx.device_info()[0]
+        buf = ctx.empty(n_blocks * 256, np.float64)
+        out = {}
+        for kind, name in enumerate(("k_valu_fma32", "k_valu_fma64", "k_valu_sqrt32")):
+            for _ in range(2):
+                ctx.valu_calibration(kind, buf, n_blocks, iters)
+            ctx.sync()
+            ctx.profile(True)
+            ctx.profile_reset()
+            for _ in range(steps):
+                ctx.valu_calibration(kind, buf, n_blocks, iters)
+            ctx.sync()
+            t, n = ctx.profile_query(name)
+            ctx.profile(False)
+            ms = t / n
+            wave_ops = n_blocks * 4 * iters * 8           # wave64 instructions of the chains
+            per_simd = wave_ops / (4 * n_cu)
+            rec = {"avg_launch_ms": ms, "wave64_ops": wave_ops,
+                   "ns_per_op_per_simd": ms * 1e6 / per_simd,
+                   "cycles_per_op_at_2p4ghz": ms * 1e-3 * 2.4e9 / per_simd}
+            if kind < 2:
+                rec["tflops"] = wave_ops * 64 * 2 / (ms * 1e-3) / 1e12
+            out[name] = rec
+        buf.free()
+        print(json.dumps({"valu_calibration": out, "n_cu": n_cu, "steps": steps}), flush=True)
+
+
 METRIC = "curve-intervals/sec and samples/sec at \u22641e-6 rel error: MLP-seeded vs plain, 1 GPU"
 
 
@@ -299,6 +333,8 @@ def main():
     ap.add_argument("--seed-weights", default=None, help="learned-seed weights (.npz); default: the packaged ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time the library's streaming copy kernel over MIB mebibytes")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only time the library's VALU rate kernels (FP32 FMA, FP64 FMA, FP32 sqrt)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (curves are independent and would "
@@ -309,6 +345,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(G, args.copy_calibration, args.steps)
+        return
+    if args.valu_calibration:
+        valu_calibration(G, args.steps)
         return
 
     t_gen = time.time()

@@ -84,6 +84,7 @@ _SIGS = [
     ("gcv_copy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("gcv_copy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("gcv_copy_d2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("gcv_valu_calibration", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_int]),
     ("gcv_sync", C.c_int, [C.c_void_p]),
     ("gcv_device_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
     ("gcv_timer_start", C.c_int, [C.c_void_p]),
@@ -219,6 +220,14 @@ class Context:
 
     def copy_d2d(self, dst: DeviceArray, src: DeviceArray, nbytes: int):
         self._check(lib().gcv_copy_d2d(self.h, dst.ptr, src.ptr, nbytes))
+
+    def valu_calibration(self, kind: int, buf: DeviceArray, n_blocks: int, iters: int):
+        """Enqueue the VALU rate kernel of `kind` (0 FP32 FMA, 1 FP64 FMA,
+        2 FP32 sqrt) over n_blocks x 256 lanes; buf holds n_blocks * 256
+        doubles."""
+        if buf.nbytes < n_blocks * 256 * 8:
+            raise ValueError("valu_calibration: buffer smaller than n_blocks * 256 doubles")
+        self._check(lib().gcv_valu_calibration(self.h, kind, buf.ptr, n_blocks, iters))
 
     def device_info(self):
         n = C.c_int()

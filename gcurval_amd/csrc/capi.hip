@@ -376,6 +376,25 @@ int gcv_copy_d2d(gcv_ctx* c, void* dst, const void* src, size_t bytes) {
     return GCV_OK;
 }
 
+int gcv_valu_calibration(gcv_ctx* c, int kind, double* dbuf, int64_t n_blocks, int iters) {
+    use_device(c);
+    if (!c || !dbuf || n_blocks <= 0 || n_blocks > INT32_MAX || iters < 0 || kind < 0 || kind > 2)
+        return c ? set_err(c, GCV_ERR_INVALID, "gcv_valu_calibration: kind 0..2, n_blocks > 0, a buffer of "
+                           "n_blocks * 256 doubles") : GCV_ERR_INVALID;
+    static const char* names[3] = {"k_valu_fma32", "k_valu_fma64", "k_valu_sqrt32"};
+    prof_begin(c, names[kind]);
+    if (kind == 0)
+        hipLaunchKernelGGL(dev::k_valu_chain<0>, dim3((unsigned)n_blocks), dim3(256), 0, c->stream, dbuf, iters);
+    else if (kind == 1)
+        hipLaunchKernelGGL(dev::k_valu_chain<1>, dim3((unsigned)n_blocks), dim3(256), 0, c->stream, dbuf, iters);
+    else
+        hipLaunchKernelGGL(dev::k_valu_chain<2>, dim3((unsigned)n_blocks), dim3(256), 0, c->stream, dbuf, iters);
+    hipError_t e = hipGetLastError();
+    prof_end(c);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "calibration launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
 int gcv_sync(gcv_ctx* c) {
     use_device(c);
     if (!c) return GCV_ERR_INVALID;

@@ -343,6 +343,39 @@ static __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// VALU issue-rate calibration: every lane runs 8 independent chains of one
+// operation (kind 0: FP32 FMA, 1: FP64 FMA, 2: FP32 square root) for `iters`
+// steps, so a full grid measures that operation's sustained rate.
+template <int Kind>
+static __global__ __launch_bounds__(256) void k_valu_chain(double* out, int iters) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (Kind == 1) {
+        double a[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = (double)(threadIdx.x + j) * 1e-3;
+        for (int i = 0; i < iters; i++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[j] = fma(a[j], 0.999999, 1e-7);
+        }
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) t += a[j];
+        out[g] = t;
+    } else {
+        float a[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = (float)(threadIdx.x + j) * 1e-3f + (Kind == 2 ? 1.0f : 0.0f);
+        for (int i = 0; i < iters; i++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[j] = Kind == 0 ? fmaf(a[j], 0.999999f, 1e-7f) : __builtin_amdgcn_sqrtf(a[j]);
+        }
+        float t = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) t += a[j];
+        out[g] = t;
+    }
+}
+
 static __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;

@@ -72,6 +72,13 @@ int gcv_copy_d2h(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Device-to-device copy by the library's streaming kernel (16-byte aligned
  * pointers and size); enqueued, no sync. Used to measure the HBM ceiling. */
 int gcv_copy_d2d(gcv_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* VALU rate calibration (no counterpart in the reference; a measurement
+ * utility beside gcv_copy_d2d): n_blocks x 256 lanes each run 8 independent
+ * chains of `iters` operations of one kind (0: FP32 FMA, 1: FP64 FMA,
+ * 2: FP32 square root) and write one double to dbuf[n_blocks * 256].
+ * Enqueued, no sync; timed by the profiling API under the names
+ * k_valu_fma32 / k_valu_fma64 / k_valu_sqrt32. */
+int gcv_valu_calibration(gcv_ctx* ctx, int kind, double* dbuf, int64_t n_blocks, int iters);
 int gcv_sync(gcv_ctx* ctx);
 int gcv_device_info(gcv_ctx* ctx, int* n_cu, char* name, int name_len);
 /* The context's HIP stream (a hipStream_t), for kernels a caller compiles
