@@ -111,6 +111,27 @@ hipError_t Launch<Dt, NGL>::subdivide_bspline(const LaunchCfg& c, int p, int D, 
 #undef GCV_SUBDIV_CASE
 }
 
+template <typename Dt, int NGL, bool kSeeded>
+static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const EllipseEval<double>& ev64,
+                            const SampleArgs& A, const int32_t* blk_curve) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, kSeeded>, kSampleBlock, 0) !=
+                hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    const int64_t n_tiles = (A.n_total + 63) / 64;
+    const int64_t wpb = kSampleBlock / 64;
+    int64_t n_blocks = (int64_t)c.n_cu * per_cu;
+    if (n_blocks * wpb > n_tiles) n_blocks = (n_tiles + wpb - 1) / wpb;
+    const int64_t tpw = (n_tiles + n_blocks * wpb - 1) / (n_blocks * wpb);
+    hipLaunchKernelGGL((k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, kSeeded>),
+                       dim3((unsigned)n_blocks), dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve, tpw);
+}
+
 template <typename Dt, int NGL>
 hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
                                            const SampleArgs& A, const int32_t* blk_curve) {
@@ -129,10 +150,13 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
         GCV_ELLIPSE_SAMPLE(true, true);
     } else if (pos) {
         GCV_ELLIPSE_SAMPLE(true, false);
-    } else if (seeded) {
-        GCV_ELLIPSE_SAMPLE(false, true);
     } else {
-        GCV_ELLIPSE_SAMPLE(false, false);
+        // uniform positions: persistent waves over contiguous runs of 64-sample
+        // tiles, as many blocks as are resident at once (queried once)
+        if (seeded)
+            run_sample_runs<Dt, NGL, true>(c, ev, ev64, A, blk_curve);
+        else
+            run_sample_runs<Dt, NGL, false>(c, ev, ev64, A, blk_curve);
     }
 #undef GCV_ELLIPSE_SAMPLE
     hipError_t e = hipGetLastError();

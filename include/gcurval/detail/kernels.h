@@ -880,6 +880,61 @@ __device__ __forceinline__ int64_t locate_interval(const SampleArgs& A, const Ta
     return i;
 }
 
+// Tail of one sample once its root is known (root k of the 2^ls-root block of
+// interval i, the prefix before it, its length, the root-local target sn):
+// re-walk a refined root to its leaf, Newton in Dt from the Hermite start,
+// then the FP64 step. c64 is the FP64 context for evaluators that load it
+// (read before any wave barrier); evaluators that widen their Dt context do
+// so here.
+template <typename Dt, int NGL, class Ev, class Ev64>
+__device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int i,
+                                                const typename Ev::Ctx& ctx, const typename Ev64::Ctx& c64,
+                                                uint64_t deep, int ls, int k, double sn, double pre, double rlen) {
+    const Dt eps0 = (Dt)A.eps0;
+    double tau = sn - pre;
+    tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
+    Dt a, b, xm;
+    task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
+    Dt len = (Dt)rlen;
+    if ((deep >> k) & 1ull) {
+        double tl;
+        const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
+        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
+        tau = tl;
+    }
+    Dt v_last;
+    const int pieces = 1;
+    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last);
+    double dd = (double)d;
+    if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
+        // one FP64 Newton step on the same leaf: the residual is evaluated in
+        // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
+        // only scales a residual that is already at FP32 rounding level)
+        typename Ev64::Ctx cw;
+        if constexpr (HasWiden<Ev64, typename Ev::Ctx>::value)
+            cw = Ev64::widen(ctx);
+        else
+            cw = c64;
+        const double ad = (double)a, wd = (double)b - (double)a;
+        const double F = quad_from_n<NGL>(ev64, cw, ad, dd, pieces) - tau;
+        if (v_last > Dt(0)) {
+            const double dn = dd - F * (double)nrcp(v_last);
+            dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
+        }
+    }
+    return ev.to_u(i, (double)a + dd);
+}
+
+// FP64 context read with the Dt one (before any wave barrier) for evaluators
+// that cannot widen theirs.
+template <class Ev64, class Ctx32>
+__device__ __forceinline__ typename Ev64::Ctx ctx64_early(const Ev64& ev64, int i, double inv_sc) {
+    if constexpr (HasWiden<Ev64, Ctx32>::value)
+        return typename Ev64::Ctx{};
+    else
+        return ev64.load(i, inv_sc);
+}
+
 template <typename Dt, int NGL, class M = RuntimeMode, class Ev, class Ev64>
 __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int c,
                                              int64_t g, int32_t* seg_out, int64_t hint_lo = 0,
@@ -909,47 +964,19 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     if constexpr (one) s = target_of<M>(A, tin, g);
     const double inv_sc = rcp_f64(sc);
     const double sn = (s - prev) * inv_sc;
-    // the context's loads go out with the root window's; the window is
+    // the contexts' loads go out with the root window's; the window is
     // written to LDS after them (a wave barrier before them would hold them
     // back, or force loads above to be repeated)
     const typename Ev::Ctx ctx = ev.load((int)i, inv_sc);
+    const typename Ev64::Ctx c64 = ctx64_early<Ev64, typename Ev::Ctx>(ev64, (int)i, inv_sc);
     if (staged) store_roots(win, rv, span);
-    const Dt eps0 = (Dt)A.eps0;
     int k;
     double pre, rlen;
     if (staged)
         find_root((const lds_double*)(win + off), ls, sn, k, pre, rlen);
     else
         find_root(A.root_prefix + ro, ls, sn, k, pre, rlen);
-    double tau = sn - pre;
-    tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
-
-    Dt a, b, xm;
-    task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
-    Dt len = (Dt)rlen;
-    if ((deep >> k) & 1ull) {
-        double tl;
-        const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
-        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
-        tau = tl;
-    }
-    Dt v_last;
-    const int pieces = 1;
-    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last);
-    double dd = (double)d;
-    if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
-        // one FP64 Newton step on the same leaf: the residual is evaluated in
-        // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
-        // only scales a residual that is already at FP32 rounding level)
-        const typename Ev64::Ctx c64 = widen_or_load(ev64, ctx, (int)i, inv_sc);
-        const double ad = (double)a, wd = (double)b - (double)a;
-        const double F = quad_from_n<NGL>(ev64, c64, ad, dd, pieces) - tau;
-        if (v_last > Dt(0)) {
-            const double dn = dd - F * (double)nrcp(v_last);
-            dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
-        }
-    }
-    return ev.to_u((int)i, (double)a + dd);
+    return finish_sample<Dt, NGL>(ev, ev64, A, (int)i, ctx, c64, deep, ls, k, sn, pre, rlen);
 }
 
 // Curve of the first sample of every block of kSampleBlock samples: thread per
@@ -1022,6 +1049,112 @@ __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, 
     }
     A.out_u[g] = u;
     if (A.out_seg) A.out_seg[g] = seg;
+}
+
+// Wave-uniform copy of a plain value (each 32-bit word from the first active
+// lane), so the compiler keeps it in scalar registers.
+template <class T>
+__device__ __forceinline__ T uniform_pod(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(T) / 4); q++) w[q] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[q]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
+// Persistent sampling of one-interval curves (ellipses) at uniform positions.
+// Every wave takes a contiguous run of 64-sample tiles. Tiles of a run mostly
+// lie on one curve, so the wave reads a curve's data (sample offsets, length,
+// evaluator contexts, refined-root mask, and the root prefixes into its LDS
+// slice) once and keeps it for every tile inside that curve. A tile that
+// reaches into later curves is finished curve by curve: the lanes on the
+// current curve, then the wave loads the next curve and takes its lanes. Same
+// arithmetic as sample_one, so the same results.
+template <typename Dt, int NGL, class Ev, class Ev64, bool kSeeded>
+__global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArgs A, const int32_t* blk_curve,
+                                                     int64_t tiles_per_wave) {
+    static_assert(OneIntervalPerCurve<Ev>::value, "run sampling needs one interval per curve");
+    __shared__ double s_run[kSampleBlock / 64][64];
+    const int lane = lane_id();
+    // the wave index is uniform, and so is the whole tile loop below
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    lds_double* rwin = (lds_double*)s_run[wib];
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+    const int64_t n_tiles = (A.n_total + 63) >> 6;
+    const int64_t t0 = gw * tiles_per_wave;
+    const int64_t t1 = t0 + tiles_per_wave < n_tiles ? t0 + tiles_per_wave : n_tiles;
+    const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
+
+    // run state (wave-uniform): curve cr holds samples [s0, s1)
+    struct State {
+        typename Ev::Ctx ctx;
+        typename Ev64::Ctx c64;
+        double rN, cl, inv_sc;
+        uint64_t deep;
+        int64_t cr, s0, s1;
+        int32_t ls, den_pos;
+    };
+    State S{};
+    for (int64_t t = t0; t < t1; t++) {
+        const int64_t gf = t << 6;
+        const int64_t gl = gf + 63 < A.n_total - 1 ? gf + 63 : A.n_total - 1;
+        const int64_t g = gf + lane;
+        int64_t next = gf;  // first sample of the tile not yet taken (wave-uniform)
+        for (;;) {
+            if (next < S.s0 || next >= S.s1) {
+                // load the curve holding sample `next`
+                const int64_t blk = next / kSampleBlock;
+                const int64_t ca = readfirstlane64(blk_curve[blk]);
+                const int64_t cb = readfirstlane64(blk + 1 < n_blk ? (int64_t)blk_curve[blk + 1] : A.n_curve - 1);
+                const int64_t cn = ca == cb ? ca : readfirstlane64(upper_bound_i64(A.soff, ca, cb + 1, next) - 1);
+                State N;
+                N.cr = cn;
+                N.s0 = A.soff[cn];
+                N.s1 = A.soff[cn + 1];
+                const double clv = A.curve_len[cn];
+                const double sc = ev.scale((int)cn);
+                N.deep = A.deep[cn];
+                int64_t ro = cn << A.log2_nsub;
+                N.ls = A.log2_nsub;
+                if constexpr (kSeeded) {
+                    ro = A.root_off[cn];
+                    N.ls = __ffs((int)(A.root_off[cn + 1] - ro)) - 1;
+                }
+                const double v = lane < (1 << N.ls) ? A.root_prefix[ro + lane] : 0.0;
+                const int64_t Ns = N.s1 - N.s0;
+                const double den = (double)(A.uniform_closed ? Ns - 1 : Ns);
+                N.den_pos = den > 0.0;
+                N.rN = rcp_f64(den > 0.0 ? den : 1.0);
+                N.cl = clv;
+                N.inv_sc = rcp_f64(sc);
+                N.ctx = ev.load((int)cn, N.inv_sc);
+                N.c64 = ctx64_early<Ev64, typename Ev::Ctx>(ev64, (int)cn, N.inv_sc);
+                S = uniform_pod(N);
+                rwin[lane] = v;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            }
+            if (g >= next && g < S.s1 && g <= gl) {
+                // target as target_of: j / (N - 1) or j / N, relative to the length
+                const int64_t j = g - S.s0;
+                const double pos = S.den_pos ? (double)j * S.rN : 0.0;
+                const double sv = A.is_relative ? pos * S.cl : pos;
+                const double sn = sv * S.inv_sc;
+                int k;
+                double pre, rlen;
+                find_root((const lds_double*)rwin, S.ls, sn, k, pre, rlen);
+                A.out_u[g] = finish_sample<Dt, NGL>(ev, ev64, A, (int)S.cr, S.ctx, S.c64, S.deep, S.ls, k, sn, pre,
+                                                    rlen);
+                if (A.out_seg) A.out_seg[g] = 0;
+            }
+            if (gl < S.s1) break;
+            next = S.s1;
+        }
+    }
 }
 
 }  // namespace dev
