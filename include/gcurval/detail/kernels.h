@@ -645,18 +645,38 @@ __device__ __forceinline__ Dt quad_from_n(const Ev& ev, const typename Ev::Ctx& 
 // width), so halving keeps the partial integrals within tol / 64 as for the
 // plain method's 64-way pre-split.
 constexpr int kExtraNewton = 16;
+// Speeds at a leaf's ends and their reciprocals (as invert_leaf computes them).
+template <typename Dt>
+struct LeafEnds {
+    Dt va, vb, rva, rvb;
+};
+template <class Ev, typename Dt>
+__device__ __forceinline__ LeafEnds<Dt> leaf_ends(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b) {
+    LeafEnds<Dt> e;
+    e.va = ev.speed(c, a);
+    e.vb = ev.speed(c, b);
+    e.rva = nrcp(e.va);
+    e.rvb = nrcp(e.vb);
+    return e;
+}
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
-                                          int iters, int pieces, Dt* v_out) {
+                                          int iters, int pieces, Dt* v_out, bool ends_known = false,
+                                          const LeafEnds<Dt>& ends_in = LeafEnds<Dt>{}) {
     const Dt w = b - a;
     Dt t = len > Dt(0) ? tau * nrcp(len) : Dt(0);
     t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
-    const Dt va = ev.speed(c, a), vb = ev.speed(c, b);
+    LeafEnds<Dt> E;
+    if (ends_known)
+        E = ends_in;
+    else
+        E = leaf_ends(ev, c, a, b);
+    const Dt va = E.va, vb = E.vb;
     Dt d;
     if (va > Dt(0) && vb > Dt(0) && len > Dt(0)) {
         const Dt t2 = t * t, t3 = t2 * t;
         const Dt h10 = t3 - Dt(2) * t2 + t, h01 = Dt(3) * t2 - Dt(2) * t3, h11 = t3 - t2;
-        d = w * h01 + len * (h10 * nrcp(va) + h11 * nrcp(vb));
+        d = w * h01 + len * (h10 * E.rva + h11 * E.rvb);
     } else {
         d = w * t;
     }
@@ -889,22 +909,27 @@ __device__ __forceinline__ int64_t locate_interval(const SampleArgs& A, const Ta
 template <typename Dt, int NGL, class Ev, class Ev64>
 __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int i,
                                                 const typename Ev::Ctx& ctx, const typename Ev64::Ctx& c64,
-                                                uint64_t deep, int ls, int k, double sn, double pre, double rlen) {
+                                                uint64_t deep, int ls, int k, double sn, double pre, double rlen,
+                                                bool root_ends_known = false,
+                                                const LeafEnds<Dt>& root_ends = LeafEnds<Dt>{}) {
     const Dt eps0 = (Dt)A.eps0;
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
     Dt a, b, xm;
     task_bounds<Dt>(k, 0, 0u, ls, a, xm, b);
     Dt len = (Dt)rlen;
+    bool ends_known = root_ends_known;
     if ((deep >> k) & 1ull) {
         double tl;
         const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
         walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
         tau = tl;
+        ends_known = false;  // the leaf is below the root
     }
     Dt v_last;
     const int pieces = 1;
-    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last);
+    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last, ends_known,
+                                  root_ends);
     double dd = (double)d;
     if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
         // one FP64 Newton step on the same leaf: the residual is evaluated in
@@ -1078,10 +1103,13 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                                                      int64_t tiles_per_wave) {
     static_assert(OneIntervalPerCurve<Ev>::value, "run sampling needs one interval per curve");
     __shared__ double s_run[kSampleBlock / 64][64];
+    __shared__ Dt s_ends[kSampleBlock / 64][64 * 4];  // per root: va, vb, 1/va, 1/vb
     const int lane = lane_id();
     // the wave index is uniform, and so is the whole tile loop below
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     lds_double* rwin = (lds_double*)s_run[wib];
+    typedef __attribute__((address_space(3))) Dt lds_dt;
+    lds_dt* ends = (lds_dt*)s_ends[wib];
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
     const int64_t n_tiles = (A.n_total + 63) >> 6;
     const int64_t t0 = gw * tiles_per_wave;
@@ -1134,6 +1162,16 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 N.c64 = ctx64_early<Ev64, typename Ev::Ctx>(ev64, (int)cn, N.inv_sc);
                 S = uniform_pod(N);
                 rwin[lane] = v;
+                if (lane < (1 << S.ls)) {
+                    // the Hermite start's end speeds of every root of the curve
+                    Dt ra, rm, rb;
+                    task_bounds<Dt>(lane, 0, 0u, S.ls, ra, rm, rb);
+                    const LeafEnds<Dt> e = leaf_ends(ev, S.ctx, ra, rb);
+                    ends[4 * lane] = e.va;
+                    ends[4 * lane + 1] = e.vb;
+                    ends[4 * lane + 2] = e.rva;
+                    ends[4 * lane + 3] = e.rvb;
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
@@ -1147,8 +1185,9 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 int k;
                 double pre, rlen;
                 find_root((const lds_double*)rwin, S.ls, sn, k, pre, rlen);
+                const LeafEnds<Dt> E{ends[4 * k], ends[4 * k + 1], ends[4 * k + 2], ends[4 * k + 3]};
                 A.out_u[g] = finish_sample<Dt, NGL>(ev, ev64, A, (int)S.cr, S.ctx, S.c64, S.deep, S.ls, k, sn, pre,
-                                                    rlen);
+                                                    rlen, true, E);
                 if (A.out_seg) A.out_seg[g] = 0;
             }
             if (gl < S.s1) break;
