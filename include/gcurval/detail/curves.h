@@ -248,6 +248,18 @@ struct EllipseEval {
     __device__ __forceinline__ double speed(const Ctx& c, double x) const {
         return sqrt_pos(fma(double(c.D), sin2_2pi(x, double(c.shift)), double(c.base)));
     }
+    // d speed / dx = D pi sin(2 pi y) / speed with y = 2x + shift (speed v > 0)
+    __device__ __forceinline__ float speed_slope(const Ctx& c, float x, float v) const {
+        return float(c.D) * 3.14159265358979f * sinpif(2.0f * fmaf(2.0f, x, float(c.shift))) / v;
+    }
+    __device__ __forceinline__ double speed_slope(const Ctx& c, double x, double v) const {
+        return double(c.D) * 3.141592653589793 * sinpi(2.0 * fma(2.0, x, double(c.shift))) / v;
+    }
+    // (min(a', b') / max(a', b'))^2
+    __device__ __forceinline__ double axis_ratio2(const Ctx& c) const {
+        const double lo = double(c.base), hi = double(c.base) + double(c.D);
+        return hi > 0.0 ? lo / hi : 1.0;
+    }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }
 };

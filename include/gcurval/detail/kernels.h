@@ -645,10 +645,12 @@ __device__ __forceinline__ Dt quad_from_n(const Ev& ev, const typename Ev::Ctx& 
 // width), so halving keeps the partial integrals within tol / 64 as for the
 // plain method's 64-way pre-split.
 constexpr int kExtraNewton = 16;
-// Speeds at a leaf's ends and their reciprocals (as invert_leaf computes them).
+// Speeds at a leaf's ends and their reciprocals (as invert_leaf computes them),
+// and for the quintic start the second derivatives of the inverse map x(s) at
+// the ends, x'' = -v'(x) / v^3.
 template <typename Dt>
 struct LeafEnds {
-    Dt va, vb, rva, rvb;
+    Dt va, vb, rva, rvb, xa2, xb2;
 };
 template <class Ev, typename Dt>
 __device__ __forceinline__ LeafEnds<Dt> leaf_ends(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b) {
@@ -657,12 +659,27 @@ __device__ __forceinline__ LeafEnds<Dt> leaf_ends(const Ev& ev, const typename E
     e.vb = ev.speed(c, b);
     e.rva = nrcp(e.va);
     e.rvb = nrcp(e.vb);
+    e.xa2 = Dt(0);
+    e.xb2 = Dt(0);
     return e;
 }
+// The same with the second derivatives (evaluators with speed_slope).
+template <class Ev, typename Dt>
+__device__ __forceinline__ LeafEnds<Dt> leaf_ends_quintic(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b) {
+    LeafEnds<Dt> e = leaf_ends(ev, c, a, b);
+    if (e.va > Dt(0) && e.vb > Dt(0)) {
+        e.xa2 = -ev.speed_slope(c, a, e.va) * e.rva * e.rva * e.rva;
+        e.xb2 = -ev.speed_slope(c, b, e.vb) * e.rvb * e.rvb * e.rvb;
+    }
+    return e;
+}
+// With `quintic` (ends known, second derivatives included) the start is the
+// quintic Hermite interpolant of x(s) and no Newton step follows here: the
+// caller takes the FP64 step from it (*v_out = the speed at the start).
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
                                           int iters, int pieces, Dt* v_out, bool ends_known = false,
-                                          const LeafEnds<Dt>& ends_in = LeafEnds<Dt>{}) {
+                                          const LeafEnds<Dt>& ends_in = LeafEnds<Dt>{}, bool quintic = false) {
     const Dt w = b - a;
     Dt t = len > Dt(0) ? tau * nrcp(len) : Dt(0);
     t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
@@ -672,6 +689,18 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
     else
         E = leaf_ends(ev, c, a, b);
     const Dt va = E.va, vb = E.vb;
+    if (ends_known && quintic && va > Dt(0) && vb > Dt(0) && len > Dt(0)) {
+        const Dt t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+        const Dt H5 = Dt(10) * t3 - Dt(15) * t4 + Dt(6) * t5;     // value at the right end
+        const Dt H1 = t - Dt(6) * t3 + Dt(8) * t4 - Dt(3) * t5;    // slope at the left end
+        const Dt H4 = -Dt(4) * t3 + Dt(7) * t4 - Dt(3) * t5;       // slope at the right end
+        const Dt H2 = Dt(0.5) * t2 - Dt(1.5) * t3 + Dt(1.5) * t4 - Dt(0.5) * t5;  // curvature, left
+        const Dt H3 = Dt(0.5) * t3 - t4 + Dt(0.5) * t5;                           // curvature, right
+        Dt d = w * H5 + len * (H1 * E.rva + H4 * E.rvb) + len * len * (H2 * E.xa2 + H3 * E.xb2);
+        d = d > Dt(0) ? (d < w ? d : w) : Dt(0);
+        *v_out = ev.speed(c, a + d);
+        return d;
+    }
     Dt d;
     if (va > Dt(0) && vb > Dt(0) && len > Dt(0)) {
         const Dt t2 = t * t, t3 = t2 * t;
@@ -911,7 +940,8 @@ __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, 
                                                 const typename Ev::Ctx& ctx, const typename Ev64::Ctx& c64,
                                                 uint64_t deep, int ls, int k, double sn, double pre, double rlen,
                                                 bool root_ends_known = false,
-                                                const LeafEnds<Dt>& root_ends = LeafEnds<Dt>{}) {
+                                                const LeafEnds<Dt>& root_ends = LeafEnds<Dt>{},
+                                                bool quintic = false) {
     const Dt eps0 = (Dt)A.eps0;
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
@@ -928,8 +958,10 @@ __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, 
     }
     Dt v_last;
     const int pieces = 1;
+    // the quintic start replaces the Dt Newton steps only where the FP64 step follows
+    const bool q = quintic && ends_known && sizeof(Dt) < sizeof(double) && A.fp64_step;
     const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last, ends_known,
-                                  root_ends);
+                                  root_ends, q);
     double dd = (double)d;
     if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
         // one FP64 Newton step on the same leaf: the residual is evaluated in
@@ -1103,7 +1135,7 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                                                      int64_t tiles_per_wave) {
     static_assert(OneIntervalPerCurve<Ev>::value, "run sampling needs one interval per curve");
     __shared__ double s_run[kSampleBlock / 64][64];
-    __shared__ Dt s_ends[kSampleBlock / 64][64 * 4];  // per root: va, vb, 1/va, 1/vb
+    __shared__ Dt s_ends[kSampleBlock / 64][64 * 6];  // per root: va, vb, 1/va, 1/vb, x''(a), x''(b)
     const int lane = lane_id();
     // the wave index is uniform, and so is the whole tile loop below
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1123,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
         double rN, cl, inv_sc;
         uint64_t deep;
         int64_t cr, s0, s1;
-        int32_t ls, den_pos;
+        int32_t ls, den_pos, quintic;
     };
     State S{};
     for (int64_t t = t0; t < t1; t++) {
@@ -1160,17 +1192,22 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 N.inv_sc = rcp_f64(sc);
                 N.ctx = ev.load((int)cn, N.inv_sc);
                 N.c64 = ctx64_early<Ev64, typename Ev::Ctx>(ev64, (int)cn, N.inv_sc);
+                // axis ratio >= 0.2: the quintic start is within 3.2e-5 of a root's width, so one FP64
+                // Newton step lands within 6e-11 of it (FP32 builds with the FP64 step only)
+                N.quintic = sizeof(Dt) < sizeof(double) && A.fp64_step && ev.axis_ratio2(N.ctx) >= 0.04;
                 S = uniform_pod(N);
                 rwin[lane] = v;
                 if (lane < (1 << S.ls)) {
                     // the Hermite start's end speeds of every root of the curve
                     Dt ra, rm, rb;
                     task_bounds<Dt>(lane, 0, 0u, S.ls, ra, rm, rb);
-                    const LeafEnds<Dt> e = leaf_ends(ev, S.ctx, ra, rb);
-                    ends[4 * lane] = e.va;
-                    ends[4 * lane + 1] = e.vb;
-                    ends[4 * lane + 2] = e.rva;
-                    ends[4 * lane + 3] = e.rvb;
+                    const LeafEnds<Dt> e = S.quintic ? leaf_ends_quintic(ev, S.ctx, ra, rb) : leaf_ends(ev, S.ctx, ra, rb);
+                    ends[6 * lane] = e.va;
+                    ends[6 * lane + 1] = e.vb;
+                    ends[6 * lane + 2] = e.rva;
+                    ends[6 * lane + 3] = e.rvb;
+                    ends[6 * lane + 4] = e.xa2;
+                    ends[6 * lane + 5] = e.xb2;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
                 __builtin_amdgcn_wave_barrier();
@@ -1185,9 +1222,10 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 int k;
                 double pre, rlen;
                 find_root((const lds_double*)rwin, S.ls, sn, k, pre, rlen);
-                const LeafEnds<Dt> E{ends[4 * k], ends[4 * k + 1], ends[4 * k + 2], ends[4 * k + 3]};
+                const LeafEnds<Dt> E{ends[6 * k],     ends[6 * k + 1], ends[6 * k + 2],
+                                     ends[6 * k + 3], ends[6 * k + 4], ends[6 * k + 5]};
                 A.out_u[g] = finish_sample<Dt, NGL>(ev, ev64, A, (int)S.cr, S.ctx, S.c64, S.deep, S.ls, k, sn, pre,
-                                                    rlen, true, E);
+                                                    rlen, true, E, S.quintic != 0);
                 if (A.out_seg) A.out_seg[g] = 0;
             }
             if (gl < S.s1) break;
