@@ -123,6 +123,30 @@ def test_tall_ellipses_b_above_a(ctx, prec):
         assert eng.stats()["overflow"] == 0
 
 
+@pytest.mark.parametrize("closed", [True, False])
+def test_uniform_runs_agree_with_explicit_positions(ctx, closed):
+    """Uniform positions take the persistent run sampler (per-curve data in
+    LDS; on curves with axis ratio >= 0.2 a quintic start followed directly
+    by the FP64 step); the same positions passed explicitly take the
+    per-sample path (cubic start, FP32 Newton, FP64 step). Both solve the
+    same FP64 residual on the same table, so they agree far below the
+    table's own error: 1e-9 rad (the quintic start's worst case after the
+    FP64 step is 6e-11 of a root's width, 6e-12 rad). Ragged counts (1 to
+    300) make tiles that span several curves."""
+    b = W.ellipse_batch(300, seed=61)
+    counts = W.ellipse_sample_counts(300, seed=62, lo=1, hi=300)
+    with build(ctx, b, normalize=True) as eng:
+        u_run = eng.arclengthReverse(counts, None, True, uniform_closed=closed)
+        pos = O.uniform_positions(counts, closed=closed)
+        u_pos = eng.arclengthReverse(counts, pos, True)
+        assert np.max(np.abs(u_run - u_pos)) <= 1e-9, float(np.max(np.abs(u_run - u_pos)))
+        # and both against the truth, as the other ellipse tests
+        ut, _ = O.Truth(b).reverse(counts, pos)
+        Lt, _ = O.Truth(b).lengths()
+        bound = np.repeat(np.maximum(1e-6, 2e-7 * Lt / np.minimum(b.a, b.b)), counts)
+        assert np.all(np.abs(u_run - ut) <= bound)
+
+
 def test_reference_sample1_dataset_lengths(ctx):
     b = W.reference_sample1_ellipses(1000)
     ref = O.RefGpuPlain(b, O.opts(0, 3))
