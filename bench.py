@@ -280,24 +280,34 @@ def comparison(runs, n_int, n_total):
             "max_rel_length_diff": float(np.max(np.abs(le["lengths"] - pl["lengths"]) / pl["lengths"]))}
 
 
-def roofline_of(run, batch, n_total, cfg_name, steps):
+def roofline_of(run, batch, counts, cfg_name, steps):
     """Dominant kernel (FP32 VALU bound): algorithmic FLOP per step from the
     counted quadratures (subdivision) or samples (sampler), divided by that
     kernel's event-timed duration per step."""
     kernels, st = run["kernels"], run["stats"]
+    n_total = int(np.sum(counts))
     if batch.kind == "ellipse":
         fnode = f_node_ellipse()
     else:
         ps = np.repeat(batch.degree, batch.intervals_per_curve())
         ds = np.repeat(batch.dim, batch.intervals_per_curve())
         fnode = float(np.mean([f_node_bspline(int(p), int(d)) for p, d in zip(ps[:100000], ds[:100000])]))
-    ngl, iters = 3, 2
+    ngl = 3
     sub_ms = kernels.get("k_subdivide", {"ms_per_step": 0.0})["ms_per_step"]
     smp_ms = kernels.get("k_sample", {"ms_per_step": 0.0})["ms_per_step"]
     quad_flop = st["quadratures"] * (ngl * fnode + 5)
-    # per sample: 2 speeds (Hermite start), one NGL quadrature and its end
-    # speed, (iters - 1) trapezoid speeds in FP32, one NGL quadrature in FP64
-    smp_flop = n_total * ((2 + ngl + iters + ngl) * fnode + 2 * 5)
+    # per sample, cubic-start path: 2 end speeds (Hermite start), one NGL
+    # quadrature and its end speed (the FP32 Newton step), one NGL quadrature
+    # in FP64; quintic path (ellipse curves with axis ratio >= 0.2 at uniform
+    # positions): the quintic start (30 FLOP: powers, basis, combination), the
+    # speed at the start and the FP64 quadrature
+    cubic = (2 + ngl + 1 + ngl) * fnode + 2 * 5
+    quintic = (1 + ngl) * fnode + 5 + 30
+    frac_q = 0.0
+    if batch.kind == "ellipse":
+        ratio = np.minimum(np.abs(batch.a), np.abs(batch.b)) / np.maximum(np.maximum(np.abs(batch.a), np.abs(batch.b)), 1e-300)
+        frac_q = float(np.sum(counts[ratio >= 0.2])) / max(1, n_total)
+    smp_flop = n_total * (frac_q * quintic + (1.0 - frac_q) * cubic)
     if smp_ms >= sub_ms:
         dom, flop, dur = "k_sample", smp_flop, smp_ms
     else:
@@ -311,6 +321,7 @@ def roofline_of(run, batch, n_total, cfg_name, steps):
             traffic = rec["read_bytes"] + rec["write_bytes"]
     return {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "flop_per_step": flop,
+            "sample_flop_cubic_path": cubic, "sample_flop_quintic_path": quintic, "quintic_sample_fraction": frac_q,
             "kernel_ms_per_step": dur,
             "hbm_gbs_of_traffic": (traffic / (dur * 1e-3) / 1e9) if (traffic and dur > 0) else None}
 
@@ -392,7 +403,7 @@ def main():
         "sample_only_samples_per_sec": n_total / (main_run["sample_ms"] * 1e-3),
         "kernels": main_run["kernels"],
         "build_stats": main_run["stats"],
-        "roofline": roofline_of(main_run, batch, n_total, cfg["workload"], args.steps),
+        "roofline": roofline_of(main_run, batch, counts, cfg["workload"], args.steps),
         "device": {"arch": arch, "n_cu": n_cu},
     }
     if not args.no_compare:
