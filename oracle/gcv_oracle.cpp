@@ -2,7 +2,8 @@
 // arc-length sampling, used ONLY as test infrastructure.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-// load this library, and only as the checker (or as the timed CPU baseline).
+// load this library, and only as the reference results are compared with (or
+// as the timed CPU baseline).
 // The product path (gcurval_amd/) never links, loads or calls it.
 //
 // Three engines live here:

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY. Loaded by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker (or as the timed CPU baseline);
+bench.py's cpu_baseline leg, as the reference results are compared with (or as the timed CPU baseline);
 never by the product package gcurval_amd.
 
 Engines (see gcv_oracle.cpp):
