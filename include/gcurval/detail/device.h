@@ -114,6 +114,16 @@ __device__ __forceinline__ T shfl_pod(const T& v, int src) {
     return out;
 }
 
+// Orders one wave's LDS stores before its later LDS loads of other lanes'
+// data. It emits no instruction (a wave's LDS operations complete in issue
+// order); the fences and the barrier keep the compiler from moving LDS
+// accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // Inclusive scan over aligned segments of 2^log2seg lanes, in double.
 __device__ __forceinline__ double seg_scan_incl(double v, int log2seg) {
     const int k = lane_id() & ((1 << log2seg) - 1);

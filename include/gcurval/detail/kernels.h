@@ -78,13 +78,51 @@ __device__ __forceinline__ int key_root_of(uint64_t k) { return key_root(k); }
 __device__ __forceinline__ int key_level_of(uint64_t k) { return key_level(k); }
 __device__ __forceinline__ uint32_t key_idx_of(uint64_t k) { return key_idx(k); }
 
+constexpr int kScanSlots = 96;  // LDS slots of one wave's segmented scan
+
 // Bytes of dynamic LDS per wave for a stack of `cap` entries: per-root FP64
-// accumulators, per-root task counters, then the stack (keys, parent lengths).
+// accumulators, the scan slots, per-root task counters, then the stack (keys,
+// parent lengths).
 template <typename Dt>
 __host__ __device__ constexpr int subdiv_lds_per_wave(int cap) {
-    return 64 * (int)sizeof(double) + 64 * (int)sizeof(int) +
+    return 64 * (int)sizeof(double) + kScanSlots * (int)sizeof(double) + 64 * (int)sizeof(int) +
            cap * (int)(sizeof(typename KeyOf<Dt>::type) + sizeof(Dt));
 }
+
+// Inclusive FP64 scan over aligned segments of S = 2^log2seg lanes through
+// the wave's kScanSlots LDS slots. Segment s owns slots [s 3S/2, (s+1) 3S/2):
+// S/2 zeros, then one slot per lane. A doubling step (d = 1, 2, .., S/2)
+// stores every lane's value and adds the slot d below its own, which is one
+// of the zeros for the segment's first d lanes, so a step is a store, a load
+// and one add, with no lane mask or clamped index. The sums are those of
+// seg_scan_incl, in the same order.
+struct LdsSegScan {
+    __attribute__((address_space(3))) double* slot;
+    int steps;
+    __device__ __forceinline__ LdsSegScan(void* slots, int log2seg) {
+        typedef __attribute__((address_space(3))) double lds_f64;
+        lds_f64* base = (lds_f64*)slots;
+        const int lane = lane_id();
+        const int S = 1 << log2seg, P = S >> 1;
+        const int seg = lane >> log2seg, k = lane & (S - 1);
+        slot = base + seg * (S + P) + P + k;
+        if (k < P) base[seg * (S + P) + k] = 0.0;
+        steps = log2seg;
+        wave_lds_sync();
+    }
+    __device__ __forceinline__ double operator()(double v) const {
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            if (j < steps) {
+                *slot = v;
+                wave_lds_sync();
+                v += slot[-(1 << j)];
+                wave_lds_sync();
+            }
+        }
+        return v;
+    }
+};
 
 // Depth cap of a root when an interval starts from 2^ls roots instead of the
 // plain 2^ls_plain: the finest width (interval / 2^(ls_plain + max_depth)) is
@@ -131,9 +169,11 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     unsigned char* wbase = smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap);
     double* s_acc = (double*)wbase;
-    int* s_cnt = (int*)(wbase + 64 * sizeof(double));
-    Key* tkey = (Key*)(wbase + 64 * (sizeof(double) + sizeof(int)));
-    Dt* tpar = (Dt*)(wbase + 64 * (sizeof(double) + sizeof(int)) + A.stack_cap * sizeof(Key));
+    unsigned char* const s_scan = wbase + 64 * sizeof(double);
+    unsigned char* const s_rest = s_scan + kScanSlots * sizeof(double);
+    int* s_cnt = (int*)s_rest;
+    Key* tkey = (Key*)(s_rest + 64 * sizeof(int));
+    Dt* tpar = (Dt*)(s_rest + 64 * sizeof(int) + A.stack_cap * sizeof(Key));
     const int32_t* list = A.int_list;
     int64_t n_slots = A.n_slots;
     if (A.slot_range) {
@@ -146,6 +186,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     const int jslot = lane >> ls;
     const int k = lane & (nsub - 1);
     const int64_t n_tiles = (n_slots + (1 << ipw_log2) - 1) >> ipw_log2;
+    const LdsSegScan scan(s_scan, ls);
 
     unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0;
     int max_stack = 0, overflow = 0;
@@ -280,9 +321,9 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         }
 
         // in-interval inclusive prefix over the n_sub roots (FP64)
-        const double pre = seg_scan_incl(valid ? (double)len : 0.0, ls);
+        const double pre = scan(valid ? (double)len : 0.0);
         double cost = 0.0;
-        if (A.int_cost) cost = seg_scan_incl(valid ? 3.0 + (fail ? 2.0 * s_cnt[lane] : 0.0) : 0.0, ls);
+        if (A.int_cost) cost = scan(valid ? 3.0 + (fail ? 2.0 * s_cnt[lane] : 0.0) : 0.0);
         if (valid) {
             A.root_prefix[rbase + k] = pre;
             if (k == nsub - 1) {

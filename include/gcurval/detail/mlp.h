@@ -46,14 +46,6 @@ __host__ __device__ constexpr int off_b2() { return off_W2() + kHid * kHid; }
 __host__ __device__ constexpr int off_W3() { return off_b2() + kHid; }
 __host__ __device__ constexpr int off_b3() { return off_W3() + kHid * kOut; }
 
-// Orders one wave's LDS writes before its other lanes read them (the wave's
-// LDS operations issue in order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
