@@ -211,43 +211,24 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         if (A.root_off) r = A.root_off[iv];
         return r;
     };
-    bool v_cur = false, v_nxt = false;
-    int i_cur = 0, i_nxt = 0;
-    typename Ev::Ctx ctx_cur{};
-    int64_t rb_cur = 0;
-    double sc_cur = 1.0;
-    if (gwave < n_tiles) {
-        i_cur = interval_of(gwave, v_cur);
-        ctx_cur = ev.load(i_cur);
-        rb_cur = root_base(i_cur);
-        sc_cur = ev.scale(i_cur);
-    }
-    if (gwave + nwaves < n_tiles) i_nxt = interval_of(gwave + nwaves, v_nxt);
+    // A stage: what tile t needs from memory (interval, context, root base, scale).
+    struct Stage {
+        typename Ev::Ctx ctx;
+        int64_t rb;
+        double sc;
+        int i;
+        bool v;
+    };
+    // unconditional loads: past the last tile iv is still an interval of the
+    // list (its first slot), so they stay in range and are simply unused
+    auto fetch = [&](int iv, bool ok) -> Stage { return Stage{ev.load(iv), root_base(iv), ev.scale(iv), iv, ok}; };
 
-    for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
-        const bool valid = v_cur;
-        const int i = i_cur;
-        const typename Ev::Ctx ctx = ctx_cur;
-        const int64_t rbase = rb_cur;
-        const double scale_i = sc_cur;
-        typename Ev::Ctx ctx_nxt{};
-        int64_t rb_nxt = 0;
-        double sc_nxt = 1.0;
-        if (tile + nwaves < n_tiles) {
-            ctx_nxt = ev.load(i_nxt);
-            rb_nxt = root_base(i_nxt);
-            sc_nxt = ev.scale(i_nxt);
-        }
-        bool v_nn = false;
-        int i_nn = 0;
-        if (tile + 2 * nwaves < n_tiles) i_nn = interval_of(tile + 2 * nwaves, v_nn);
-        i_cur = i_nxt;
-        v_cur = v_nxt;
-        ctx_cur = ctx_nxt;
-        rb_cur = rb_nxt;
-        sc_cur = sc_nxt;
-        i_nxt = i_nn;
-        v_nxt = v_nn;
+    auto tile_body = [&](const Stage& cur) __attribute__((always_inline)) {
+        const bool valid = cur.v;
+        const int i = cur.i;
+        const typename Ev::Ctx& ctx = cur.ctx;
+        const int64_t rbase = cur.rb;
+        const double scale_i = cur.sc;
 
         Dt x0, xm, x1;
         task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
@@ -335,6 +316,24 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
                 A.deep[i] = bits;
             }
         }
+    };
+
+    // one copy of the tile body: results must not depend on which tiles a
+    // wave takes, and two inlined copies may contract their FMAs differently
+    Stage cur{};
+    bool v_nxt = false;
+    int i_nxt = 0;
+    if (gwave < n_tiles) {
+        bool ok;
+        const int iv = interval_of(gwave, ok);
+        cur = fetch(iv, ok);
+        i_nxt = interval_of(gwave + nwaves, v_nxt);
+    }
+    for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
+        const Stage nxt = fetch(i_nxt, v_nxt);
+        i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
+        tile_body(cur);
+        cur = nxt;
     }
     if (lane == 0) {
         unsigned long long* st = A.stats + (blockIdx.x & (kStatSlots - 1)) * kStatStride;
