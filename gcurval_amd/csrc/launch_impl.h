@@ -125,7 +125,13 @@ static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const
     }
     const int64_t n_tiles = (A.n_total + 63) / 64;
     const int64_t wpb = kSampleBlock / 64;
-    int64_t n_blocks = (int64_t)c.n_cu * per_cu;
+    // 16 rounds of resident blocks: curves differ in cost (refined roots, the
+    // cubic or quintic start), so one run per resident wave left SIMDs idle
+    // while the slowest runs finished; shorter runs are handed out by the
+    // dispatcher as blocks retire (1M ellipses, 970M samples: 9.14 -> 7.64 ms;
+    // 32 rounds the same, 64 and 128 slower as runs start more often)
+    constexpr int kRunRounds = 16;
+    int64_t n_blocks = (int64_t)c.n_cu * per_cu * kRunRounds;
     if (n_blocks * wpb > n_tiles) n_blocks = (n_tiles + wpb - 1) / wpb;
     const int64_t tpw = (n_tiles + n_blocks * wpb - 1) / (n_blocks * wpb);
     hipLaunchKernelGGL((k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, kSeeded>),
