@@ -369,6 +369,27 @@ class BSplineBatch:
         return np.diff(self.knot_off) - 2 * self.degree.astype(np.int64) - 1
 
 
+def sample_offsets(n_ptrs_for_curves, n_curve: int, sample_position=None):
+    """Sample offsets (the partial sum of the per-curve counts, n_curve + 1
+    entries, as gcv_sample takes them) and the positions as a flat float64
+    array (or None), after checking what the device cannot: one non-negative
+    count per curve, and one position per sample."""
+    n = np.ascontiguousarray(n_ptrs_for_curves, dtype=np.int64)
+    if n.ndim != 1 or n.size != n_curve:
+        raise ValueError(f"n_ptrs_for_curves: {n_curve} per-curve counts expected, got shape {n.shape}")
+    if np.any(n < 0):
+        raise ValueError("n_ptrs_for_curves: negative sample count")
+    off = np.zeros(n_curve + 1, dtype=np.int64)
+    np.cumsum(n, out=off[1:])
+    pos = None
+    if sample_position is not None:
+        pos = np.ascontiguousarray(sample_position, dtype=np.float64).reshape(-1)
+        if pos.size != int(off[-1]):
+            raise ValueError(f"sample_position: {int(off[-1])} positions expected (the sum of the counts), "
+                             f"got {pos.size}")
+    return off, pos
+
+
 class ArclengthCalculateGpu:
     """Arc-length table + sampler over one curve batch on one GPU.
 
@@ -521,13 +542,10 @@ class ArclengthCalculateGpu:
         DeviceArray (owned by the caller, free with .free()).
         """
         points = what == gcurvalReverseReturnPoints
-        n = np.ascontiguousarray(n_ptrs_for_curves, dtype=np.int64)
-        assert n.size == self.n_curve
-        off = np.zeros(self.n_curve + 1, dtype=np.int64)
-        np.cumsum(n, out=off[1:])
+        off, pos = sample_offsets(n_ptrs_for_curves, self.n_curve, sample_position)
         total = int(off[-1])
         d_off = self.ctx.array(off)
-        d_pos = self.ctx.array(np.asarray(sample_position, dtype=np.float64)) if sample_position is not None else None
+        d_pos = self.ctx.array(pos) if pos is not None else None
         d_out = self.ctx.empty(max(total, 1), np.float64)
         d_seg = self.ctx.empty(max(total, 1), np.int32) if (return_segments or points) else None
         try:

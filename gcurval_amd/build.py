@@ -66,7 +66,8 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     return LIB
 
 
-TEST_DRIVERS = [("tests/cpp/template_surface.hip", "tests/cpp/template_surface")]
+TEST_DRIVERS = [("tests/cpp/template_surface.hip", "tests/cpp/template_surface"),
+                ("tests/cpp/plugin_engines.hip", "tests/cpp/plugin_engines")]
 
 
 def build_tests(verbose: bool = True) -> list:

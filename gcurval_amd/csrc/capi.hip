@@ -897,10 +897,14 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
     gcv_ctx* c = t->ctx;
     hipError_t e = setup_done ? hipSuccess : build_setup<Dt, NGL>(t);
     if (e != hipSuccess) return e;
-    const int n_cls = t->kind == KIND_ELLIPSE ? 1 : (int)std::max<size_t>(1, t->classes.size());
-    const int n_bins = n_cls * dev::kSeedBins;
     const int64_t n = t->n_int;
-    if (n > 0) {
+    if (n == 0) {  // an empty batch: no bins, no launches (an empty B-spline batch has no classes)
+        t->seeded = true;
+        return build_scan(t);
+    }
+    const int n_cls = t->kind == KIND_ELLIPSE ? 1 : (int)t->classes.size();
+    const int n_bins = n_cls * dev::kSeedBins;
+    {
         const unsigned g1 = (unsigned)((n + 255) / 256);
         if (floor_done) {
             // the learned predictor wrote the floor with its features
@@ -1336,7 +1340,7 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     }
     prof_begin(c, "k_block_curve");
     hipLaunchKernelGGL(dev::k_block_curve, dim3((unsigned)((t->n_curve + 255) / 256)), dim3(256), 0, c->stream,
-                       t->n_curve, d_soff, t->blk_curve);
+                       t->n_curve, d_soff, n_blk, t->blk_curve);
     e = hipGetLastError();
     prof_end(c);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "block-curve launch failed: %s", hipGetErrorString(e));

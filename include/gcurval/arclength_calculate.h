@@ -236,20 +236,35 @@ inline void launch_check(const char* what) {
 // ---------------------------------------------------------------------------
 // The engine.
 // ---------------------------------------------------------------------------
+// BufferManager: accepted for the reference's five-parameter signature and
+// ignored (BufferManagerBase in base.h).
 template <typename Curve, typename CurveTraits,
           int max_recursive_depth = MaxRecursiveDepth<typename Curve::DataType>::value,
-          int n_subdivided = N_SUBDIVIDED>
+          int n_subdivided = N_SUBDIVIDED,
+          typename BufferManager = BufferManagerBase<typename Curve::DataType, max_recursive_depth,
+                                                     typename CurveTraits::ForwardSpace,
+                                                     typename CurveTraits::BackwardSpace>>
 class ArclengthCalculateGpu {
 public:
     using Dt = typename CurveTraits::DataType;
     using PointType = typename CurveTraits::PointType;
     using Eval = detail::PluginEval<Curve, CurveTraits>;
     using Alloc = typename CurveTraits::SharedMemAllocator;
+    static constexpr int MAX_RECURSIVE_DEPTH = max_recursive_depth;
 
     static_assert(n_subdivided >= 1 && n_subdivided <= 64 && (n_subdivided & (n_subdivided - 1)) == 0,
                   "n_subdivided must be a power of two in [1, 64]");
     static_assert(max_recursive_depth >= 1 && max_recursive_depth <= (sizeof(Dt) <= 4 ? 20 : 31),
                   "max_recursive_depth exceeds the task-key range (20 in FP32, 31 in FP64)");
+    // FP32 task bounds are exact dyadic floats only while depth + log2(n_subdivided) <= 24
+    static_assert(sizeof(Dt) > 4 || max_recursive_depth + (n_subdivided >= 64   ? 6
+                                                            : n_subdivided >= 32 ? 5
+                                                            : n_subdivided >= 16 ? 4
+                                                            : n_subdivided >= 8  ? 3
+                                                            : n_subdivided >= 4  ? 2
+                                                            : n_subdivided >= 2  ? 1
+                                                                                 : 0) <= 24,
+                  "FP32: max_recursive_depth + log2(n_subdivided) must not exceed 24");
 
     // n_seg: total knot intervals over all curves (the plug-in's locator
     // ranges must tile [0, n_seg) in curve order).
@@ -350,7 +365,7 @@ public:
             copyToDevice(d_pos, pos.data(), pos.size());
         }
         hipLaunchKernelGGL(dev::k_block_curve, dim3((unsigned)((n_curve + 255) / 256)), dim3(256), 0, s, n_curve,
-                           (const int64_t*)d_off, d_blk);
+                           (const int64_t*)d_off, n_blk, d_blk);
         detail::launch_check("k_block_curve");
 
         dev::SampleArgs A{};
@@ -441,7 +456,7 @@ private:
 
     int n_seg_;
     int n_cu_ = 256;
-    double tol_ = sizeof(Dt) <= 4 ? 1e-6 : 1e-14;
+    double tol_ = default_max_error<Dt>();
     bool located_ = false, built_ = false;
     std::vector<void*> owned_;
     int64_t* ioff_ = nullptr;

@@ -4,10 +4,13 @@
 // gcurval/base/functors.inl.h:9-342, gcurval/base/traits.inl.h:11-30,
 // gcurval/base/util.h:14-104) so existing plug-ins compile unchanged apart
 // from the CUDA/Eigen includes. Compiled by hipcc in the user's translation
-// unit; device code targets gfx950.
+// unit (device code targets gfx950); a host compiler (g++) may include it too,
+// for the CPU engine alone (arclength_calculate_cpu.h).
 #pragma once
 
+#if defined(__HIP__)
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cmath>
 #include <cstdint>
@@ -15,7 +18,11 @@
 #include <utility>
 
 #ifndef GCURVAL_HD_FUNC
+#if defined(__HIP__)
 #define GCURVAL_HD_FUNC __host__ __device__
+#else
+#define GCURVAL_HD_FUNC
+#endif
 #endif
 
 namespace gcurval {
@@ -23,13 +30,35 @@ namespace gcurval {
 // Where a result lives and what it holds (gcurval/base/util.h:71-77).
 enum GcurvalEnum { gcurvalDevice, gcurvalHost, gcurvalReverseReturnUs, gcurvalReverseReturnPoints };
 
-// Root sub-intervals per knot interval and the depth cap of the adaptive test
-// (gcurval/config.h:17-43).
+// Root sub-intervals per knot interval, the error bound and the depth cap of
+// the adaptive test, with the reference's override macros GCURVAL_N_SUBDIVIDED,
+// GCURVAL_MAX_ERROR and GCURVAL_MAX_RECURSIVE_DEPTH (gcurval/config.h:17-43).
+#ifdef GCURVAL_N_SUBDIVIDED
+constexpr int N_SUBDIVIDED = GCURVAL_N_SUBDIVIDED;
+#else
 constexpr int N_SUBDIVIDED = 64;
+#endif
+#ifdef GCURVAL_MAX_ERROR
+constexpr float MAX_ERROR_FLOAT = GCURVAL_MAX_ERROR;
+constexpr double MAX_ERROR_DOUBLE = GCURVAL_MAX_ERROR;
+#else
+constexpr float MAX_ERROR_FLOAT = 1e-6f;
+constexpr double MAX_ERROR_DOUBLE = 1e-14;
+#endif
 template <typename Dt>
 struct MaxRecursiveDepth {
+#ifdef GCURVAL_MAX_RECURSIVE_DEPTH
+    static constexpr int value = GCURVAL_MAX_RECURSIVE_DEPTH;
+#else
     static constexpr int value = sizeof(Dt) <= 4 ? 14 : 23;
+#endif
 };
+// Default absolute error bound of an engine in Dt (MAX_ERROR_FLOAT or
+// MAX_ERROR_DOUBLE as a double).
+template <typename Dt>
+constexpr double default_max_error() {
+    return sizeof(Dt) <= 4 ? (double)MAX_ERROR_FLOAT : MAX_ERROR_DOUBLE;
+}
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
@@ -55,7 +84,7 @@ template <typename T, int N>
 struct VectorN {
     T v[N];
 
-    GCURVAL_HD_FUNC VectorN() {}
+    VectorN() = default;  // VectorN{} is the zero vector
     template <typename... A, typename = std::enable_if_t<sizeof...(A) == N && (N > 1)>>
     GCURVAL_HD_FUNC VectorN(A... a) : v{T(a)...} {}
     GCURVAL_HD_FUNC explicit VectorN(T s) {
@@ -155,7 +184,7 @@ struct CurvesBase {
     using DataType = Dt;
     using PointType = Pt;
     int n_curve = 0;
-    GCURVAL_HD_FUNC CurvesBase(int n = 0) : n_curve(n) {}
+    GCURVAL_HD_FUNC constexpr CurvesBase(int n = 0) : n_curve(n) {}
 };
 
 // Knot interval [begin, end] of curve cid, its segment index sid, and
@@ -209,6 +238,18 @@ struct ParaPackParserBase {
 template <typename Dt>
 struct ParaPackSaverBase {};
 
+// The fifth template parameter of the GPU engine
+// (gcurval/detail/cuda/arclength_calculate.h:13-90, the per-level task
+// buffers of the reference's BFS). Accepted so that engine declarations keep
+// their signature; the engine ignores it: its refinement tasks live in a
+// wave-local LDS stack, so there are no per-level buffers to manage.
+template <typename Dt, int max_recursive_depth, typename ForwardSpace = ForwardSpaceBase<Dt>,
+          typename BackwardSpace = BackwardSpaceBase<Dt>>
+struct BufferManagerBase {
+    static constexpr int N_FSPACE = 2;
+    static constexpr int N_BSPACE = max_recursive_depth + 1;
+};
+
 // One knot interval per curve: global interval range {cid, cid + 1}.
 template <typename Curve>
 struct CurveLocatorBase {
@@ -252,7 +293,7 @@ struct GaussLegendreQuadrature {
     GCURVAL_HD_FUNC Dt eval(Dt begin, Dt end, const GetDerivative& d, const Pack& pack, View* view = nullptr) const {
         const Dt k = (end - begin) / Dt(2);
         const Dt b = (end + begin) / Dt(2);
-        const Dt x1 = Dt(0.7745966692414834), w0 = Dt(0.8888888888888888), w1 = Dt(0.5555555555555556);
+        const Dt x1 = Dt(0.7745966692414834), w0 = Dt(0.8888888888888889), w1 = Dt(0.5555555555555556);
         Dt acc = w0 * detail::magnitude<Dt>(detail::call(d, b, pack, view));
         acc += w1 * detail::magnitude<Dt>(detail::call(d, b - k * x1, pack, view));
         acc += w1 * detail::magnitude<Dt>(detail::call(d, b + k * x1, pack, view));

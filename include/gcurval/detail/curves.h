@@ -178,7 +178,7 @@ __device__ __forceinline__ double sqrt_pos(double v) {
 // cos^2(2 pi x). FP32: V is a degree-5 weighted least-squares fit (1.9e-7
 // relative for sin^2, 2.3e-7 for cos^2 in FP32 evaluation); FP64: degree 9,
 // the truncated Chebyshev expansion in powers of h (about 2 ulp).
-// Derivation: gcurval_amd/fit_sincos.py.
+// Derivation: tests/golden/fit_sincos.py.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float sin2_2pi(float x, float shift) {
     const float y = fmaf(2.0f, x, shift);

@@ -580,6 +580,20 @@ __device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t lo,
     return lo;
 }
 
+// blk_curve[b] as a valid curve index: an entry no curve wrote (offsets that
+// do not cover [0, n_total)) reads as some curve, never out of range.
+__device__ __forceinline__ int64_t blk_curve_at(const SampleArgs& A, const int32_t* blk_curve, int64_t b) {
+    const int64_t c = blk_curve[b];
+    return c < 0 ? 0 : (c < A.n_curve ? c : A.n_curve - 1);
+}
+
+// The curve c in [c0, c1] (c0 <= c1) holding sample g: the last with
+// soff[c] <= g, and c0 when none is (inconsistent offsets).
+__device__ __forceinline__ int curve_in(const SampleArgs& A, int64_t c0, int64_t c1, int64_t g) {
+    const int64_t c = upper_bound_i64(A.soff, c0, c1 + 1, g) - 1;
+    return (int)(c > c0 ? c : c0);
+}
+
 // Number of entries of the non-decreasing a[0, n) below v, i.e. the first
 // index with a[idx] >= v (n if none); 32-bit offsets from a 64-bit base.
 template <class P>
@@ -915,7 +929,7 @@ static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, con
     const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n_blk) return;
-    const int c = blk_curve[b];
+    const int c = (int)blk_curve_at(A, blk_curve, b);
     const double s = sample_target(A, c, b * kSampleBlock);
     blk_int[b] = (int32_t)interval_of_target(A, c, s, 0, INT64_MAX);
 }
@@ -1079,22 +1093,28 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
 // Curve of the first sample of every block of kSampleBlock samples: thread per
 // curve writes the blocks whose first sample falls in its range, so the
 // sampler finds a lane's curve with one load and a search over the few
-// curves one block spans.
+// curves one block spans. The block range is clamped to [0, n_blk), so sample
+// offsets that are not non-decreasing from 0 to n_total write nothing out of
+// bounds (their results are unspecified, see blk_curve_at).
 
-static __global__ __launch_bounds__(256) void k_block_curve(int n_curve, const int64_t* soff, int32_t* blk_curve) {
+static __global__ __launch_bounds__(256) void k_block_curve(int n_curve, const int64_t* soff, int64_t n_blk,
+                                                            int32_t* blk_curve) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_curve) return;
-    const int64_t b0 = (soff[c] + kSampleBlock - 1) / kSampleBlock;
-    const int64_t b1 = (soff[c + 1] + kSampleBlock - 1) / kSampleBlock;
+    int64_t b0 = (soff[c] + kSampleBlock - 1) / kSampleBlock;
+    int64_t b1 = (soff[c + 1] + kSampleBlock - 1) / kSampleBlock;
+    b0 = b0 < 0 ? 0 : b0;
+    b1 = b1 < n_blk ? b1 : n_blk;
     for (int64_t b = b0; b < b1; b++) blk_curve[b] = (int32_t)c;
 }
 
 // Curve of sample g of block `blk` (blockDim.x == kSampleBlock).
 __device__ __forceinline__ int sample_curve(const SampleArgs& A, const int32_t* blk_curve, int64_t blk, int64_t g) {
     const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
-    const int64_t c0 = blk_curve[blk];
-    const int64_t c1 = blk + 1 < n_blk ? blk_curve[blk + 1] : A.n_curve - 1;
-    return (int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1);
+    const int64_t c0 = blk_curve_at(A, blk_curve, blk);
+    int64_t c1 = blk + 1 < n_blk ? blk_curve_at(A, blk_curve, blk + 1) : A.n_curve - 1;
+    c1 = c1 > c0 ? c1 : c0;
+    return curve_in(A, c0, c1, g);
 }
 
 // Curve of sample g and, with the block interval map (uniform positions), the
@@ -1108,12 +1128,13 @@ struct SampleLoc {
 __device__ __forceinline__ SampleLoc locate_sample(const SampleArgs& A, const int32_t* blk_curve, int64_t blk,
                                                    int64_t g) {
     const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
-    const int64_t c0 = blk_curve[blk];
+    const int64_t c0 = blk_curve_at(A, blk_curve, blk);
     const bool has_next = blk + 1 < n_blk;
-    const int64_t c1 = has_next ? blk_curve[blk + 1] : A.n_curve - 1;
+    int64_t c1 = has_next ? blk_curve_at(A, blk_curve, blk + 1) : A.n_curve - 1;
+    c1 = c1 > c0 ? c1 : c0;
     // a block whose first sample and the next block's are on one curve lies
     // on that curve
-    SampleLoc L{c0 == c1 ? (int)c0 : (int)(upper_bound_i64(A.soff, c0, c1 + 1, g) - 1), 0, INT64_MAX};
+    SampleLoc L{c0 == c1 ? (int)c0 : curve_in(A, c0, c1, g), 0, INT64_MAX};
     if (A.blk_int) {
         if (L.c == c0) L.lo = (int64_t)A.blk_int[blk] - 1;
         if (has_next && L.c == c1) L.hi = (int64_t)A.blk_int[blk + 1] + 1;
@@ -1135,8 +1156,8 @@ __global__ __launch_bounds__(256) void k_sample(Ev ev, Ev64 ev64, SampleArgs A, 
     int32_t seg;
     double u;
     const int64_t n_blk = (A.n_total + kSampleBlock - 1) / kSampleBlock;
-    const int c0 = blk_curve[blockIdx.x];
-    if (!A.blk_int && blockIdx.x + 1 < n_blk && blk_curve[blockIdx.x + 1] == c0) {
+    const int c0 = (int)blk_curve_at(A, blk_curve, blockIdx.x);
+    if (!A.blk_int && blockIdx.x + 1 < n_blk && blk_curve_at(A, blk_curve, blockIdx.x + 1) == c0) {
         // the whole block lies on curve c0: the curve index is block-uniform,
         // so its per-curve data comes through scalar loads issued together
         u = sample_one<Dt, NGL, M>(ev, ev64, A, c0, g, &seg, 0, INT64_MAX, win, iwin);
@@ -1207,9 +1228,10 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
             if (next < S.s0 || next >= S.s1) {
                 // load the curve holding sample `next`
                 const int64_t blk = next / kSampleBlock;
-                const int64_t ca = readfirstlane64(blk_curve[blk]);
-                const int64_t cb = readfirstlane64(blk + 1 < n_blk ? (int64_t)blk_curve[blk + 1] : A.n_curve - 1);
-                const int64_t cn = ca == cb ? ca : readfirstlane64(upper_bound_i64(A.soff, ca, cb + 1, next) - 1);
+                const int64_t ca = readfirstlane64(blk_curve_at(A, blk_curve, blk));
+                int64_t cb = readfirstlane64(blk + 1 < n_blk ? blk_curve_at(A, blk_curve, blk + 1) : A.n_curve - 1);
+                cb = cb > ca ? cb : ca;
+                const int64_t cn = ca == cb ? ca : readfirstlane64(curve_in(A, ca, cb, next));
                 State N;
                 N.cr = cn;
                 N.s0 = A.soff[cn];
@@ -1268,7 +1290,9 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                                                     rlen, true, E, S.quintic != 0);
                 if (A.out_seg) A.out_seg[g] = 0;
             }
-            if (gl < S.s1) break;
+            // S.s1 <= next only for offsets that do not cover [0, n_total):
+            // the rest of the tile is left unwritten instead of looping
+            if (gl < S.s1 || S.s1 <= next) break;
             next = S.s1;
         }
     }

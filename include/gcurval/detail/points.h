@@ -47,7 +47,10 @@ static __global__ __launch_bounds__(256) void k_points_bspline(int n_curve, int6
     const int p = degree[c], D = dim[c];
     const double* T = knots + knot_off[c];
     const double* P = ctrl + ctrl_off[c] * ctrl_stride;
-    const int s = seg[g] + p;
+    // span of the sampler's interval index, clamped to the curve's spans
+    const int np = (int)(ctrl_off[c + 1] - ctrl_off[c]);
+    int s = seg[g] + p;
+    s = s < p ? p : (s > np - 1 ? np - 1 : s);
     const double x = u[g];
     double d[6][3];
     for (int j = 0; j <= p; j++)

@@ -178,6 +178,10 @@ int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, u
  * 408-505; kernel/reverse.inl.cuh:32-99, 216-341).
  *   d_sample_off: device int64[n_curve+1], samples of curve c are
  *                 [off[c], off[c+1]) (the partial sum of n_ptrs_for_curves).
+ *                 The offsets must be non-decreasing with off[0] = 0 and
+ *                 off[n_curve] = n_total (they stay on the device, so the
+ *                 call does not check them): other offsets give unspecified
+ *                 parameters, but no out-of-bounds access and no hang.
  *   d_pos:        device double[n_total] positions, or NULL for uniform
  *                 positions j/(N_c-1) (uniform_closed=1) or j/N_c (0).
  *   is_relative:  positions are fractions of the curve length (else arc lengths).

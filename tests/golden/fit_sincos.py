@@ -14,7 +14,7 @@ relative error on a Chebyshev grid in h, then rounded to float.
 FP64: V of degree 9, the Chebyshev expansion of V on h in [0, 1/4] truncated
 after T_9 and converted to powers of h in 40-digit arithmetic (the first
 omitted Chebyshev coefficient is 6.7e-17 of min V).
-Run: python -m gcurval_amd.fit_sincos
+Run: python tests/golden/fit_sincos.py
 """
 import mpmath as mp
 import numpy as np

@@ -270,6 +270,25 @@ def test_empty_batch_and_zero_samples(ctx):
         assert u.size == 5 and np.all(np.isfinite(u))
 
 
+def test_empty_bspline_batch_seeded_and_learned_builds(ctx):
+    """An empty B-spline batch has no evaluator classes: seeded, uniform and
+    learned builds must launch nothing and leave an empty table."""
+    from gcurval_amd.seed import SeedMLP
+    z64, z32 = np.zeros(0, np.int64), np.zeros(0, np.int32)
+    b = G.BSplineBatch(z32, z32, np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(0), np.zeros((0, 3)))
+    with G.ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
+        k = ctx.array(np.zeros(1, np.int8))
+        eng.createArclengthTableSeeded(k)
+        assert eng.getAllCurveArclengths().size == 0
+        eng.createArclengthTableUniform(3)
+        assert eng.getAllCurveArclengths().size == 0
+        with SeedMLP.pretrained(ctx) as mlp:
+            eng.createArclengthTableLearned(mlp)
+            assert eng.getAllCurveArclengths().size == 0
+        assert eng.arclengthReverse(z64, None, True).size == 0
+        k.free()
+
+
 @pytest.mark.parametrize("closed", [True, False])
 def test_ragged_sample_counts_bspline(ctx, closed):
     """Per-curve counts 0, 1, 2, 3 and several hundred in one batch, so sample

@@ -64,3 +64,16 @@ def test_degenerate_span_gives_zero_polynomial():
     ctrl = np.random.default_rng(0).normal(size=(6, 3))
     with pytest.raises(G.GcvError):  # span 4 has zero width
         G.host_bspline_span_poly(3, 3, 4, knots, ctrl)
+
+
+def test_sample_offsets_rejects_bad_counts_and_positions():
+    off, pos = G.sample_offsets([2, 0, 3], 3)
+    assert off.tolist() == [0, 2, 2, 5] and pos is None
+    off, pos = G.sample_offsets([1, 2], 2, [0.0, 0.5, 1.0])
+    assert pos.dtype == np.float64 and pos.size == 3
+    with pytest.raises(ValueError, match="negative"):
+        G.sample_offsets([2, -1, 3], 3)
+    with pytest.raises(ValueError, match="per-curve counts"):
+        G.sample_offsets([2, 3], 3)
+    with pytest.raises(ValueError, match="positions expected"):
+        G.sample_offsets([2, 3], 2, np.zeros(4))

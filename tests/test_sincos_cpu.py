@@ -10,6 +10,7 @@ out exactly zero. The derivation pins the FP64 constants, and a planted
 wrong coefficient in the oracle of the polynomial fails the accuracy bar."""
 import os
 import re
+import sys
 
 import mpmath as mp
 import numpy as np
@@ -79,7 +80,8 @@ def test_fp32_polynomial_within_3e7():
 
 
 def test_constants_are_the_derivation():
-    from gcurval_amd import fit_sincos as F
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import fit_sincos as F
     np.testing.assert_array_equal(np.array(_constants("double")[::-1]), np.array(F.cheb_fp64()))
 
 
