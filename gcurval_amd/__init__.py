@@ -109,6 +109,8 @@ _SIGS = [
     ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5),
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("gcv_sample_noalloc", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("gcv_ctx_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_points", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("gcv_graph_begin", C.c_int, [C.c_void_p]),

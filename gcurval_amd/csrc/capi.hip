@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -36,8 +37,37 @@ struct ProfRec {
 // launches are spread over them so one bin's tail overlaps the next bin.
 constexpr int kAuxStreams = 3;
 
+// Scratch of gcv_sample_noalloc: the block -> curve and block -> interval
+// maps of up to kNoAllocBlocks sample blocks (kNoAllocBlocks * kSampleBlock
+// samples), 8 MiB held by every device the library runs on until the process
+// ends. In host code `scratch` only names the symbol; kernels get the device
+// address that hipGetSymbolAddress returns for the current device.
+constexpr int64_t kNoAllocBlocks = int64_t(1) << 20;
+static __device__ __align__(256) int32_t scratch[2 * kNoAllocBlocks];
+
+// Device address of `scratch` per device, resolved once per device (with
+// that device current) by the first context created on it.
+static std::mutex g_scratch_mu;
+static std::map<int, int32_t*> g_scratch_ptr;
+
+static hipError_t resolve_scratch(int device, int32_t** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (int32_t*)ptr;
+    *out = (int32_t*)ptr;
+    return hipSuccess;
+}
+
 struct gcv_ctx {
     int device = 0;
+    int32_t* scratch_ptr = nullptr;  // this device's `scratch` (gcv_sample_noalloc)
     int n_cu = 0;
     char dev_name[256] = {0};
     hipStream_t stream = nullptr;
@@ -216,7 +246,7 @@ int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
     for (int i = 0; ok && i < kAuxStreams; i++)
         ok = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
+    if (!ok || resolve_scratch(device, &c->scratch_ptr) != hipSuccess) {
         ctx_release(c);
         return GCV_ERR_HIP;
     }
@@ -1278,16 +1308,25 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
     return GCV_OK;
 }
 
-extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
-                          int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg) {
-    if (t) use_device(t->ctx);
-    if (!t) return GCV_ERR_INVALID;
+// Arguments gcv_sample and gcv_sample_noalloc check alike (0: go on).
+static int sample_check(gcv_table* t, const int64_t* d_soff, int64_t n_total, double* d_out_u) {
     gcv_ctx* c = t->ctx;
     if (!t->built) return set_err(c, GCV_ERR_STATE, "arclengthReverse before createArclengthTable");
     if (n_total < 0) return set_err(c, GCV_ERR_INVALID, "n_total < 0");
     if (n_total == 0) return GCV_OK;
     if (!d_soff || !d_out_u) return set_err(c, GCV_ERR_INVALID, "null sample offsets or output");
     if (t->n_curve == 0) return set_err(c, GCV_ERR_INVALID, "samples requested from an empty batch");
+    return GCV_OK;
+}
+
+// Sampling with the block maps in blk_curve / blk_int (n_total / kSampleBlock
+// entries each, rounded up): the kernels of gcv_sample. `profiled`: the
+// launches are timed when the context profiles (event creation included).
+static int sample_launch(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
+                         int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg,
+                         int32_t* blk_curve, int32_t* blk_int, bool profiled) {
+    gcv_ctx* c = t->ctx;
+    const int64_t n_blk = (n_total + dev::kSampleBlock - 1) / dev::kSampleBlock;
     dev::SampleArgs A{};
     A.n_curve = t->n_curve;
     A.n_total = n_total;
@@ -1311,9 +1350,55 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
     A.out_u = d_out_u;
     A.out_seg = d_out_seg;
     A.class_of_curve = t->class_of_curve;
-    const LaunchCfg L = cfg_of(c);
+    LaunchCfg L = cfg_of(c);
+    void* const prof = profiled ? (void*)c : nullptr;  // the profiling hooks skip a null owner
+    L.owner = prof;
     hipError_t e;
-    // block -> curve table (grown on demand; reused while large enough)
+    prof_begin(prof, "k_block_curve");
+    hipLaunchKernelGGL(dev::k_block_curve, dim3((unsigned)((t->n_curve + 255) / 256)), dim3(256), 0, c->stream,
+                       t->n_curve, d_soff, n_blk, blk_curve);
+    e = hipGetLastError();
+    prof_end(prof);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "block-curve launch failed: %s", hipGetErrorString(e));
+    if (!d_pos && t->n_int > t->n_curve) {
+        // uniform targets are monotone along each curve: per-block interval
+        // bounds shorten every sample's interval search
+        prof_begin(prof, "k_block_interval");
+        hipLaunchKernelGGL(dev::k_block_interval, dim3((unsigned)((n_blk + 255) / 256)), dim3(256), 0, c->stream, A,
+                           (const int32_t*)blk_curve, blk_int);
+        e = hipGetLastError();
+        prof_end(prof);
+        if (e != hipSuccess)
+            return set_err(c, GCV_ERR_HIP, "block-interval launch failed: %s", hipGetErrorString(e));
+        A.blk_int = blk_int;
+    }
+    const bool f32 = t->o.precision == GCV_FP32;
+    const bool g3 = t->o.gl_order == 3;
+    const int32_t* bc = blk_curve;
+    if (t->kind == KIND_ELLIPSE) {
+        if (f32) e = g3 ? Launch<float, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
+                        : Launch<float, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
+        else e = g3 ? Launch<double, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
+                    : Launch<double, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
+    } else {
+        const BSplineTableDev T = bspline_dev(t);
+        if (f32) e = g3 ? Launch<float, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
+                        : Launch<float, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
+        else e = g3 ? Launch<double, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
+                    : Launch<double, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
+    }
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "sample launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
+extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
+                          int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg) {
+    if (t) use_device(t->ctx);
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (int rc = sample_check(t, d_soff, n_total, d_out_u)) return rc;
+    if (n_total == 0) return GCV_OK;
+    // block maps of the table (grown on demand; reused while large enough)
     const int64_t n_blk = (n_total + dev::kSampleBlock - 1) / dev::kSampleBlock;
     if (n_blk > t->blk_cap) {
         if (c->capturing)
@@ -1338,41 +1423,28 @@ extern "C" int gcv_sample(gcv_table* t, const int64_t* d_soff, int64_t n_total, 
         t->blk_int = (int32_t*)q;
         t->blk_cap = n_blk;
     }
-    prof_begin(c, "k_block_curve");
-    hipLaunchKernelGGL(dev::k_block_curve, dim3((unsigned)((t->n_curve + 255) / 256)), dim3(256), 0, c->stream,
-                       t->n_curve, d_soff, n_blk, t->blk_curve);
-    e = hipGetLastError();
-    prof_end(c);
-    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "block-curve launch failed: %s", hipGetErrorString(e));
-    if (!d_pos && t->n_int > t->n_curve) {
-        // uniform targets are monotone along each curve: per-block interval
-        // bounds shorten every sample's interval search
-        prof_begin(c, "k_block_interval");
-        hipLaunchKernelGGL(dev::k_block_interval, dim3((unsigned)((n_blk + 255) / 256)), dim3(256), 0, c->stream, A,
-                           (const int32_t*)t->blk_curve, t->blk_int);
-        e = hipGetLastError();
-        prof_end(c);
-        if (e != hipSuccess)
-            return set_err(c, GCV_ERR_HIP, "block-interval launch failed: %s", hipGetErrorString(e));
-        A.blk_int = t->blk_int;
-    }
-    const bool f32 = t->o.precision == GCV_FP32;
-    const bool g3 = t->o.gl_order == 3;
-    const int32_t* bc = t->blk_curve;
-    if (t->kind == KIND_ELLIPSE) {
-        if (f32) e = g3 ? Launch<float, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
-                        : Launch<float, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
-        else e = g3 ? Launch<double, 3>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc)
-                    : Launch<double, 5>::sample_ellipse(L, t->a, t->b, t->o.normalize, A, bc);
-    } else {
-        const BSplineTableDev T = bspline_dev(t);
-        if (f32) e = g3 ? Launch<float, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
-                        : Launch<float, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
-        else e = g3 ? Launch<double, 3>::sample_bspline(L, t->uniform_class, T, A, bc)
-                    : Launch<double, 5>::sample_bspline(L, t->uniform_class, T, A, bc);
-    }
-    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "sample launch failed: %s", hipGetErrorString(e));
-    return GCV_OK;
+    return sample_launch(t, d_soff, n_total, d_pos, uniform_closed, is_relative, d_out_u, d_out_seg, t->blk_curve,
+                         t->blk_int, true);
+}
+
+extern "C" int gcv_sample_noalloc(gcv_table* t, const int64_t* d_soff, int64_t n_total, const double* d_pos,
+                                  int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg) {
+    if (t) use_device(t->ctx);
+    if (!t) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (int rc = sample_check(t, d_soff, n_total, d_out_u)) return rc;
+    if (n_total == 0) return GCV_OK;
+    const int64_t n_blk = (n_total + dev::kSampleBlock - 1) / dev::kSampleBlock;
+    if (n_blk > kNoAllocBlocks)
+        return set_err(c, GCV_ERR_INVALID,
+                       "gcv_sample_noalloc: %lld samples exceed its static scratch (at most %lld samples)",
+                       (long long)n_total, (long long)(kNoAllocBlocks * dev::kSampleBlock));
+    if (!c->scratch_ptr) return set_err(c, GCV_ERR_STATE, "gcv_sample_noalloc: device scratch not resolved");
+    // the device copy of `scratch`, never the host-side symbol
+    int32_t* blk_curve = c->scratch_ptr;
+    int32_t* blk_int = c->scratch_ptr + kNoAllocBlocks;
+    return sample_launch(t, d_soff, n_total, d_pos, uniform_closed, is_relative, d_out_u, d_out_seg, blk_curve,
+                         blk_int, false);
 }
 
 extern "C" int gcv_host_bspline_span_poly(int p, int D, int s, const double* knots, const double* ctrl,

@@ -191,6 +191,19 @@ int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, u
 int gcv_sample(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_pos,
                int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg);
 
+/* gcv_sample that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as gcv_sample (same kernels), but the per-256-sample block maps
+ * live in a static device array of the library (8 MiB per device, held until
+ * the process ends) instead of the table: at most 268,435,456 samples
+ * (2^20 blocks of 256), larger n_total returns GCV_ERR_INVALID. Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (other
+ * streams, other contexts) race; calls on one context's stream are ordered.
+ * No profiling events are recorded for it. */
+int gcv_sample_noalloc(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_pos,
+                       int uniform_closed, int is_relative, double* d_out_u, int32_t* d_out_seg);
+
 /* Curve points at sampled parameters: the gcurvalReverseReturnPoints variant
  * of arclengthReverse (kn_toPoints, gcurval/detail/cuda/kernel/reverse.inl.cuh:344-363;
  * cuda/arclength_calculate.h:458-500). d_u and d_seg are gcv_sample's outputs
