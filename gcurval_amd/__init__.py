@@ -106,7 +106,7 @@ _SIGS = [
     ("gcv_table_lengths", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gcv_table_device_views", C.c_int,
      [C.c_void_p] + [C.POINTER(C.c_void_p)] * 5),
-    ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5),
+    ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 7),
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("gcv_sample_noalloc", C.c_int,
@@ -450,9 +450,12 @@ class ArclengthCalculateGpu:
         return out
 
     def root_layout(self):
-        sd, ls, ro = C.c_int(), C.c_int(), C.c_void_p()
-        self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(ro)))
-        return bool(sd.value), ls.value, ro.value
+        """(seeded, log2 roots of a uniform build, device int8 log2 root count
+        per interval of a seeded build, whose root block of interval i starts
+        at i * 64)."""
+        sd, ls, rk = C.c_int(), C.c_int(), C.c_void_p()
+        self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(rk)))
+        return bool(sd.value), ls.value, rk.value
 
     def createArclengthTableLearned(self, mlp, hedge: bool = False, d_cost: "DeviceArray | None" = None):
         """Seeded build whose per-interval root counts come from the learned predictor."""
@@ -495,9 +498,13 @@ class ArclengthCalculateGpu:
         self._lengths = None
 
     def stats(self):
-        v = [C.c_uint64() for _ in range(5)]
+        """Build statistics of the last build: quadratures, refined roots,
+        deepest task stack, overflow flag (must be 0), refinement tasks,
+        leaves that still failed the error test at the depth cap, and
+        seeded tiles re-run with the full task stack."""
+        v = [C.c_uint64() for _ in range(7)]
         self.ctx._check(lib().gcv_table_stats(self.h, *[C.byref(x) for x in v]))
-        keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks")
+        keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks", "depth_capped", "reruns")
         return {k: x.value for k, x in zip(keys, v)}
 
     def device_views(self):
@@ -506,24 +513,33 @@ class ArclengthCalculateGpu:
         return [x.value for x in v]
 
     def tables(self):
-        """Host copies: (int_cum[n_int], root_prefix[n_int*n_sub], deep[n_int], curve_len, int_off)."""
+        """Host copies: (int_cum[n_int], root prefixes, deep[n_int], curve_len, int_off). The root
+        prefixes are interval after interval, 2^log2_roots per interval (uniform builds) or 2^k_i
+        for interval i (seeded builds, compacted from their 64-slot blocks)."""
         ic, rp, dp, cl, io = self.device_views()
         L = lib()
         n_int = self.n_intervals
-        seeded, ls, ro = self.root_layout()
-        if seeded:
-            roff = np.empty(n_int + 1, np.int64)
-            if roff.nbytes:
-                self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(roff), ro, roff.nbytes))
-            n_roots = int(roff[-1])
-        else:
-            n_roots = n_int << ls
-        out = (np.empty(n_int, np.float64), np.empty(n_roots, np.float64), np.empty(n_int, np.uint64),
-               np.empty(self.n_curve, np.float64), np.empty(self.n_curve + 1, np.int64))
+        seeded, ls, rk = self.root_layout()
+        n_slots = n_int << (6 if seeded else ls)
+        out = [np.empty(n_int, np.float64), np.empty(n_slots, np.float64), np.empty(n_int, np.uint64),
+               np.empty(self.n_curve, np.float64), np.empty(self.n_curve + 1, np.int64)]
         for arr, p in zip(out, (ic, rp, dp, cl, io)):
             if arr.nbytes:
                 self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(arr), p, arr.nbytes))
-        return out
+        if seeded and n_int:
+            k = np.empty(n_int, np.int8)
+            self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(k), rk, k.nbytes))
+            blocks = out[1].reshape(n_int, 64)
+            out[1] = np.concatenate([blocks[i, :1 << int(k[i])] for i in range(n_int)])
+        return tuple(out)
+
+    def root_counts(self) -> np.ndarray:
+        """log2 root count per interval of the last build."""
+        seeded, ls, rk = self.root_layout()
+        k = np.full(self.n_intervals, ls, np.int8)
+        if seeded and k.size:
+            self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(k), rk, k.nbytes))
+        return k
 
     # -- sampling (arclengthReverse) --
     def sample_device(self, d_off: DeviceArray, n_total: int, d_pos: DeviceArray | None, is_relative: bool,

@@ -544,14 +544,14 @@ struct gcv_table {
     int tune_grid_pct = 100;
     int tune_streams = 1;                 // streams of the seeded build's per-bin launches (default_streams)
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
-    int64_t* root_off = nullptr;          // [n_int + 1]
-    int32_t* bucket_list = nullptr;       // [n_int]
+    int8_t* root_k = nullptr;             // [n_int] log2 root count of a seeded build (root block at i << 6)
+    int32_t* bucket_list = nullptr;       // [n_int] intervals sorted by (class, k)
     int32_t* bucket_hist = nullptr;       // [kMaxBins * kBucketMaxWaves] per-wave bin counts
     int8_t* bucket_bin = nullptr;         // [n_int] bin of each interval
     int32_t* bucket_totals = nullptr;     // [kMaxBins] intervals per bin
-    int64_t* bucket_ranges = nullptr;     // [2 kMaxBins]
-    int64_t* scan_blocks = nullptr;       // block sums of the root-offset scan
-    int64_t n_scan_blocks = 0;
+    int64_t* bucket_ranges = nullptr;     // [2 kMaxBins] {first slot, count}
+    int32_t* rerun_list = nullptr;        // [n_int] intervals of seeded tiles that outgrew the small stack
+    int seed_stack_cap = 0;               // task-stack entries per wave of the seeded bins kernel
     bool seeded = false;                  // last build used per-interval root counts
     int built_ls = 6;                     // log2 roots per interval of the last uniform build
     unsigned long long* stats = nullptr;
@@ -620,25 +620,25 @@ static int log2i(int v) {
 // Allocation shared by both families.
 static int table_alloc_common(gcv_table* t) {
     int rc;
-    // room for 64 roots per interval: seeded and uniform builds may use up to 2^6
-    const int64_t nroots = t->n_int << std::max(t->log2_nsub, (int)dev::kMaxSeedK);
+    // 64 root slots per interval: seeded builds keep interval i's 2^k_i roots at i << 6
+    const int64_t nroots = t->n_int << 6;
     if ((rc = talloc(t, &t->ioff, t->n_curve + 1))) return rc;
     if ((rc = talloc(t, &t->int_cum, t->n_int))) return rc;
     if ((rc = talloc(t, &t->root_prefix, nroots))) return rc;
     if ((rc = talloc(t, &t->deep, t->n_int))) return rc;
     if ((rc = talloc(t, &t->curve_len, t->n_curve))) return rc;
-    if ((rc = talloc(t, &t->stats, dev::kStatWords))) return rc;
+    if ((rc = talloc(t, &t->stats, dev::kStatBufWords))) return rc;
     // seeded-build arrays (allocated once, so builds never allocate)
-    t->n_scan_blocks = (t->n_int + 256 * dev::kScanItems - 1) / (256 * dev::kScanItems);
     if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
-        (rc = talloc(t, &t->root_off, t->n_int + 1)) ||
-        (rc = talloc(t, &t->bucket_list, t->n_int)) || (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketMaxWaves)) ||
+        (rc = talloc(t, &t->root_k, t->n_int)) || (rc = talloc(t, &t->bucket_list, t->n_int)) ||
+        (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketMaxWaves)) ||
         (rc = talloc(t, &t->bucket_bin, t->n_int)) || (rc = talloc(t, &t->bucket_totals, dev::kMaxBins)) ||
-        (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) ||
-        (rc = talloc(t, &t->scan_blocks, t->n_scan_blocks + 1)))
+        (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) || (rc = talloc(t, &t->rerun_list, t->n_int)))
         return rc;
     // depth bound of the wave-local stack: 64 (max_depth + 1), plus one block of slack
     t->stack_cap = 64 * (t->o.max_depth + 2);
+    // the seeded bins kernel keeps a small stack (tiles that outgrow it are re-run)
+    t->seed_stack_cap = 256;
     const int nsub = 1 << t->log2_nsub;
     if (t->o.precision == GCV_FP32) t->eps0 = (double)((float)t->o.tolerance / (float)nsub);
     else t->eps0 = t->o.tolerance / (double)nsub;
@@ -821,9 +821,9 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
 template <typename Dt, int NGL>
 static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const dev::FeatSetup* fs = nullptr) {
     gcv_ctx* c = t->ctx;
-    if (reset_stats)
-        LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3((dev::kStatWords + 255) / 256), dim3(256), 0,
-                         c->stream, t->stats, dev::kStatWords, 0ull);
+    if (reset_stats)  // statistics rows and the seeded build's queue words
+        LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3((dev::kStatBufWords + 255) / 256), dim3(256), 0,
+                         c->stream, t->stats, dev::kStatBufWords, 0ull);
     if (t->kind == KIND_ELLIPSE) return hipSuccess;
     dev::BSplineDev B{t->n_curve, t->bat.ctrl_stride, t->bat.degree, t->bat.dim, t->bat.knot_off,
                       t->bat.ctrl_off, t->bat.knots, t->bat.ctrl};
@@ -918,9 +918,13 @@ static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
     return build_scan(t);
 }
 
-// Seeded build: interval i starts from 2^k_i roots. Intervals are bucketed
-// by (evaluator class, k) on the device and root storage is compacted by an
-// exclusive scan of 2^k_i; every launch reads its bucket range on the device.
+// Seeded build: interval i starts from 2^k_i roots, k_i = max(k[i], safety
+// floor). Per evaluator class, the chunk kernel bins the class's intervals by
+// k wave by wave and runs the 1..32-root tiles, appending the 64-root
+// intervals to the class's region of whole_list, which the whole-interval
+// kernel then runs (its range is read on the device). No host sync, nothing
+// allocated, graph-capturable; classes are independent, so with
+// tune_streams > 1 they run on the auxiliary streams.
 template <typename Dt, int NGL>
 static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bool setup_done = false,
                                bool floor_done = false) {
@@ -928,12 +932,12 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
     hipError_t e = setup_done ? hipSuccess : build_setup<Dt, NGL>(t);
     if (e != hipSuccess) return e;
     const int64_t n = t->n_int;
-    if (n == 0) {  // an empty batch: no bins, no launches (an empty B-spline batch has no classes)
+    if (n == 0) {  // an empty batch: no launches (an empty B-spline batch has no classes)
         t->seeded = true;
         return build_scan(t);
     }
     const int n_cls = t->kind == KIND_ELLIPSE ? 1 : (int)t->classes.size();
-    const int n_bins = n_cls * dev::kSeedBins;
+    if (n_cls > dev::kQueueClasses) return hipErrorInvalidValue;
     {
         const unsigned g1 = (unsigned)((n + 255) / 256);
         if (floor_done) {
@@ -946,7 +950,10 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
                              t->int_curve, t->bat.degree, t->bat.dim, t->coef32, t->coef64, t->cstride, t->log2_nsub,
                              t->seed_floor);
         }
-        dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve};
+    }
+    {
+        const int n_bins = n_cls * dev::kSeedBins;
+        dev::BinArgs BA{kk, t->seed_floor, t->cls_idx, t->kind == KIND_ELLIPSE ? nullptr : t->int_curve, t->root_k};
         // wave chunks of kBucketChunk intervals, longer when that would exceed kBucketMaxWaves
         const int64_t wchunk = std::max<int64_t>(
             dev::kBucketChunk, ((n + dev::kBucketMaxWaves - 1) / dev::kBucketMaxWaves + 63) / 64 * 64);
@@ -960,48 +967,53 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
                          t->bucket_hist, (const int32_t*)t->bucket_totals, t->bucket_ranges);
         LAUNCH_OR_RETURN("k_bucket_place", dev::k_bucket_place, dim3(gb), dim3(256), 0, c->stream, n, n_bins,
                          wchunk, nw, (const int8_t*)t->bucket_bin, (const int32_t*)t->bucket_hist, t->bucket_list);
-        const unsigned nb = (unsigned)t->n_scan_blocks;
-        LAUNCH_OR_RETURN("k_root_block_sums", dev::k_root_block_sums, dim3(nb), dim3(256), 0, c->stream, n, kk,
-                         t->seed_floor, t->scan_blocks);
-        LAUNCH_OR_RETURN("k_scan_block_sums", dev::k_scan_block_sums, dim3(1), dim3(1024), 0, c->stream,
-                         (int64_t)nb, t->scan_blocks);
-        LAUNCH_OR_RETURN("k_root_apply", dev::k_root_apply, dim3(nb), dim3(256), 0, c->stream, n, kk,
-                         t->seed_floor, t->scan_blocks, t->root_off);
     }
     const BSplineTableDev T = bspline_dev(t);
-    // the (class, k) bins are independent: with tune_streams > 1 they are
-    // spread over the auxiliary streams so one launch's tail overlaps the next
+    const bool f32 = sizeof(Dt) == 4;
     const int n_streams = std::min(std::max(t->tune_streams, 1), kAuxStreams);
-    const bool multi = n_streams > 1;
+    const bool multi = n_streams > 1 && n_cls > 1;
     if (multi && (e = aux_fork(c)) != hipSuccess) return e;
-    int launch_no = 0;
-    for (int ci = 0; ci < n_cls; ci++) {
-        for (int k = 0; k <= dev::kMaxSeedK; k++) {
-            LaunchCfg L = cfg_of_table(t);
-            if (multi) L.stream = c->aux[launch_no++ % n_streams];
-            dev::SubdivArgs<Dt> A = subdiv_args<Dt>(t);
-            A.log2_nsub = k;
-            A.eps0 = eps_of<Dt>(t);
-            A.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, k, sizeof(Dt) == 4);
-            A.stack_cap = 64 * (A.max_depth + 2);
-            A.int_list = t->bucket_list;
-            A.slot_range = t->bucket_ranges + 2 * (ci * dev::kSeedBins + k);
-            A.n_slots = n;  // upper bound: sizes the persistent grid
-            A.root_off = t->root_off;
-            A.int_cost = cost;
-            if (t->kind == KIND_ELLIPSE) {
-                e = Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
-            } else {
-                const ClassRange& r = t->classes[ci];
-                e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, A);
-            }
-            if (e != hipSuccess) {
-                if (multi) aux_join(c);
-                return e;
-            }
+    for (int ci = 0; ci < n_cls && e == hipSuccess; ci++) {
+        LaunchCfg L = cfg_of_table(t);
+        if (multi) L.stream = c->aux[ci % n_streams];
+        const int64_t off = t->kind == KIND_ELLIPSE ? 0 : t->classes[ci].offset;
+        const int64_t cnt = t->kind == KIND_ELLIPSE ? n : t->classes[ci].count;
+        unsigned long long* q = t->stats + dev::kStatWords + (int64_t)ci * dev::kQueueStride;
+        const int64_t* ranges = t->bucket_ranges + 2 * (ci * dev::kSeedBins);
+        dev::SeededLaunch<Dt> S;
+        S.bins = subdiv_args<Dt>(t);
+        S.bins.n_slots = cnt;
+        S.bins.int_list = t->bucket_list;
+        S.bins.eps0 = eps_of<Dt>(t);
+        S.bins.stack_cap = t->seed_stack_cap;
+        S.bins.int_cost = cost;
+        S.run = dev::BinRun{ranges, t->log2_nsub, t->o.max_depth, t->rerun_list + off, q};
+        S.whole = subdiv_args<Dt>(t);
+        S.whole.n_slots = cnt;
+        S.whole.int_list = t->bucket_list;
+        S.whole.slot_range = ranges + 2 * dev::kMaxSeedK;
+        S.whole.log2_nsub = 6;
+        S.whole.eps0 = eps_of<Dt>(t);
+        S.whole.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, 6, f32);
+        S.whole.stack_cap = 64 * (S.whole.max_depth + 2);
+        S.whole.int_cost = cost;
+        S.rerun = subdiv_args<Dt>(t);
+        S.rerun.eps0 = eps_of<Dt>(t);
+        S.rerun.stack_cap = 64 * (dev::effective_depth(t->o.max_depth, t->log2_nsub, 0, f32) + 2);
+        S.rerun.int_cost = cost;
+        S.root_k = t->root_k;
+        if (t->kind == KIND_ELLIPSE) {
+            e = Launch<Dt, NGL>::seeded_ellipse(L, t->a, t->b, t->o.normalize, S);
+        } else {
+            const ClassRange& r = t->classes[ci];
+            e = Launch<Dt, NGL>::seeded_bspline(L, r.p, r.D, T, S);
         }
     }
-    if (multi && (e = aux_join(c)) != hipSuccess) return e;
+    if (multi) {
+        const hipError_t ej = aux_join(c);
+        if (e == hipSuccess) e = ej;
+    }
+    if (e != hipSuccess) return e;
     t->seeded = true;
     return build_scan(t);
 }
@@ -1091,12 +1103,12 @@ extern "C" int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor) {
     return GCV_OK;
 }
 
-extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int64_t** d_root_off) {
+extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int8_t** d_root_k) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (seeded) *seeded = t->seeded ? 1 : 0;
     if (log2_roots) *log2_roots = t->built_ls;
-    if (d_root_off) *d_root_off = t->root_off;
+    if (d_root_k) *d_root_k = t->root_k;
     return GCV_OK;
 }
 
@@ -1283,7 +1295,7 @@ extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int s
 }
 
 extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
-                               uint64_t* overflow, uint64_t* tasks) {
+                               uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (int rc = capture_guard(t->ctx, "gcv_table_stats")) return rc;
@@ -1297,6 +1309,8 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
         s[dev::kStatQuad] += row[dev::kStatQuad];
         s[dev::kStatDeepRoots] += row[dev::kStatDeepRoots];
         s[dev::kStatTasks] += row[dev::kStatTasks];
+        s[dev::kStatDepthCap] += row[dev::kStatDepthCap];
+        s[dev::kStatRerun] += row[dev::kStatRerun];
         s[dev::kStatMaxStack] = std::max(s[dev::kStatMaxStack], row[dev::kStatMaxStack]);
         s[dev::kStatOverflow] = std::max(s[dev::kStatOverflow], row[dev::kStatOverflow]);
     }
@@ -1305,6 +1319,8 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
     if (max_stack) *max_stack = s[dev::kStatMaxStack];
     if (overflow) *overflow = s[dev::kStatOverflow];
     if (tasks) *tasks = s[dev::kStatTasks];
+    if (depth_capped) *depth_capped = s[dev::kStatDepthCap];
+    if (reruns) *reruns = s[dev::kStatRerun];
     return GCV_OK;
 }
 
@@ -1339,7 +1355,7 @@ static int sample_launch(gcv_table* t, const int64_t* d_soff, int64_t n_total, c
     A.int_cum = t->int_cum;
     A.curve_len = t->curve_len;
     A.root_prefix = t->root_prefix;
-    A.root_off = t->seeded ? t->root_off : nullptr;
+    A.root_k = t->seeded ? t->root_k : nullptr;
     A.deep = t->deep;
     A.log2_nsub = t->built_ls;
     A.eps0 = t->o.precision == GCV_FP32 ? (double)eps_of<float>(t) : eps_of<double>(t);

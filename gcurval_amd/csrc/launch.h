@@ -49,6 +49,12 @@ struct Launch {
     // Subdivision of one B-spline class (degree p, dimension D).
     static hipError_t subdivide_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
                                         dev::SubdivArgs<Dt> A);
+    // Seeded subdivision of the bucketed intervals of one class: the bins
+    // k = 0..5, bin 6, then the re-run of tiles that outgrew the small stack.
+    static hipError_t seeded_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
+                                     const dev::SeededLaunch<Dt>& S);
+    static hipError_t seeded_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
+                                     const dev::SeededLaunch<Dt>& S);
     // blk_curve: curve of the first sample of every kSampleBlock samples (k_block_curve).
     static hipError_t sample_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
                                      const dev::SampleArgs& A, const int32_t* blk_curve);
@@ -58,6 +64,7 @@ struct Launch {
                                      const dev::SampleArgs& A, const int32_t* blk_curve);
     // LDS per block and waves per block of the subdivision kernel.
     static void subdivide_shape(int stack_cap, int want_wpb, int* waves_per_block, size_t* lds_bytes);
+
 };
 
 }  // namespace rt

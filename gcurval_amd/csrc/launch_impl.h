@@ -1,6 +1,8 @@
 // Definitions of the launchers; included once per instantiation unit.
 #pragma once
 
+#include <atomic>
+
 #include "launch.h"
 
 namespace gcurval {
@@ -19,26 +21,37 @@ void Launch<Dt, NGL>::subdivide_shape(int stack_cap, int want_wpb, int* waves_pe
 
 constexpr int kMaxLdsKey = 257;  // LDS sizes in 256-byte units, up to 64 KiB per block
 
-template <typename Dt, int NGL, class Ev>
-static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name) {
-    if (A.n_slots <= 0) return hipSuccess;
+// Resident blocks per CU of `kernel` for (waves per block, LDS bytes),
+// queried once per shape and kernel; the cache is shared by every context and
+// host thread (relaxed atomics: a race only repeats the query).
+template <class K>
+static hipError_t resident_blocks(K kernel, std::atomic<int> (&cache)[4][kMaxLdsKey], int wpb, size_t lds,
+                                  int* per_cu) {
+    const int lds_key = (lds & 255) ? kMaxLdsKey : (int)(lds >> 8);  // sizes here are multiples of 512
+    const bool cached = wpb >= 1 && wpb <= 4 && lds_key < kMaxLdsKey;
+    int n = cached ? cache[wpb - 1][lds_key].load(std::memory_order_relaxed) : 0;
+    if (n == 0) {
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 64 * wpb, lds);
+        if (e != hipSuccess) return e;
+        if (n < 1) n = 1;
+        if (cached) cache[wpb - 1][lds_key].store(n, std::memory_order_relaxed);
+    }
+    *per_cu = n;
+    return hipSuccess;
+}
+
+template <typename Dt, int NGL, class Ev, bool kWhole>
+static hipError_t run_subdivide_as(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name) {
     int wpb;
     size_t lds;
     Launch<Dt, NGL>::subdivide_shape(A.stack_cap, c.sub_wpb, &wpb, &lds);
     const int threads = 64 * wpb;
     const int ipw = 64 >> A.log2_nsub;
     const int64_t n_tiles = (A.n_slots + ipw - 1) / ipw;
-    // resident blocks per CU for this LDS size (queried once per size and instantiation)
-    static int cached_per_cu[4][kMaxLdsKey] = {};
-    const int lds_key = (lds & 255) ? kMaxLdsKey : (int)(lds >> 8);  // sizes here are multiples of 512
-    int per_cu = (wpb <= 4 && lds_key < kMaxLdsKey) ? cached_per_cu[wpb - 1][lds_key] : 0;
-    hipError_t e = hipSuccess;
-    if (per_cu == 0) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subdivide<Dt, NGL, Ev>, threads, lds);
-        if (e != hipSuccess) return e;
-        if (per_cu < 1) per_cu = 1;
-        if (wpb <= 4 && lds_key < kMaxLdsKey) cached_per_cu[wpb - 1][lds_key] = per_cu;
-    }
+    static std::atomic<int> cache[4][kMaxLdsKey] = {};
+    int per_cu = 0;
+    hipError_t e = resident_blocks(k_subdivide<Dt, NGL, Ev, kWhole>, cache, wpb, lds, &per_cu);
+    if (e != hipSuccess) return e;
     // uniform builds launch 4 rounds of resident blocks: tiles differ in
     // refinement work, and blocks that retire early are replaced instead of
     // leaving their SIMDs idle (1M ellipses 0.438 -> 0.412 ms, 4M cubic
@@ -50,10 +63,69 @@ static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivAr
     const int64_t need = (n_tiles + wpb - 1) / wpb;
     if (blocks > need) blocks = need;
     prof_begin(c.owner, name, c.stream);
-    hipLaunchKernelGGL((k_subdivide<Dt, NGL, Ev>), dim3((unsigned)blocks), dim3(threads), lds, c.stream, ev, A);
+    hipLaunchKernelGGL((k_subdivide<Dt, NGL, Ev, kWhole>), dim3((unsigned)blocks), dim3(threads), lds, c.stream, ev,
+                       A);
     e = hipGetLastError();
     prof_end(c.owner, c.stream);
     return e;
+}
+
+template <typename Dt, int NGL, class Ev>
+static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name);
+
+// Seeded subdivision of one class: the bins kernel over bins k = 0..5 (4
+// rounds of resident blocks, as uniform builds), the whole-interval kernel
+// over bin 6, and the re-run kernel (one wave per listed interval; the list
+// is usually empty and its waves return at once).
+template <typename Dt, int NGL, class Ev>
+static hipError_t run_seeded(const LaunchCfg& c, const Ev& ev, const SeededLaunch<Dt>& S, const char* name) {
+    const SubdivArgs<Dt>& A = S.bins;
+    if (A.n_slots <= 0) return hipSuccess;
+    hipError_t e;
+    {
+        int wpb;
+        size_t lds;
+        Launch<Dt, NGL>::subdivide_shape(A.stack_cap, c.sub_wpb, &wpb, &lds);
+        static std::atomic<int> cache[4][kMaxLdsKey] = {};
+        int per_cu = 0;
+        if ((e = resident_blocks(k_subdivide_bins<Dt, NGL, Ev>, cache, wpb, lds, &per_cu)) != hipSuccess) return e;
+        int64_t blocks = (int64_t)c.n_cu * per_cu * 4 * (c.sub_grid_pct > 0 ? c.sub_grid_pct : 100) / 100;
+        const int64_t need = (A.n_slots + wpb - 1) / wpb;  // at most one tile per interval
+        if (blocks > need) blocks = need;
+        if (blocks < 1) blocks = 1;
+        prof_begin(c.owner, name, c.stream);
+        hipLaunchKernelGGL((k_subdivide_bins<Dt, NGL, Ev>), dim3((unsigned)blocks), dim3(64 * wpb), lds, c.stream,
+                           ev, A, S.run);
+        e = hipGetLastError();
+        prof_end(c.owner, c.stream);
+        if (e != hipSuccess) return e;
+    }
+    if ((e = run_subdivide<Dt, NGL>(c, ev, S.whole, name)) != hipSuccess) return e;
+    {
+        const size_t lds = (size_t)subdiv_lds_per_wave<Dt>(S.rerun.stack_cap);
+        const int64_t blocks = std::min<int64_t>((int64_t)c.n_cu, A.n_slots);
+        hipLaunchKernelGGL((k_subdivide_rerun<Dt, NGL, Ev>), dim3((unsigned)blocks), dim3(64), lds, c.stream, ev,
+                           S.rerun, (const int32_t*)S.run.rerun, (const unsigned long long*)S.run.rerun_count,
+                           S.root_k, S.run.ls_plain, S.run.base_depth);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::seeded_ellipse(const LaunchCfg& c, const double* a, const double* b, int normalize,
+                                           const SeededLaunch<Dt>& S) {
+    EllipseEval<Dt> ev{a, b, normalize};
+    return run_seeded<Dt, NGL>(c, ev, S, "k_subdivide_seeded_ellipse");
+}
+
+// Tiles of one whole 64-root interval (the plain method, 64-root bins) take
+// the wave-uniform kernel; other root counts the per-lane one.
+template <typename Dt, int NGL, class Ev>
+static hipError_t run_subdivide(const LaunchCfg& c, const Ev& ev, const SubdivArgs<Dt>& A, const char* name) {
+    if (A.n_slots <= 0) return hipSuccess;
+    if (A.log2_nsub == 6) return run_subdivide_as<Dt, NGL, Ev, true>(c, ev, A, name);
+    return run_subdivide_as<Dt, NGL, Ev, false>(c, ev, A, name);
 }
 
 template <typename Dt, int NGL>
@@ -117,10 +189,36 @@ hipError_t Launch<Dt, NGL>::subdivide_bspline(const LaunchCfg& c, int p, int D, 
 #undef GCV_SUBDIV_CASE
 }
 
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::seeded_bspline(const LaunchCfg& c, int p, int D, const BSplineTableDev& T,
+                                           const SeededLaunch<Dt>& S) {
+#define GCV_SEEDED_CASE(PP, DD) \
+    case 10 * PP + DD:          \
+        return run_seeded<Dt, NGL>(c, MakeEv<Dt, PP, DD>::get(T), S, "k_subdivide_seeded_bspline");
+    switch (bspline_class(p, D)) {
+        GCV_SEEDED_CASE(1, 2)
+        GCV_SEEDED_CASE(1, 3)
+        GCV_SEEDED_CASE(2, 2)
+        GCV_SEEDED_CASE(2, 3)
+        GCV_SEEDED_CASE(3, 2)
+        GCV_SEEDED_CASE(3, 3)
+        GCV_SEEDED_CASE(4, 2)
+        GCV_SEEDED_CASE(4, 3)
+        GCV_SEEDED_CASE(5, 2)
+        GCV_SEEDED_CASE(5, 3)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef GCV_SEEDED_CASE
+}
+
 template <typename Dt, int NGL, bool kSeeded>
 static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const EllipseEval<double>& ev64,
                             const SampleArgs& A, const int32_t* blk_curve) {
-    static int per_cu = 0;
+    // resident blocks per CU, queried once (shared by every context and host
+    // thread; a race only repeats the query)
+    static std::atomic<int> cached{0};
+    int per_cu = cached.load(std::memory_order_relaxed);
     if (per_cu == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -128,6 +226,7 @@ static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const
                 hipSuccess || n < 1)
             n = 1;
         per_cu = n;
+        cached.store(n, std::memory_order_relaxed);
     }
     const int64_t n_tiles = (A.n_total + 63) / 64;
     const int64_t wpb = kSampleBlock / 64;
@@ -154,7 +253,7 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
     prof_begin(c.owner, "k_sample_ellipse");
     // the common modes (uniform or explicit positions, plain or seeded roots)
     // are compiled in, so the kernel has no branch between its loads
-    const bool pos = A.pos != nullptr, seeded = A.root_off != nullptr;
+    const bool pos = A.pos != nullptr, seeded = A.root_k != nullptr;
 #define GCV_ELLIPSE_SAMPLE(P, S)                                                                          \
     hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, StaticMode<P, S>>),       \
                        dim3((unsigned)blocks), dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve)
@@ -221,7 +320,7 @@ hipError_t Launch<Dt, NGL>::sample_bspline(const LaunchCfg& c, int uniform_class
     if (A.n_total <= 0) return hipSuccess;
     const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_bspline");
-    const bool pos = A.pos != nullptr, seeded = A.root_off != nullptr;
+    const bool pos = A.pos != nullptr, seeded = A.root_k != nullptr;
 #define GCV_BSPLINE_SAMPLE(P, S)                                                                        \
     hipLaunchKernelGGL((k_sample_bspline<Dt, NGL, StaticMode<P, S>>), dim3((unsigned)blocks),          \
                        dim3(kSampleBlock), 0, c.stream, T, uniform_class, A, blk_curve)

@@ -379,7 +379,7 @@ public:
         A.int_cum = int_cum_;
         A.curve_len = curve_len_;
         A.root_prefix = root_prefix_;
-        A.root_off = nullptr;
+        A.root_k = nullptr;
         A.deep = deep_;
         A.log2_nsub = kLog2Sub;
         A.eps0 = eps0();
@@ -533,7 +533,6 @@ private:
         A.eps0 = eps0();
         A.max_depth = max_recursive_depth;
         A.root_prefix = root_prefix_;
-        A.root_off = nullptr;
         A.int_len = int_cum_;
         A.curve_len = nullptr;
         A.deep = deep_;
@@ -550,14 +549,15 @@ private:
         const int ipw = 64 >> kLog2Sub;
         const int64_t n_tiles = ((int64_t)n_seg_ + ipw - 1) / ipw;
         int per_cu = 0;
-        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_subdivide<Dt, 3, Eval>, 64 * wpb, lds) ==
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_subdivide<Dt, 3, Eval, kLog2Sub == 6>,
+                                                           64 * wpb, lds) ==
                       hipSuccess
                   ? 0
                   : 3,
               "occupancy query");
         if (per_cu < 1) per_cu = 1;
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t)n_cu_ * per_cu, (n_tiles + wpb - 1) / wpb));
-        hipLaunchKernelGGL((dev::k_subdivide<Dt, 3, Eval>), dim3((unsigned)blocks), dim3(64 * wpb), lds, s,
+        hipLaunchKernelGGL((dev::k_subdivide<Dt, 3, Eval, kLog2Sub == 6>), dim3((unsigned)blocks), dim3(64 * wpb), lds, s,
                            eval((unsigned)stacks), A);
         detail::launch_check("k_subdivide");
     }

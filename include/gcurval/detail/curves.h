@@ -15,6 +15,10 @@
 #include <type_traits>
 #include <utility>
 
+#if defined(__HIPCC__)
+#include "device.h"
+#endif
+
 #ifndef GCURVAL_HD
 #if defined(__HIPCC__)
 #define GCURVAL_HD __host__ __device__
@@ -236,11 +240,22 @@ struct EllipseEval {
     }
     __device__ __forceinline__ Ctx load(int i) const { return load(i, rcp_f64(scale(i))); }
     // with the reciprocal of scale(i) already at hand
-    __device__ __forceinline__ Ctx load(int i, double inv_scale) const {
-        const double A = kTwoPi * a[i] * inv_scale, B = kTwoPi * b[i] * inv_scale;
+    __device__ __forceinline__ Ctx load(int i, double inv_scale) const { return ctx_of(a[i], b[i], inv_scale); }
+    __device__ __forceinline__ static Ctx ctx_of(double ai, double bi, double inv_scale) {
+        const double A = kTwoPi * ai * inv_scale, B = kTwoPi * bi * inv_scale;
         const double A2 = A * A, B2 = B * B;
         const bool sine = A2 >= B2;
         return Ctx{Dt(sine ? B2 : A2), Dt(sine ? A2 - B2 : B2 - A2), Dt(sine ? 0.0 : -0.5)};
+    }
+    __device__ __forceinline__ double scale_of(double ai, double bi) const {
+        const double s = fmax(fabs(ai), fabs(bi));
+        return normalize && s > 0.0 ? s : 1.0;
+    }
+    // context and scale of wave-uniform interval i through scalar loads
+    __device__ __forceinline__ void load_s(int i, Ctx& c, double& sc) const {
+        const double ai = dev::load_s(a + i), bi = dev::load_s(b + i);
+        sc = scale_of(ai, bi);
+        c = ctx_of(ai, bi, rcp_f64(sc));
     }
     __device__ __forceinline__ float speed(const Ctx& c, float x) const {
         return __builtin_amdgcn_sqrtf(fmaf(float(c.D), sin2_2pi(x, float(c.shift)), float(c.base)));
@@ -288,6 +303,13 @@ struct BSplineEval {
 #pragma unroll
         for (int j = 0; j < P * D; j++) k.c[j] = Dt(src[j]);
         return k;
+    }
+    // context and scale of wave-uniform interval i: the coefficients through
+    // vector loads (a block of up to 15 values in scalar registers, twice
+    // over for the prefetch, spilled them), the scale through a scalar load
+    __device__ __forceinline__ void load_s(int i, Ctx& k, double& sc) const {
+        k = load(i);
+        sc = int_scale ? dev::load_s(int_scale + i) : 1.0;
     }
     // coefficients are stored pre-normalised
     __device__ __forceinline__ Ctx load(int i, double) const { return load(i); }

@@ -97,7 +97,26 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Load through the scalar data cache (s_load) of read-only data at a
+// wave-uniform address: a constant-address-space view of the same memory.
+// Scalar loads are counted apart from vector memory operations, so waiting
+// for them does not wait for the wave's outstanding vector stores.
+template <typename T>
+__device__ __forceinline__ T load_s(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
+// A wave-uniform copy of v (the first active lane's), kept in a scalar register.
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ double shfl_up_d(double v, int d) { return __shfl_up(v, (unsigned)d, 64); }
+// Lane `src`'s v, for a wave-uniform src (v_readlane, no LDS round trip).
+__device__ __forceinline__ double readlane_d(double v, int src) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
 // Lane `src`'s copy of a trivially copyable value, one 32-bit word at a time.

@@ -32,7 +32,7 @@ namespace dev {
 // Subdivision
 // ---------------------------------------------------------------------------
 enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverflow = 3, kStatTasks = 4,
-                kStatCount = 8 };
+                kStatDepthCap = 5, kStatRerun = 6, kStatCount = 8 };
 // Build statistics live in kStatSlots rows of kStatStride counters (one
 // 128-byte line each); block b adds into row b % kStatSlots, and readers sum
 // (or take the maximum over) the rows. Spreading the rows keeps thousands of
@@ -40,17 +40,22 @@ enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverf
 constexpr int kStatSlots = 32;
 constexpr int kStatStride = 16;
 constexpr int kStatWords = kStatSlots * kStatStride;
+// Counters of the seeded build after the statistics rows, zeroed with them
+// at the start of every build: per evaluator class the number of intervals
+// whose tile outgrew the small task stack (re-run with the full one).
+constexpr int kQueueStride = 2;
+constexpr int kQueueClasses = 16;
+constexpr int kStatBufWords = kStatWords + kQueueStride * kQueueClasses;
 
 template <typename Dt>
 struct SubdivArgs {
     int64_t n_slots;            // interval slots (n_int, or int_list length)
     const int32_t* int_list;    // optional slot -> interval map
     const int64_t* slot_range;  // optional device {offset, count} into int_list; overrides n_slots
-    int log2_nsub;              // roots per interval = 2^log2_nsub (<= 64)
+    int log2_nsub;              // roots per interval = 2^log2_nsub (<= 64); root block of interval i at i << log2_nsub
     Dt eps0;                    // level-0 tolerance per root (tol / n_sub)
     int max_depth;
     double* root_prefix;        // inclusive prefix of root lengths in each interval (normalised units)
-    const int64_t* root_off;    // optional root base per interval (default i << log2_nsub)
     double* int_len;            // [n_int] interval length (curve units)
     double* curve_len;          // optional: one interval per curve -> also curve length
     uint64_t* deep;             // [n_int] bit k set if root k was refined
@@ -155,45 +160,238 @@ __device__ __forceinline__ void task_bounds(int kr, int l, uint32_t idx, int ls,
 template <typename Dt>
 __device__ __forceinline__ Dt dabs(Dt v) { return v < Dt(0) ? -v : v; }
 
+__device__ __forceinline__ int clamp_k_seed(int k) { return k < 0 ? 0 : (k > 6 ? 6 : k); }
+
+__device__ __forceinline__ int64_t readfirstlane64(int64_t v) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)v >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
+// A wave's refinement-task stack in LDS: keys and parent lengths.
+template <typename Dt>
+struct TaskStack {
+    typedef typename KeyOf<Dt>::type Key;
+    Key* key;
+    Dt* par;
+    int cap;
+};
+
+// Per-wave counters of a subdivision launch, added into the statistics rows
+// when the wave ends.
+struct WaveCounters {
+    unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0, n_cap = 0, n_rerun = 0;
+    int max_stack = 0, overflow = 0;
+    __device__ __forceinline__ void flush(unsigned long long* stats) const {
+        if (lane_id() != 0) return;
+        unsigned long long* st = stats + (blockIdx.x & (kStatSlots - 1)) * kStatStride;
+        if (n_quad) atomicAdd(&st[kStatQuad], n_quad);
+        if (n_deep) atomicAdd(&st[kStatDeepRoots], n_deep);
+        if (n_tasks) atomicAdd(&st[kStatTasks], n_tasks);
+        if (n_cap) atomicAdd(&st[kStatDepthCap], n_cap);
+        if (n_rerun) atomicAdd(&st[kStatRerun], n_rerun);
+        if (max_stack) atomicMax(&st[kStatMaxStack], (unsigned long long)max_stack);
+        if (overflow) atomicMax(&st[kStatOverflow], 1ull);
+    }
+};
+
+// The wave's LDS of a subdivision kernel: per-root FP64 accumulators, the
+// scan slots, per-root task counters, then the stack.
+template <typename Dt>
+struct SubdivLds {
+    double* acc;
+    unsigned char* scan;
+    int* cnt;
+    TaskStack<Dt> stack;
+    __device__ __forceinline__ SubdivLds(unsigned char* wbase, const SubdivArgs<Dt>& A) {
+        typedef typename KeyOf<Dt>::type Key;
+        acc = (double*)wbase;
+        scan = wbase + 64 * sizeof(double);
+        unsigned char* rest = scan + kScanSlots * sizeof(double);
+        cnt = (int*)rest;
+        stack.key = (Key*)(rest + 64 * sizeof(int));
+        stack.par = (Dt*)(rest + 64 * sizeof(int) + A.stack_cap * sizeof(Key));
+        stack.cap = A.stack_cap;
+    }
+};
+
+// One wave tile: 64 roots, 2^ls per interval (64 / 2^ls intervals; kWhole:
+// one interval, ls = 6). Each lane evaluates its root's whole (P) and halves
+// (L, R) and tests |L + R - P| > eps0; failing roots are refined through the
+// task stack (pop the top <= 64 tasks, push 2 children per failing task at
+// ballot / mbcnt offsets, leaves add into per-root FP64 accumulators); then
+// the FP64 in-interval prefix of the root lengths, the interval length and
+// the refined-root mask are stored. `ctx` is lane r's interval context for
+// root r; `rb` the lane's root block, root k of it at rb + k. Returns true
+// (wave-uniform) if the refinement outgrew the stack: the tile's results are
+// then incomplete and the caller has it re-run with a larger stack.
+template <typename Dt, int NGL, class Ev, bool kWhole>
+__device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt>& A, const typename Ev::Ctx& ctx,
+                                               int i, bool valid, int64_t rb, double scale_i, int ls,
+                                               int max_depth, const SubdivLds<Dt>& W, const LdsSegScan& scan,
+                                               WaveCounters& wc) {
+    typedef typename KeyOf<Dt>::type Key;
+    const int lane = lane_id();
+    const int nsub = 1 << ls;
+    const int ipw_log2 = 6 - ls;
+    const int k = lane & (nsub - 1);
+    const TaskStack<Dt>& stk = W.stack;
+
+    Dt x0, xm, x1;
+    task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
+    const Dt P = quad<NGL>(ev, ctx, x0, x1);
+    const Dt L = quad<NGL>(ev, ctx, x0, xm);
+    const Dt R = quad<NGL>(ev, ctx, xm, x1);
+    Dt len = L + R;
+    const bool fail = valid && max_depth > 0 && dabs(len - P) > A.eps0;
+    wc.n_quad += 3ull * popc64(ballot(valid));
+    const uint64_t fmask = ballot(fail);
+    bool overflowed = false;
+
+    if (fmask) {
+        wc.n_deep += popc64(fmask);
+        W.acc[lane] = 0.0;
+        W.cnt[lane] = 0;
+        const int pos = 2 * mbcnt(fmask);
+        if (fail) {  // <= 128 entries: always in LDS (stack_cap >= 128)
+            stk.key[pos] = make_key<Key>(lane, 1, 0u);
+            stk.par[pos] = L;
+            stk.key[pos + 1] = make_key<Key>(lane, 1, 1u);
+            stk.par[pos + 1] = R;
+        }
+        int size = 2 * popc64(fmask);
+        if (size > wc.max_stack) wc.max_stack = size;
+        while (size > 0) {
+            const int n = size < 64 ? size : 64;
+            const bool take = lane < n;
+            Key kk = 0;
+            Dt par = Dt(0);
+            if (take) {
+                kk = stk.key[size - 1 - lane];
+                par = stk.par[size - 1 - lane];
+            }
+            size -= n;
+            const int r = key_root_of(kk);
+            const int l = key_level_of(kk);
+            const uint32_t idx = key_idx_of(kk);
+            // lane r holds the context of root r's interval
+            const typename Ev::Ctx cr = (kWhole || ipw_log2 == 0) ? ctx : shfl_pod(ctx, r);
+            Dt a0, am, a1;
+            task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
+            const Dt L2 = quad<NGL>(ev, cr, a0, am);
+            const Dt R2 = quad<NGL>(ev, cr, am, a1);
+            const Dt lr = L2 + R2;
+            const bool over = dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
+            bool f2 = take && l < max_depth && over;
+            if (take && l >= max_depth && over) wc.n_cap++;  // a leaf that still fails: the depth cap
+            const uint64_t tmask = ballot(take);
+            wc.n_quad += 2ull * popc64(tmask);
+            wc.n_tasks += popc64(tmask);
+            if (A.int_cost && take) atomicAdd(&W.cnt[r], 1);
+            uint64_t m2 = ballot(f2);
+            if (size + 2 * popc64(m2) > stk.cap) {
+                // The children do not fit: the tile stops refining (its
+                // results are marked incomplete) and never writes past the
+                // stack. A stack of 64 (max_depth + 1) entries always fits.
+                overflowed = true;
+                f2 = false;
+                m2 = 0;
+            }
+            if (f2) {
+                const int p2 = size + 2 * mbcnt(m2);
+                stk.key[p2] = make_key<Key>(r, l + 1, 2u * idx);
+                stk.par[p2] = L2;
+                stk.key[p2 + 1] = make_key<Key>(r, l + 1, 2u * idx + 1u);
+                stk.par[p2 + 1] = R2;
+            } else if (take) {
+                atomicAdd(&W.acc[r], (double)lr);
+            }
+            size += 2 * popc64(m2);
+            if (size > wc.max_stack) wc.max_stack = size;
+        }
+        if (fail) len = (Dt)W.acc[lane];
+    }
+
+    // in-interval inclusive prefix over the n_sub roots (FP64)
+    const double pre = scan(valid ? (double)len : 0.0);
+    double cost = 0.0;
+    if (A.int_cost) cost = scan(valid ? 3.0 + (fail ? 2.0 * W.cnt[lane] : 0.0) : 0.0);
+    // Stores without branches: every lane stores, a lane with nothing of
+    // its own to write repeats the store of a lane that has (same address,
+    // same value). A fixed number of vector memory operations per tile
+    // lets the compiler wait for the prefetched loads alone (it waits for
+    // every outstanding operation where the count depends on a branch).
+    // Interval values come from the interval's last root; a lane past the
+    // end of the slot list takes lane 0's interval (always valid).
+    const int own = valid ? lane : 0;                  // lane whose root prefix this lane stores
+    const int last = (own & ~(nsub - 1)) + nsub - 1;   // last root of that lane's interval
+    const double pre_o = kWhole ? pre : shfl_d(pre, own);
+    const int64_t rb_o = kWhole ? rb : (int64_t)__shfl((long long)rb, own, 64);
+    A.root_prefix[rb_o + (own & (nsub - 1))] = pre_o;
+    const double pre_l = kWhole ? readlane_d(pre, 63) : shfl_d(pre, last);
+    const double cost_l = kWhole ? readlane_d(cost, 63) : shfl_d(cost, last);
+    const int i_l = kWhole ? i : __shfl(i, last, 64);
+    const double L_int = pre_l * (kWhole ? scale_i : shfl_d(scale_i, last));
+    A.int_len[i_l] = L_int;
+    // optional outputs: a duplicate store into the same word of int_len
+    // (curve lengths) or into the low word of deep (quadrature counts,
+    // overwritten below). deep is written as two 32-bit words, a type the
+    // count's store may alias, so the compiler keeps the two stores in order.
+    double* const cl = A.curve_len ? A.curve_len : A.int_len;
+    cl[i_l] = L_int;
+    uint32_t* const dw = (uint32_t*)(A.deep + i_l);
+    int32_t* const ic = A.int_cost ? A.int_cost + i_l : (int32_t*)dw;
+    *ic = (int32_t)cost_l;
+    const uint64_t bits = (ls == 6) ? fmask : ((fmask >> ((last >> ls) << ls)) & ((1ull << nsub) - 1ull));
+    dw[0] = (uint32_t)bits;
+    dw[1] = (uint32_t)(bits >> 32);
+    return overflowed;
+}
+
+// Evaluators that can load the context and scale of a wave-uniform interval
+// through scalar loads (load_s(i, ctx, scale)).
+template <class Ev, class = void>
+struct HasScalarLoad : std::false_type {};
+template <class Ev>
+struct HasScalarLoad<Ev, decltype((void)std::declval<const Ev&>().load_s(0, std::declval<typename Ev::Ctx&>(),
+                                                                        std::declval<double&>()))>
+    : std::true_type {};
+
 // The wave-local stack holds at most 64 (max_depth + 1) tasks: each pop takes
 // the top 64 entries and pushes their children on top, so the blocks left
 // below the top hold <= 64 entries each and have strictly increasing minimum
 // levels (1 .. max_depth), and the top block holds <= 128.
-template <typename Dt, int NGL, class Ev>
+//
+// Tiles of 2^log2_nsub roots per interval over the slots [0, n_slots) (of
+// int_list when given), each wave striding over tiles. kWhole: every tile is
+// one whole interval of 64 roots (log2_nsub = 6), so the interval, its
+// context, scale and root base are wave-uniform: they come through scalar
+// loads where the evaluator has them, and the tile loop issues no vector
+// loads but the coefficients' (B-splines). Otherwise a tile holds
+// 64 / 2^log2_nsub intervals and each lane loads its own.
+template <typename Dt, int NGL, class Ev, bool kWhole>
 __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
-    typedef typename KeyOf<Dt>::type Key;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = lane_id();
-    const int wib = (int)(threadIdx.x >> 6);
+    const int wib = uniform_int((int)(threadIdx.x >> 6));
     const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    unsigned char* wbase = smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap);
-    double* s_acc = (double*)wbase;
-    unsigned char* const s_scan = wbase + 64 * sizeof(double);
-    unsigned char* const s_rest = s_scan + kScanSlots * sizeof(double);
-    int* s_cnt = (int*)s_rest;
-    Key* tkey = (Key*)(s_rest + 64 * sizeof(int));
-    Dt* tpar = (Dt*)(s_rest + 64 * sizeof(int) + A.stack_cap * sizeof(Key));
+    const SubdivLds<Dt> W(smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap), A);
     const int32_t* list = A.int_list;
     int64_t n_slots = A.n_slots;
     if (A.slot_range) {
-        list = A.int_list + A.slot_range[0];
-        n_slots = A.slot_range[1];
+        list = A.int_list + load_s(A.slot_range);
+        n_slots = load_s(A.slot_range + 1);
     }
-    const int ls = A.log2_nsub;
-    const int nsub = 1 << ls;
+    const int ls = kWhole ? 6 : A.log2_nsub;
     const int ipw_log2 = 6 - ls;  // interval slots per wave tile (log2)
     const int jslot = lane >> ls;
-    const int k = lane & (nsub - 1);
     const int64_t n_tiles = (n_slots + (1 << ipw_log2) - 1) >> ipw_log2;
-    const LdsSegScan scan(s_scan, ls);
-
-    unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0;
-    int max_stack = 0, overflow = 0;
+    const LdsSegScan scan(W.scan, ls);
+    WaveCounters wc;
 
     // Software pipeline over this wave's tiles: the interval id of tile t + 2
     // and the evaluator context, root base and scale of tile t + 1 are in
-    // flight while tile t computes, so the dependent global loads (list ->
+    // flight while tile t computes, so the dependent loads (list ->
     // coefficients) overlap the quadratures instead of stalling every tile.
     // (one unconditional-address load per form: a conditional load whose
     // register another path writes directly makes the compiler wait for every
@@ -203,119 +401,29 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         ok = slot < n_slots;
         const int64_t s = ok ? slot : 0;
         int r = (int)s;
-        if (list) r = list[s];
+        if (list) r = kWhole ? load_s(list + s) : list[s];
         return r;
     };
-    auto root_base = [&](int iv) -> int64_t {
-        int64_t r = (int64_t)iv << ls;
-        if (A.root_off) r = A.root_off[iv];
-        return r;
-    };
-    // A stage: what tile t needs from memory (interval, context, root base, scale).
+    // A stage: what tile t needs from memory (interval, context, scale).
     struct Stage {
         typename Ev::Ctx ctx;
-        int64_t rb;
         double sc;
         int i;
         bool v;
     };
     // unconditional loads: past the last tile iv is still an interval of the
     // list (its first slot), so they stay in range and are simply unused
-    auto fetch = [&](int iv, bool ok) -> Stage { return Stage{ev.load(iv), root_base(iv), ev.scale(iv), iv, ok}; };
-
-    auto tile_body = [&](const Stage& cur) __attribute__((always_inline)) {
-        const bool valid = cur.v;
-        const int i = cur.i;
-        const typename Ev::Ctx& ctx = cur.ctx;
-        const int64_t rbase = cur.rb;
-        const double scale_i = cur.sc;
-
-        Dt x0, xm, x1;
-        task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
-        const Dt P = quad<NGL>(ev, ctx, x0, x1);
-        const Dt L = quad<NGL>(ev, ctx, x0, xm);
-        const Dt R = quad<NGL>(ev, ctx, xm, x1);
-        Dt len = L + R;
-        const bool fail = valid && A.max_depth > 0 && dabs(len - P) > A.eps0;
-        n_quad += 3ull * popc64(ballot(valid));
-        const uint64_t fmask = ballot(fail);
-
-        if (fmask) {
-            n_deep += popc64(fmask);
-            s_acc[lane] = 0.0;
-            s_cnt[lane] = 0;
-            const int pos = 2 * mbcnt(fmask);
-            if (fail) {
-                tkey[pos] = make_key<Key>(lane, 1, 0u);
-                tpar[pos] = L;
-                tkey[pos + 1] = make_key<Key>(lane, 1, 1u);
-                tpar[pos + 1] = R;
-            }
-            int size = 2 * popc64(fmask);
-            if (size > max_stack) max_stack = size;
-            while (size > 0) {
-                const int n = size < 64 ? size : 64;
-                const bool take = lane < n;
-                Key kk = 0;
-                Dt par = Dt(0);
-                if (take) {
-                    kk = tkey[size - 1 - lane];
-                    par = tpar[size - 1 - lane];
-                }
-                size -= n;
-                const int r = key_root_of(kk);
-                const int l = key_level_of(kk);
-                const uint32_t idx = key_idx_of(kk);
-                // lane r holds the context of root r's interval
-                const typename Ev::Ctx cr = (ipw_log2 == 0) ? ctx : shfl_pod(ctx, r);
-                Dt a0, am, a1;
-                task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
-                const Dt L2 = quad<NGL>(ev, cr, a0, am);
-                const Dt R2 = quad<NGL>(ev, cr, am, a1);
-                const Dt lr = L2 + R2;
-                bool f2 = take && l < A.max_depth && dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
-                const uint64_t tmask = ballot(take);
-                n_quad += 2ull * popc64(tmask);
-                n_tasks += popc64(tmask);
-                if (A.int_cost && take) atomicAdd(&s_cnt[r], 1);
-                uint64_t m2 = ballot(f2);
-                if (size + 2 * popc64(m2) > A.stack_cap) {
-                    // Unreachable while stack_cap >= 64 (max_depth + 1); a bad
-                    // capacity ends as flagged leaves, never as an out-of-bounds write.
-                    overflow = 1;
-                    f2 = false;
-                    m2 = 0;
-                }
-                if (f2) {
-                    const int p2 = size + 2 * mbcnt(m2);
-                    tkey[p2] = make_key<Key>(r, l + 1, 2u * idx);
-                    tpar[p2] = L2;
-                    tkey[p2 + 1] = make_key<Key>(r, l + 1, 2u * idx + 1u);
-                    tpar[p2 + 1] = R2;
-                } else if (take) {
-                    atomicAdd(&s_acc[r], (double)lr);
-                }
-                size += 2 * popc64(m2);
-                if (size > max_stack) max_stack = size;
-            }
-            if (fail) len = (Dt)s_acc[lane];
+    auto fetch = [&](int iv, bool ok) -> Stage {
+        Stage st;
+        if constexpr (kWhole && HasScalarLoad<Ev>::value) {
+            ev.load_s(iv, st.ctx, st.sc);
+        } else {
+            st.ctx = ev.load(iv);
+            st.sc = ev.scale(iv);
         }
-
-        // in-interval inclusive prefix over the n_sub roots (FP64)
-        const double pre = scan(valid ? (double)len : 0.0);
-        double cost = 0.0;
-        if (A.int_cost) cost = scan(valid ? 3.0 + (fail ? 2.0 * s_cnt[lane] : 0.0) : 0.0);
-        if (valid) {
-            A.root_prefix[rbase + k] = pre;
-            if (k == nsub - 1) {
-                if (A.int_cost) A.int_cost[i] = (int32_t)cost;
-                const double L_int = pre * scale_i;
-                A.int_len[i] = L_int;
-                if (A.curve_len) A.curve_len[i] = L_int;
-                const uint64_t bits = (ls == 6) ? fmask : ((fmask >> (jslot << ls)) & ((1ull << nsub) - 1ull));
-                A.deep[i] = bits;
-            }
-        }
+        st.i = iv;
+        st.v = ok;
+        return st;
     };
 
     // one copy of the tile body: results must not depend on which tiles a
@@ -332,17 +440,163 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
         const Stage nxt = fetch(i_nxt, v_nxt);
         i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
-        tile_body(cur);
+        // the stack holds 64 (max_depth + 2) entries here, so it cannot overflow
+        if (subdivide_tile<Dt, NGL, Ev, kWhole>(ev, A, cur.ctx, cur.i, cur.v, (int64_t)cur.i << ls, cur.sc, ls,
+                                                A.max_depth, W, scan, wc))
+            wc.overflow = 1;
         cur = nxt;
     }
-    if (lane == 0) {
-        unsigned long long* st = A.stats + (blockIdx.x & (kStatSlots - 1)) * kStatStride;
-        if (n_quad) atomicAdd(&st[kStatQuad], n_quad);
-        if (n_deep) atomicAdd(&st[kStatDeepRoots], n_deep);
-        if (n_tasks) atomicAdd(&st[kStatTasks], n_tasks);
-        if (max_stack) atomicMax(&st[kStatMaxStack], (unsigned long long)max_stack);
-        if (overflow) atomicMax(&st[kStatOverflow], 1ull);
+    wc.flush(A.stats);
+}
+
+// ---------------------------------------------------------------------------
+// Seeded subdivision: interval i starts from 2^k_i roots (k_i in root_k);
+// its root block is always i << 6 (the table keeps 64 root slots per
+// interval). The bucketing sorts a class's intervals by k; k_subdivide_bins
+// runs the class's bins k = 0..5 as one tile sequence (bin k's tiles hold
+// 64 / 2^k intervals, the last one possibly partial), and the whole-interval
+// kernel runs bin 6. The bins kernel keeps a small task stack, so more waves
+// fit in LDS; a tile whose refinement outgrows it lists its intervals, which
+// k_subdivide_rerun then runs again with the full stack.
+// ---------------------------------------------------------------------------
+constexpr int kSeedPartBins = 6;  // k = 0..5
+
+struct BinRun {
+    const int64_t* ranges;  // [2 * kSeedPartBins] {first slot, count} of this class's bins k = 0..5 (device)
+    int ls_plain;           // log2 roots of the plain build (depth caps)
+    int base_depth;         // depth cap of the plain build
+    int32_t* rerun;         // out: intervals of tiles that outgrew the stack
+    unsigned long long* rerun_count;
+};
+
+// Bin and slot of lane's interval in tile tl of the bin sequence (tile
+// starts ts[k] per bin, wave-uniform).
+struct BinTile {
+    int k;
+    int64_t slot;  // the lane's slot (lane 0's for a lane past the bin's end)
+    bool ok;
+};
+
+template <typename Dt, int NGL, class Ev>
+__global__ __launch_bounds__(256) void k_subdivide_bins(Ev ev, SubdivArgs<Dt> A, BinRun B) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = lane_id();
+    const int wib = uniform_int((int)(threadIdx.x >> 6));
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const SubdivLds<Dt> W(smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap), A);
+    constexpr bool kF32 = sizeof(Dt) == 4;
+    int64_t first[kSeedPartBins], count[kSeedPartBins], ts[kSeedPartBins + 1];
+    ts[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kSeedPartBins; k++) {
+        first[k] = load_s(B.ranges + 2 * k);
+        count[k] = load_s(B.ranges + 2 * k + 1);
+        ts[k + 1] = ts[k] + ((count[k] + (64 >> k) - 1) >> (6 - k));
     }
+    const int64_t n_tiles = ts[kSeedPartBins];
+    WaveCounters wc;
+
+    auto locate = [&](int64_t tl) -> BinTile {
+        int k = 0;
+#pragma unroll
+        for (int j = 1; j < kSeedPartBins; j++)
+            if (tl >= ts[j]) k = j;
+        const int64_t base = (tl - ts[k]) << (6 - k);  // first interval of the tile within its bin
+        const int64_t s = base + (lane >> k);
+        BinTile t;
+        t.k = k;
+        t.ok = s < count[k];
+        t.slot = first[k] + (t.ok ? s : base);
+        return t;
+    };
+    // software pipeline as in k_subdivide: interval ids of tile t + 2,
+    // contexts of tile t + 1 in flight while tile t computes
+    struct Stage {
+        typename Ev::Ctx ctx;
+        double sc;
+        int i;
+        int k;
+        bool v;
+    };
+    auto fetch = [&](int iv, const BinTile& t) -> Stage {
+        Stage st;
+        st.ctx = ev.load(iv);
+        st.sc = ev.scale(iv);
+        st.i = iv;
+        st.k = t.k;
+        st.v = t.ok;
+        return st;
+    };
+    Stage cur{};
+    BinTile t_nxt{};
+    int i_nxt = 0;
+    if (gwave < n_tiles) {
+        const BinTile t0 = locate(gwave);
+        cur = fetch(A.int_list[t0.slot], t0);
+        t_nxt = locate(gwave + nwaves < n_tiles ? gwave + nwaves : gwave);
+        i_nxt = A.int_list[t_nxt.slot];
+    }
+    for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
+        const Stage nxt = fetch(i_nxt, t_nxt);
+        const int64_t t2 = tile + 2 * nwaves;
+        t_nxt = locate(t2 < n_tiles ? t2 : tile);
+        i_nxt = A.int_list[t_nxt.slot];
+        const int k = cur.k;
+        const LdsSegScan scan(W.scan, k);
+        const bool over = subdivide_tile<Dt, NGL, Ev, false>(ev, A, cur.ctx, cur.i, cur.v, (int64_t)cur.i << 6,
+                                                             cur.sc, k, effective_depth(B.base_depth, B.ls_plain,
+                                                                                        k, kF32),
+                                                             W, scan, wc);
+        if (over) {
+            // list the tile's intervals (one lane each) for the full-stack re-run
+            const bool mine = cur.v && ((lane & ((1 << k) - 1)) == (1 << k) - 1);
+            const uint64_t m = ballot(mine);
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(B.rerun_count, (unsigned long long)popc64(m));
+            base = (unsigned long long)readfirstlane64((int64_t)__shfl((long long)base, 0, 64));
+            if (mine) B.rerun[base + mbcnt(m)] = cur.i;
+            wc.n_rerun++;
+        }
+        cur = nxt;
+    }
+    wc.flush(A.stats);
+}
+
+// Everything one class's seeded subdivision needs: the bins launch (small
+// stack), the whole-interval launch over bin 6, and the re-run launch.
+template <typename Dt>
+struct SeededLaunch {
+    SubdivArgs<Dt> bins;     // int_list = bucket list, n_slots = the class's interval count
+    BinRun run;
+    SubdivArgs<Dt> whole;    // int_list = bucket list, slot_range = bin 6's range
+    SubdivArgs<Dt> rerun;    // full stack
+    const int8_t* root_k;
+};
+
+// The intervals listed by k_subdivide_bins, one wave tile each (root count
+// 2^root_k[i], the lanes past it idle), with the full task stack in LDS.
+template <typename Dt, int NGL, class Ev>
+__global__ __launch_bounds__(64) void k_subdivide_rerun(Ev ev, SubdivArgs<Dt> A, const int32_t* list,
+                                                        const unsigned long long* count, const int8_t* root_k,
+                                                        int ls_plain, int base_depth) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = lane_id();
+    const SubdivLds<Dt> W(smem, A);
+    constexpr bool kF32 = sizeof(Dt) == 4;
+    const int64_t n = (int64_t)load_s(count);
+    WaveCounters wc;
+    for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
+        const int i = load_s(list + j);
+        const int k = load_s(root_k + i);
+        const typename Ev::Ctx ctx = ev.load(i);
+        const double sc = ev.scale(i);
+        const LdsSegScan scan(W.scan, k);
+        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, ctx, i, lane < (1 << k), (int64_t)i << 6, sc, k,
+                                               effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc))
+            wc.overflow = 1;  // unreachable: the stack holds 64 (depth + 2) entries
+    }
+    wc.flush(A.stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -556,7 +810,8 @@ struct SampleArgs {
     const double* int_cum;      // [n_int] cumulative interval lengths (curve units)
     const double* curve_len;    // [n_curve]
     const double* root_prefix;  // inclusive root prefix per interval
-    const int64_t* root_off;    // optional root base per interval (seeded builds); else i << log2_nsub
+    const int8_t* root_k;       // seeded builds: log2 root count per interval, root block at i << 6;
+                                // else (null) 2^log2_nsub roots at i << log2_nsub
     const uint64_t* deep;       // [n_int]
     int log2_nsub;
     double eps0;                // level-0 tolerance per root (same value as the build)
@@ -848,7 +1103,7 @@ __device__ __forceinline__ typename Ev64::Ctx widen_or_load(const Ev64& ev64, co
 // Samples per block of the sampling kernels (blockDim.x).
 constexpr int kSampleBlock = 256;
 
-// Sampling mode: explicit positions (pos) and seeded root offsets (root_off)
+// Sampling mode: explicit positions (pos) and seeded root counts (root_k)
 // read from the arguments at run time, or fixed at compile time so that a
 // kernel has no branch between its loads.
 struct RuntimeMode {
@@ -864,7 +1119,7 @@ __device__ __forceinline__ bool mode_pos(const SampleArgs& A) {
 }
 template <class M>
 __device__ __forceinline__ bool mode_seeded(const SampleArgs& A) {
-    if constexpr (M::kStatic) return M::kSeeded; else return A.root_off != nullptr;
+    if constexpr (M::kStatic) return M::kSeeded; else return A.root_k != nullptr;
 }
 
 // Target arc length of sample g (curve c) in curve units, in two parts: the
@@ -932,11 +1187,6 @@ static __global__ __launch_bounds__(256) void k_block_interval(SampleArgs A, con
     const int c = (int)blk_curve_at(A, blk_curve, b);
     const double s = sample_target(A, c, b * kSampleBlock);
     blk_int[b] = (int32_t)interval_of_target(A, c, s, 0, INT64_MAX);
-}
-
-__device__ __forceinline__ int64_t readfirstlane64(int64_t v) {
-    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)v >> 32)) << 32) |
-                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
 }
 
 // Interval of sample g on curve c (several intervals per curve) and the
@@ -1066,8 +1316,8 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     int ls = A.log2_nsub;
     int64_t ro = (int64_t)i << ls;
     if (mode_seeded<M>(A)) {
-        ro = A.root_off[i];
-        ls = __ffs((int)(A.root_off[i + 1] - ro)) - 1;
+        ro = (int64_t)i << 6;
+        ls = A.root_k[i];
     }
     int off = 0, span = 0;
     double rv[kRootWin / 64];
@@ -1242,8 +1492,8 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 int64_t ro = cn << A.log2_nsub;
                 N.ls = A.log2_nsub;
                 if constexpr (kSeeded) {
-                    ro = A.root_off[cn];
-                    N.ls = __ffs((int)(A.root_off[cn + 1] - ro)) - 1;
+                    ro = cn << 6;
+                    N.ls = A.root_k[cn];
                 }
                 const double v = lane < (1 << N.ls) ? A.root_prefix[ro + lane] : 0.0;
                 const int64_t Ns = N.s1 - N.s0;

@@ -1,8 +1,9 @@
-// Kernels of the seeded table build: each knot interval i starts from
-// 2^k_i roots (k_i in 0..6, predicted per interval) instead of a fixed
-// n_sub. Intervals are bucketed by k (counting sort, wave-aggregated
-// atomics) so every subdivision launch runs a uniform tile shape, and the
-// root storage of interval i starts at root_off[i] (exclusive scan of 2^k_i).
+// Kernels of the seeded table build, where each knot interval i starts from
+// 2^k_i roots (k_i in 0..6, predicted per interval) instead of a fixed n_sub:
+// the safety floor that raises a seed where the speed nearly vanishes, and
+// the bucketing of intervals by (evaluator class, k) (a stable counting
+// sort, wave-aggregated) so every subdivision tile has one shape. Interval
+// i's roots live at i * 64 whatever k_i is, so no root offsets are needed.
 #pragma once
 
 #include "device.h"
@@ -12,12 +13,11 @@ namespace gcurval {
 namespace dev {
 
 constexpr int kMaxSeedK = 6;
-constexpr int kScanItems = 4;  // elements per thread in the scan kernels
+
+__device__ __forceinline__ int clamp_k(int k) { return k < 0 ? 0 : (k > kMaxSeedK ? kMaxSeedK : k); }
 
 constexpr int kSeedBins = kMaxSeedK + 1;  // bins per evaluator class
 constexpr int kMaxBins = 80;               // 10 classes x 7 k values, rounded up
-
-__device__ __forceinline__ int clamp_k(int k) { return k < 0 ? 0 : (k > kMaxSeedK ? kMaxSeedK : k); }
 
 // Bucket of interval i: evaluator class index x 7 + k_i, where k_i is the
 // seed raised to the interval's safety floor.
@@ -26,16 +26,12 @@ struct BinArgs {
     const int8_t* kmin;       // [n_int] safety floor (k_seed_floor)
     const int8_t* cls_idx;    // optional [n_curve] class index of each curve
     const int32_t* int_curve; // interval -> curve (needed with cls_idx)
+    int8_t* root_k;           // out [n_int]: k_i, the log2 root count the build uses
 };
 __device__ __forceinline__ int k_eff(const int8_t* kk, const int8_t* kmin, int64_t i) {
     const int k = clamp_k(kk[i]);
     const int m = kmin ? (int)kmin[i] : 0;
     return k > m ? k : m;
-}
-__device__ __forceinline__ int bin_of(const BinArgs& B, int64_t i) {
-    int b = k_eff(B.kk, B.kmin, i);
-    if (B.cls_idx) b += kSeedBins * B.cls_idx[B.int_curve ? B.int_curve[i] : i];
-    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -165,7 +161,7 @@ __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* s_wave, i
 // of 64); bin counters live in registers, lane j holding bin j (bins 64..127
 // in a second register), and each 64-interval group is ranked with one
 // ballot per distinct bin, so there are no LDS atomics and no block barriers.
-//   k_bucket_hist   bins8[i] = bin of i; per-wave counts, bin-major:
+//   k_bucket_hist   bins8[i] = bin of i, root_k[i] = k_i; per-wave counts, bin-major:
 //                   hist[b * nw + w]
 //   k_bucket_totals block per bin: total count of the bin
 //   k_bucket_rows   block per bin: exclusive scan of the bin's row in place,
@@ -223,7 +219,13 @@ static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B
 #pragma unroll
         for (int u = 0; u < kBucketUnroll; u++) {
             const int64_t i = g0 + 64 * u + lane;
-            bb[u] = i < b1 ? bin_of(B, i) : 0;
+            int k = 0, c = 0;
+            if (i < b1) {
+                k = k_eff(B.kk, B.kmin, i);
+                if (B.cls_idx) c = B.cls_idx[B.int_curve ? B.int_curve[i] : i];
+                B.root_k[i] = (int8_t)k;
+            }
+            bb[u] = kSeedBins * c + k;
         }
 #pragma unroll
         for (int u = 0; u < kBucketUnroll; u++) {
@@ -307,62 +309,6 @@ static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, int n_bi
             if (valid) list[pos] = (int32_t)i;
         }
     }
-}
-
-// Exclusive scan of 2^k_i into root_off[0..n] in three passes: per-block
-// sums of kScanItems * 256 elements, a single-block scan of those sums, and
-// the per-block apply.
-
-static __global__ __launch_bounds__(256) void k_root_block_sums(int64_t n, const int8_t* kk, const int8_t* kmin,
-                                                              int64_t* block_sums) {
-    __shared__ int64_t s_wave[4];
-    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kScanItems;
-    int64_t v = 0;
-    for (int j = 0; j < kScanItems; j++)
-        if (base + j < n) v += (int64_t)1 << k_eff(kk, kmin, base + j);
-    int64_t tot;
-    block_scan_excl(v, s_wave, &tot);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
-}
-
-// In place exclusive scan of nb block sums by one block of 1024 threads.
-static __global__ __launch_bounds__(1024) void k_scan_block_sums(int64_t nb, int64_t* block_sums) {
-    __shared__ int64_t s_wave[16];
-    int64_t carry = 0;
-    for (int64_t base = 0; base < nb; base += blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const int64_t v = i < nb ? block_sums[i] : 0;
-        int64_t tot;
-        const int64_t ex = block_scan_excl(v, s_wave, &tot);
-        if (i < nb) block_sums[i] = carry + ex;
-        carry += tot;
-        __syncthreads();
-    }
-}
-
-static __global__ __launch_bounds__(256) void k_root_apply(int64_t n, const int8_t* kk, const int8_t* kmin,
-                                                           const int64_t* block_offs, int64_t* root_off) {
-    __shared__ int64_t s_wave[4];
-    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kScanItems;
-    int64_t vals[kScanItems];
-    int64_t v = 0;
-    for (int j = 0; j < kScanItems; j++) {
-        vals[j] = base + j < n ? (int64_t)1 << k_eff(kk, kmin, base + j) : 0;
-        v += vals[j];
-    }
-    int64_t tot;
-    int64_t run = block_offs[blockIdx.x] + block_scan_excl(v, s_wave, &tot);
-    for (int j = 0; j < kScanItems; j++) {
-        if (base + j < n) root_off[base + j] = run;
-        run += vals[j];
-        if (base + j == n - 1) root_off[n] = run;
-    }
-}
-
-// Constant k for every interval (uniform-k builds and tests).
-static __global__ void k_fill_i8(int64_t n, int8_t* p, int8_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 }  // namespace dev
