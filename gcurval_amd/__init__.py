@@ -99,14 +99,15 @@ _SIGS = [
     ("gcv_table_build", C.c_int, [C.c_void_p]),
     ("gcv_table_build_uniform", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("gcv_table_build_seeded", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
-    ("gcv_table_root_layout", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p)]),
+    ("gcv_table_root_layout", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p)]),
     ("gcv_table_seed_floor", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_table_destroy", C.c_int, [C.c_void_p]),
     ("gcv_table_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     ("gcv_table_lengths", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gcv_table_device_views", C.c_int,
      [C.c_void_p] + [C.POINTER(C.c_void_p)] * 5),
-    ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 7),
+    ("gcv_table_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint64)] * 9),
     ("gcv_sample", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("gcv_sample_noalloc", C.c_int,
@@ -451,11 +452,11 @@ class ArclengthCalculateGpu:
 
     def root_layout(self):
         """(seeded, log2 roots of a uniform build, device int8 log2 root count
-        per interval of a seeded build, whose root block of interval i starts
-        at i * 64)."""
-        sd, ls, rk = C.c_int(), C.c_int(), C.c_void_p()
-        self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(rk)))
-        return bool(sd.value), ls.value, rk.value
+        per interval of a seeded build, device int64 first root of each
+        interval of a seeded build: roots dense in interval order)."""
+        sd, ls, rk, ro = C.c_int(), C.c_int(), C.c_void_p(), C.c_void_p()
+        self.ctx._check(lib().gcv_table_root_layout(self.h, C.byref(sd), C.byref(ls), C.byref(rk), C.byref(ro)))
+        return bool(sd.value), ls.value, rk.value, ro.value
 
     def createArclengthTableLearned(self, mlp, hedge: bool = False, d_cost: "DeviceArray | None" = None):
         """Seeded build whose per-interval root counts come from the learned predictor."""
@@ -500,11 +501,13 @@ class ArclengthCalculateGpu:
     def stats(self):
         """Build statistics of the last build: quadratures, refined roots,
         deepest task stack, overflow flag (must be 0), refinement tasks,
-        leaves that still failed the error test at the depth cap, and
-        seeded tiles re-run with the full task stack."""
-        v = [C.c_uint64() for _ in range(7)]
+        leaves that still failed the error test at the depth cap, seeded
+        tiles re-run with the full task stack, 64-root wave tiles, and
+        refinement passes over popped tasks."""
+        v = [C.c_uint64() for _ in range(9)]
         self.ctx._check(lib().gcv_table_stats(self.h, *[C.byref(x) for x in v]))
-        keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks", "depth_capped", "reruns")
+        keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks", "depth_capped", "reruns", "tiles",
+                "passes")
         return {k: x.value for k, x in zip(keys, v)}
 
     def device_views(self):
@@ -515,27 +518,31 @@ class ArclengthCalculateGpu:
     def tables(self):
         """Host copies: (int_cum[n_int], root prefixes, deep[n_int], curve_len, int_off). The root
         prefixes are interval after interval, 2^log2_roots per interval (uniform builds) or 2^k_i
-        for interval i (seeded builds, compacted from their 64-slot blocks)."""
+        for interval i (seeded builds)."""
         ic, rp, dp, cl, io = self.device_views()
         L = lib()
         n_int = self.n_intervals
-        seeded, ls, rk = self.root_layout()
-        n_slots = n_int << (6 if seeded else ls)
-        out = [np.empty(n_int, np.float64), np.empty(n_slots, np.float64), np.empty(n_int, np.uint64),
+        n_roots = int(np.sum(1 << self.root_counts().astype(np.int64)))
+        out = [np.empty(n_int, np.float64), np.empty(n_roots, np.float64), np.empty(n_int, np.uint64),
                np.empty(self.n_curve, np.float64), np.empty(self.n_curve + 1, np.int64)]
         for arr, p in zip(out, (ic, rp, dp, cl, io)):
             if arr.nbytes:
                 self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(arr), p, arr.nbytes))
-        if seeded and n_int:
-            k = np.empty(n_int, np.int8)
-            self.ctx._check(L.gcv_copy_d2h(self.ctx.h, _ptr(k), rk, k.nbytes))
-            blocks = out[1].reshape(n_int, 64)
-            out[1] = np.concatenate([blocks[i, :1 << int(k[i])] for i in range(n_int)])
         return tuple(out)
+
+    def root_offsets(self) -> np.ndarray:
+        """First root of each interval in the root-prefix view."""
+        seeded, ls, _, ro = self.root_layout()
+        if not seeded:
+            return np.arange(self.n_intervals, dtype=np.int64) << ls
+        out = np.empty(self.n_intervals, np.int64)
+        if out.nbytes:
+            self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(out), ro, out.nbytes))
+        return out
 
     def root_counts(self) -> np.ndarray:
         """log2 root count per interval of the last build."""
-        seeded, ls, rk = self.root_layout()
+        seeded, ls, rk, _ = self.root_layout()
         k = np.full(self.n_intervals, ls, np.int8)
         if seeded and k.size:
             self.ctx._check(lib().gcv_copy_d2h(self.ctx.h, _ptr(k), rk, k.nbytes))

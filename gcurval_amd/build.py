@@ -29,6 +29,10 @@ HIP_FLAGS = [
     "-fPIC",
     "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core products accumulate in ordinary VGPRs: AGPRs are allocated
+    # on top of a kernel's VGPR peak, so in kernels that mix interval work
+    # and the learned seed they would cost occupancy
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I" + os.path.join(ROOT, "include"),
     "-Wno-unused-result",
 ]

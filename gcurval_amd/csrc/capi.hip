@@ -539,12 +539,13 @@ struct gcv_table {
     // seeded builds: per-interval k, root offsets, (class, k) buckets
     int8_t* seed_k = nullptr;             // [n_int] k used by uniform builds
     int8_t* seed_floor = nullptr;         // [n_int] safety floor of seeded builds
-    uint16_t* feat_x = nullptr;           // [n_int][16] learned-seed input rows (allocated on first use)
     int tune_wpb = 0;                     // subdivision launch shape picked by gcv_table_autotune
     int tune_grid_pct = 100;
     int tune_streams = 1;                 // streams of the seeded build's per-bin launches (default_streams)
     int8_t* cls_idx = nullptr;            // [n_curve] class index (mixed batches)
-    int8_t* root_k = nullptr;             // [n_int] log2 root count of a seeded build (root block at i << 6)
+    int8_t* root_k = nullptr;             // [n_int] log2 root count of a seeded build
+    int64_t* root_off = nullptr;          // [n_int] first root of each interval (seeded builds: dense roots)
+    int64_t* bucket_root_chunk = nullptr; // [kBucketMaxWaves] roots of each bucketing chunk, then their offsets
     int32_t* bucket_list = nullptr;       // [n_int] intervals sorted by (class, k)
     int32_t* bucket_hist = nullptr;       // [kMaxBins * kBucketMaxWaves] per-wave bin counts
     int8_t* bucket_bin = nullptr;         // [n_int] bin of each interval
@@ -620,7 +621,7 @@ static int log2i(int v) {
 // Allocation shared by both families.
 static int table_alloc_common(gcv_table* t) {
     int rc;
-    // 64 root slots per interval: seeded builds keep interval i's 2^k_i roots at i << 6
+    // at most 64 roots per interval (the plain build; seeded builds store fewer, densely)
     const int64_t nroots = t->n_int << 6;
     if ((rc = talloc(t, &t->ioff, t->n_curve + 1))) return rc;
     if ((rc = talloc(t, &t->int_cum, t->n_int))) return rc;
@@ -630,7 +631,8 @@ static int table_alloc_common(gcv_table* t) {
     if ((rc = talloc(t, &t->stats, dev::kStatBufWords))) return rc;
     // seeded-build arrays (allocated once, so builds never allocate)
     if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
-        (rc = talloc(t, &t->root_k, t->n_int)) || (rc = talloc(t, &t->bucket_list, t->n_int)) ||
+        (rc = talloc(t, &t->root_k, t->n_int)) || (rc = talloc(t, &t->root_off, t->n_int)) ||
+        (rc = talloc(t, &t->bucket_root_chunk, dev::kBucketMaxWaves)) || (rc = talloc(t, &t->bucket_list, t->n_int)) ||
         (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketMaxWaves)) ||
         (rc = talloc(t, &t->bucket_bin, t->n_int)) || (rc = talloc(t, &t->bucket_totals, dev::kMaxBins)) ||
         (rc = talloc(t, &t->bucket_ranges, 2 * dev::kMaxBins)) || (rc = talloc(t, &t->rerun_list, t->n_int)))
@@ -816,10 +818,17 @@ static BSplineTableDev bspline_dev(gcv_table* t) {
         if (_le != hipSuccess) return _le;     \
     } while (0)
 
+// The learned seed predicted inside the coefficient pass (B-spline tables).
+struct SeedPredict {
+    dev::MlpParams P;
+    dev::SeedOut O;
+};
+
 // Build phases. Setup: statistics reset, and for B-splines the per-curve
-// scale / interval map and the per-interval derivative polynomials.
+// scale / interval map and the per-interval derivative polynomials (with
+// `sp`, also the learned seed's k and safety floor of every interval).
 template <typename Dt, int NGL>
-static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const dev::FeatSetup* fs = nullptr) {
+static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedPredict* sp = nullptr) {
     gcv_ctx* c = t->ctx;
     if (reset_stats)  // statistics rows and the seeded build's queue words
         LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3((dev::kStatBufWords + 255) / 256), dim3(256), 0,
@@ -832,21 +841,23 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const dev::
         LAUNCH_OR_RETURN("k_bspline_curve_setup", dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0,
                          c->stream, B, t->ioff, t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
     }
-    const dev::FeatSetup F = fs ? *fs : dev::FeatSetup{};
     for (const ClassRange& r : t->classes) {
         if (r.count == 0) continue;
         const int32_t* list = t->class_list ? t->class_list + r.offset : nullptr;
         const unsigned blocks = (unsigned)((r.count + 255) / 256);
-#define GCV_COEF_CASE(PP, DD)                                                                                 \
-    case 10 * PP + DD:                                                                                        \
-        if (fs)                                                                                               \
-            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD, true>), dim3(blocks), dim3(256),  \
-                             0, c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,  \
-                             t->coef32, t->cstride, t->u0, t->h, F);                                                      \
-        else                                                                                                  \
-            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD, false>), dim3(blocks), dim3(256), \
-                             0, c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,  \
-                             t->coef32, t->cstride, t->u0, t->h, F);                                                      \
+        // fused pass: 4 waves of 64 intervals per block, grid-stride
+        const unsigned sblocks = (unsigned)std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
+#define GCV_COEF_CASE(PP, DD)                                                                                    \
+    case 10 * PP + DD:                                                                                           \
+        if (sp)                                                                                                  \
+            LAUNCH_OR_RETURN("k_bspline_coef_seed", (dev::k_bspline_coef_seed<PP, DD>), dim3(sblocks),          \
+                             dim3(256), dev::kMlpLdsBytes + 4 * dev::kWaveLds, c->stream, B, list, r.count,      \
+                             t->ioff, t->int_curve, t->int_scale, t->coef64, t->coef32, t->cstride, t->u0, t->h, \
+                             sp->P, sp->O);                                                                      \
+        else                                                                                                     \
+            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD>), dim3(blocks), dim3(256), 0,       \
+                             c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,        \
+                             t->coef32, t->cstride, t->u0, t->h);                                                \
         break;
         switch (bspline_class(r.p, r.D)) {
             GCV_COEF_CASE(1, 2)
@@ -919,11 +930,12 @@ static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
 }
 
 // Seeded build: interval i starts from 2^k_i roots, k_i = max(k[i], safety
-// floor). Per evaluator class, the chunk kernel bins the class's intervals by
-// k wave by wave and runs the 1..32-root tiles, appending the 64-root
-// intervals to the class's region of whole_list, which the whole-interval
-// kernel then runs (its range is read on the device). No host sync, nothing
-// allocated, graph-capturable; classes are independent, so with
+// floor). The bucketing sorts the intervals by (evaluator class, k) and
+// writes each interval's first root (dense roots in interval order); per
+// class, the bins kernel runs the 1..32-root tiles, the whole-interval kernel
+// the 64-root intervals (bin ranges are read on the device), and the re-run
+// kernel the tiles that outgrew the bins kernel's small stack. No host sync,
+// nothing allocated, graph-capturable; classes are independent, so with
 // tune_streams > 1 they run on the auxiliary streams.
 template <typename Dt, int NGL>
 static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bool setup_done = false,
@@ -960,13 +972,15 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         const int nw = (int)((n + wchunk - 1) / wchunk);
         const unsigned gb = (unsigned)((nw + 3) / 4);
         LAUNCH_OR_RETURN("k_bucket_hist", dev::k_bucket_hist, dim3(gb), dim3(256), 0, c->stream, n, BA, n_bins,
-                         wchunk, nw, t->bucket_bin, t->bucket_hist);
+                         wchunk, nw, t->bucket_bin, t->bucket_hist, t->bucket_root_chunk);
         LAUNCH_OR_RETURN("k_bucket_totals", dev::k_bucket_totals, dim3((unsigned)n_bins), dim3(1024), 0, c->stream,
                          nw, (const int32_t*)t->bucket_hist, t->bucket_totals);
-        LAUNCH_OR_RETURN("k_bucket_rows", dev::k_bucket_rows, dim3((unsigned)n_bins), dim3(1024), 0, c->stream, nw,
-                         t->bucket_hist, (const int32_t*)t->bucket_totals, t->bucket_ranges);
+        LAUNCH_OR_RETURN("k_bucket_rows", dev::k_bucket_rows, dim3((unsigned)n_bins + 1), dim3(1024), 0, c->stream,
+                         nw, n_bins, t->bucket_hist, (const int32_t*)t->bucket_totals, t->bucket_ranges,
+                         t->bucket_root_chunk);
         LAUNCH_OR_RETURN("k_bucket_place", dev::k_bucket_place, dim3(gb), dim3(256), 0, c->stream, n, n_bins,
-                         wchunk, nw, (const int8_t*)t->bucket_bin, (const int32_t*)t->bucket_hist, t->bucket_list);
+                         wchunk, nw, (const int8_t*)t->bucket_bin, (const int32_t*)t->bucket_hist,
+                         (const int64_t*)t->bucket_root_chunk, t->bucket_list, t->root_off);
     }
     const BSplineTableDev T = bspline_dev(t);
     const bool f32 = sizeof(Dt) == 4;
@@ -987,6 +1001,7 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         S.bins.eps0 = eps_of<Dt>(t);
         S.bins.stack_cap = t->seed_stack_cap;
         S.bins.int_cost = cost;
+        S.bins.root_off = t->root_off;
         S.run = dev::BinRun{ranges, t->log2_nsub, t->o.max_depth, t->rerun_list + off, q};
         S.whole = subdiv_args<Dt>(t);
         S.whole.n_slots = cnt;
@@ -997,10 +1012,12 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         S.whole.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, 6, f32);
         S.whole.stack_cap = 64 * (S.whole.max_depth + 2);
         S.whole.int_cost = cost;
+        S.whole.root_off = t->root_off;
         S.rerun = subdiv_args<Dt>(t);
         S.rerun.eps0 = eps_of<Dt>(t);
         S.rerun.stack_cap = 64 * (dev::effective_depth(t->o.max_depth, t->log2_nsub, 0, f32) + 2);
         S.rerun.int_cost = cost;
+        S.rerun.root_off = t->root_off;
         S.root_k = t->root_k;
         if (t->kind == KIND_ELLIPSE) {
             e = Launch<Dt, NGL>::seeded_ellipse(L, t->a, t->b, t->o.normalize, S);
@@ -1037,9 +1054,9 @@ struct SetupOnly {
     static hipError_t run(gcv_table* t, bool reset_stats) { return build_setup<Dt, NGL>(t, reset_stats); }
 };
 template <typename Dt, int NGL>
-struct SetupWithFeatures {
-    static hipError_t run(gcv_table* t, bool reset_stats, const dev::FeatSetup* fs) {
-        return build_setup<Dt, NGL>(t, reset_stats, fs);
+struct SetupWithSeed {
+    static hipError_t run(gcv_table* t, bool reset_stats, const SeedPredict* sp) {
+        return build_setup<Dt, NGL>(t, reset_stats, sp);
     }
 };
 template <typename Dt, int NGL>
@@ -1103,12 +1120,14 @@ extern "C" int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor) {
     return GCV_OK;
 }
 
-extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int8_t** d_root_k) {
+extern "C" int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int8_t** d_root_k,
+                                     const int64_t** d_root_off) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (seeded) *seeded = t->seeded ? 1 : 0;
     if (log2_roots) *log2_roots = t->built_ls;
     if (d_root_k) *d_root_k = t->root_k;
+    if (d_root_off) *d_root_off = t->root_off;
     return GCV_OK;
 }
 
@@ -1295,7 +1314,8 @@ extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int s
 }
 
 extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
-                               uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns) {
+                               uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns,
+                               uint64_t* tiles, uint64_t* passes) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
     if (int rc = capture_guard(t->ctx, "gcv_table_stats")) return rc;
@@ -1311,6 +1331,8 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
         s[dev::kStatTasks] += row[dev::kStatTasks];
         s[dev::kStatDepthCap] += row[dev::kStatDepthCap];
         s[dev::kStatRerun] += row[dev::kStatRerun];
+        s[dev::kStatTiles] += row[dev::kStatTiles];
+        s[dev::kStatPasses] += row[dev::kStatPasses];
         s[dev::kStatMaxStack] = std::max(s[dev::kStatMaxStack], row[dev::kStatMaxStack]);
         s[dev::kStatOverflow] = std::max(s[dev::kStatOverflow], row[dev::kStatOverflow]);
     }
@@ -1321,6 +1343,8 @@ extern "C" int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* de
     if (tasks) *tasks = s[dev::kStatTasks];
     if (depth_capped) *depth_capped = s[dev::kStatDepthCap];
     if (reruns) *reruns = s[dev::kStatRerun];
+    if (tiles) *tiles = s[dev::kStatTiles];
+    if (passes) *passes = s[dev::kStatPasses];
     return GCV_OK;
 }
 
@@ -1356,6 +1380,7 @@ static int sample_launch(gcv_table* t, const int64_t* d_soff, int64_t n_total, c
     A.curve_len = t->curve_len;
     A.root_prefix = t->root_prefix;
     A.root_k = t->seeded ? t->root_k : nullptr;
+    A.root_off = t->seeded ? t->root_off : nullptr;
     A.deep = t->deep;
     A.log2_nsub = t->built_ls;
     A.eps0 = t->o.precision == GCV_FP32 ? (double)eps_of<float>(t) : eps_of<double>(t);
@@ -1612,28 +1637,20 @@ static hipError_t setup_and_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t*
                                     bool reset_stats) {
     gcv_ctx* c = t->ctx;
     const int64_t n = t->n_int;
+    const dev::SeedOut O{d_k, d_raw, t->seed_floor, hedge, t->log2_nsub,
+                         (float)(t->o.tolerance / (double)(1 << t->log2_nsub))};
+    if (t->kind == KIND_BSPLINE) {
+        const SeedPredict sp{mlp_dev(m), O};
+        return dispatch_build<SetupWithSeed>(t, reset_stats, &sp);
+    }
+    hipError_t e = dispatch_build<SetupOnly>(t, reset_stats);
+    if (e != hipSuccess || n == 0) return e;
     const int64_t waves = (n + 63) / 64;
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, (int64_t)c->n_cu * 8));
-    hipError_t e;
-    if (t->kind == KIND_BSPLINE) {
-        if (!t->feat_x && n > 0) {
-            if (c->capturing) return hipErrorStreamCaptureUnsupported;  // first learned build must run eagerly
-            if (talloc(t, &t->feat_x, n * dev::kFeat)) return hipErrorOutOfMemory;
-        }
-        const dev::FeatSetup fs{t->feat_x, t->seed_floor, m->mean, m->istd,
-                                (float)(t->o.tolerance / (double)(1 << t->log2_nsub)), t->log2_nsub};
-        e = dispatch_build<SetupWithFeatures>(t, reset_stats, &fs);
-        if (e != hipSuccess || n == 0) return e;
-        prof_begin(c, "k_mlp_predict");
-        hipLaunchKernelGGL(dev::k_mlp_predict_rows, dim3((unsigned)blocks), dim3(256), 0, c->stream,
-                           (const uint16_t*)t->feat_x, n, mlp_dev(m), hedge, d_k, d_raw);
-    } else {
-        e = dispatch_build<SetupOnly>(t, reset_stats);
-        if (e != hipSuccess || n == 0) return e;
-        prof_begin(c, "k_mlp_predict");
-        hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), 4 * dev::kWaveLds, c->stream,
-                           feat_args(t), mlp_dev(m), hedge, d_k, t->seed_floor, d_raw);
-    }
+    prof_begin(c, "k_mlp_predict");
+    hipLaunchKernelGGL(dev::k_mlp_predict, dim3((unsigned)blocks), dim3(256), dev::kMlpLdsBytes + 4 * dev::kWaveLds,
+                       c->stream,
+                       feat_args(t), mlp_dev(m), O);
     e = hipGetLastError();
     prof_end(c);
     return e;

@@ -149,34 +149,5 @@ __device__ void features_ellipse(double a, double b, float tol0, float* f) {
     f[15] = hi > 0.f ? fdiv(lo, hi) : 1.f;
 }
 
-// Optional outputs of the B-spline setup pass for a learned-seed build:
-// standardised BF16 features (the network's input rows) and the safety floor.
-struct FeatSetup {
-    uint16_t* xb;       // [n_int][16] (x - mean) * istd as BF16, or null
-    int8_t* kmin;       // [n_int] safety floor
-    const float* mean;  // [16]
-    const float* istd;  // [16]
-    float tol0;         // level-0 tolerance (normalised units)
-    int ls_plain;       // log2 n_subdivided
-};
-
-template <int P, int D>
-__device__ __forceinline__ void setup_features(const FeatSetup& F, int64_t i, const float* cf, double h, int nint,
-                                               const double* ctrl, int pstride, int s) {
-    float f[kFeat], r2;
-    features_bspline<P, D>(cf, h, nint, ctrl, pstride, s, F.tol0, f, &r2);
-    uint32_t w[kFeat / 2];
-#pragma unroll
-    for (int j = 0; j < kFeat / 2; j++) {
-        const uint32_t lo = f2bf((f[2 * j] - F.mean[2 * j]) * F.istd[2 * j]);
-        const uint32_t hi = f2bf((f[2 * j + 1] - F.mean[2 * j + 1]) * F.istd[2 * j + 1]);
-        w[j] = lo | (hi << 16);
-    }
-    uint4* dst = (uint4*)(F.xb + i * kFeat);
-    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
-    F.kmin[i] = floor_from_ratio(r2, F.ls_plain);
-}
-
 }  // namespace dev
 }  // namespace gcurval

@@ -32,7 +32,7 @@ namespace dev {
 // Subdivision
 // ---------------------------------------------------------------------------
 enum StatSlot { kStatQuad = 0, kStatDeepRoots = 1, kStatMaxStack = 2, kStatOverflow = 3, kStatTasks = 4,
-                kStatDepthCap = 5, kStatRerun = 6, kStatCount = 8 };
+                kStatDepthCap = 5, kStatRerun = 6, kStatTiles = 7, kStatPasses = 8, kStatCount = 9 };
 // Build statistics live in kStatSlots rows of kStatStride counters (one
 // 128-byte line each); block b adds into row b % kStatSlots, and readers sum
 // (or take the maximum over) the rows. Spreading the rows keeps thousands of
@@ -53,6 +53,8 @@ struct SubdivArgs {
     const int32_t* int_list;    // optional slot -> interval map
     const int64_t* slot_range;  // optional device {offset, count} into int_list; overrides n_slots
     int log2_nsub;              // roots per interval = 2^log2_nsub (<= 64); root block of interval i at i << log2_nsub
+    const int64_t* root_off;    // seeded builds: root block of interval i at root_off[i] (whole-interval,
+                                // bins and re-run kernels); null: at i << log2_nsub
     Dt eps0;                    // level-0 tolerance per root (tol / n_sub)
     int max_depth;
     double* root_prefix;        // inclusive prefix of root lengths in each interval (normalised units)
@@ -179,7 +181,7 @@ struct TaskStack {
 // Per-wave counters of a subdivision launch, added into the statistics rows
 // when the wave ends.
 struct WaveCounters {
-    unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0, n_cap = 0, n_rerun = 0;
+    unsigned long long n_quad = 0, n_deep = 0, n_tasks = 0, n_cap = 0, n_rerun = 0, n_tiles = 0, n_pass = 0;
     int max_stack = 0, overflow = 0;
     __device__ __forceinline__ void flush(unsigned long long* stats) const {
         if (lane_id() != 0) return;
@@ -189,6 +191,8 @@ struct WaveCounters {
         if (n_tasks) atomicAdd(&st[kStatTasks], n_tasks);
         if (n_cap) atomicAdd(&st[kStatDepthCap], n_cap);
         if (n_rerun) atomicAdd(&st[kStatRerun], n_rerun);
+        if (n_tiles) atomicAdd(&st[kStatTiles], n_tiles);
+        if (n_pass) atomicAdd(&st[kStatPasses], n_pass);
         if (max_stack) atomicMax(&st[kStatMaxStack], (unsigned long long)max_stack);
         if (overflow) atomicMax(&st[kStatOverflow], 1ull);
     }
@@ -244,6 +248,7 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
     Dt len = L + R;
     const bool fail = valid && max_depth > 0 && dabs(len - P) > A.eps0;
     wc.n_quad += 3ull * popc64(ballot(valid));
+    wc.n_tiles++;
     const uint64_t fmask = ballot(fail);
     bool overflowed = false;
 
@@ -261,6 +266,7 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
         int size = 2 * popc64(fmask);
         if (size > wc.max_stack) wc.max_stack = size;
         while (size > 0) {
+            wc.n_pass++;
             const int n = size < 64 ? size : 64;
             const bool take = lane < n;
             Key kk = 0;
@@ -404,10 +410,12 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         if (list) r = kWhole ? load_s(list + s) : list[s];
         return r;
     };
-    // A stage: what tile t needs from memory (interval, context, scale).
+    // A stage: what tile t needs from memory (interval, context, scale,
+    // root block).
     struct Stage {
         typename Ev::Ctx ctx;
         double sc;
+        int64_t rb;
         int i;
         bool v;
     };
@@ -420,6 +428,10 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         } else {
             st.ctx = ev.load(iv);
             st.sc = ev.scale(iv);
+        }
+        st.rb = (int64_t)iv << ls;
+        if constexpr (kWhole) {  // wave-uniform: a scalar load (a vector one would need a wave-wide wait)
+            if (A.root_off) st.rb = load_s(A.root_off + iv);
         }
         st.i = iv;
         st.v = ok;
@@ -441,8 +453,8 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         const Stage nxt = fetch(i_nxt, v_nxt);
         i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
         // the stack holds 64 (max_depth + 2) entries here, so it cannot overflow
-        if (subdivide_tile<Dt, NGL, Ev, kWhole>(ev, A, cur.ctx, cur.i, cur.v, (int64_t)cur.i << ls, cur.sc, ls,
-                                                A.max_depth, W, scan, wc))
+        if (subdivide_tile<Dt, NGL, Ev, kWhole>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, ls, A.max_depth, W,
+                                                scan, wc))
             wc.overflow = 1;
         cur = nxt;
     }
@@ -451,8 +463,9 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
 
 // ---------------------------------------------------------------------------
 // Seeded subdivision: interval i starts from 2^k_i roots (k_i in root_k);
-// its root block is always i << 6 (the table keeps 64 root slots per
-// interval). The bucketing sorts a class's intervals by k; k_subdivide_bins
+// the roots are stored densely in interval order, interval i's block at
+// root_off[i] = sum of 2^k_j over j < i (written by the bucketing). The
+// bucketing sorts a class's intervals by k; k_subdivide_bins
 // runs the class's bins k = 0..5 as one tile sequence (bin k's tiles hold
 // 64 / 2^k intervals, the last one possibly partial), and the whole-interval
 // kernel runs bin 6. The bins kernel keeps a small task stack, so more waves
@@ -469,88 +482,63 @@ struct BinRun {
     unsigned long long* rerun_count;
 };
 
-// Bin and slot of lane's interval in tile tl of the bin sequence (tile
-// starts ts[k] per bin, wave-uniform).
-struct BinTile {
-    int k;
-    int64_t slot;  // the lane's slot (lane 0's for a lane past the bin's end)
-    bool ok;
-};
-
-template <typename Dt, int NGL, class Ev>
-__global__ __launch_bounds__(256) void k_subdivide_bins(Ev ev, SubdivArgs<Dt> A, BinRun B) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = lane_id();
-    const int wib = uniform_int((int)(threadIdx.x >> 6));
-    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    const SubdivLds<Dt> W(smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap), A);
+// The tiles of bin K (2^K roots per interval, 64 / 2^K intervals per tile)
+// that fall to this wave. The bins' tiles form one sequence, tile T going to
+// wave T mod nwaves; `t_begin` is bin K's first tile in the sequence and is
+// advanced past the bin. K is a compile-time constant, so the root count,
+// masks and depth cap of the tile body are constants too. Software pipeline
+// as in k_subdivide: interval ids of tile t + 2, contexts of tile t + 1 in
+// flight while tile t computes.
+template <int K, typename Dt, int NGL, class Ev>
+__device__ __forceinline__ void subdivide_bin(const Ev& ev, const SubdivArgs<Dt>& A, const BinRun& B,
+                                              const SubdivLds<Dt>& W, int64_t gwave, int64_t nwaves,
+                                              int64_t& t_begin, WaveCounters& wc) {
     constexpr bool kF32 = sizeof(Dt) == 4;
-    int64_t first[kSeedPartBins], count[kSeedPartBins], ts[kSeedPartBins + 1];
-    ts[0] = 0;
-#pragma unroll
-    for (int k = 0; k < kSeedPartBins; k++) {
-        first[k] = load_s(B.ranges + 2 * k);
-        count[k] = load_s(B.ranges + 2 * k + 1);
-        ts[k + 1] = ts[k] + ((count[k] + (64 >> k) - 1) >> (6 - k));
-    }
-    const int64_t n_tiles = ts[kSeedPartBins];
-    WaveCounters wc;
-
-    auto locate = [&](int64_t tl) -> BinTile {
-        int k = 0;
-#pragma unroll
-        for (int j = 1; j < kSeedPartBins; j++)
-            if (tl >= ts[j]) k = j;
-        const int64_t base = (tl - ts[k]) << (6 - k);  // first interval of the tile within its bin
-        const int64_t s = base + (lane >> k);
-        BinTile t;
-        t.k = k;
-        t.ok = s < count[k];
-        t.slot = first[k] + (t.ok ? s : base);
-        return t;
+    constexpr int ipw_log2 = 6 - K;  // intervals per tile (log2)
+    const int lane = lane_id();
+    const int64_t first = load_s(B.ranges + 2 * K);
+    const int64_t count = load_s(B.ranges + 2 * K + 1);
+    const int64_t n_tiles = (count + (1 << ipw_log2) - 1) >> ipw_log2;
+    int64_t t0 = (gwave - t_begin) % nwaves;  // this wave's first tile of the bin
+    if (t0 < 0) t0 += nwaves;
+    t_begin += n_tiles;
+    if (t0 >= n_tiles) return;
+    const int jslot = lane >> K;
+    const LdsSegScan scan(W.scan, K);
+    const int max_depth = effective_depth(B.base_depth, B.ls_plain, K, kF32);
+    const int32_t* list = A.int_list + first;
+    auto interval_of = [&](int64_t tl, bool& ok) -> int {
+        const int64_t slot = (tl << ipw_log2) + jslot;
+        ok = slot < count;
+        return list[ok ? slot : 0];
     };
-    // software pipeline as in k_subdivide: interval ids of tile t + 2,
-    // contexts of tile t + 1 in flight while tile t computes
     struct Stage {
         typename Ev::Ctx ctx;
         double sc;
+        int64_t rb;
         int i;
-        int k;
         bool v;
     };
-    auto fetch = [&](int iv, const BinTile& t) -> Stage {
+    auto fetch = [&](int iv, bool ok) -> Stage {
         Stage st;
         st.ctx = ev.load(iv);
         st.sc = ev.scale(iv);
+        st.rb = A.root_off[iv];
         st.i = iv;
-        st.k = t.k;
-        st.v = t.ok;
+        st.v = ok;
         return st;
     };
-    Stage cur{};
-    BinTile t_nxt{};
-    int i_nxt = 0;
-    if (gwave < n_tiles) {
-        const BinTile t0 = locate(gwave);
-        cur = fetch(A.int_list[t0.slot], t0);
-        t_nxt = locate(gwave + nwaves < n_tiles ? gwave + nwaves : gwave);
-        i_nxt = A.int_list[t_nxt.slot];
-    }
-    for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
-        const Stage nxt = fetch(i_nxt, t_nxt);
-        const int64_t t2 = tile + 2 * nwaves;
-        t_nxt = locate(t2 < n_tiles ? t2 : tile);
-        i_nxt = A.int_list[t_nxt.slot];
-        const int k = cur.k;
-        const LdsSegScan scan(W.scan, k);
-        const bool over = subdivide_tile<Dt, NGL, Ev, false>(ev, A, cur.ctx, cur.i, cur.v, (int64_t)cur.i << 6,
-                                                             cur.sc, k, effective_depth(B.base_depth, B.ls_plain,
-                                                                                        k, kF32),
-                                                             W, scan, wc);
-        if (over) {
+    bool ok0;
+    Stage cur = fetch(interval_of(t0, ok0), ok0);
+    bool v_nxt;
+    int i_nxt = interval_of(t0 + nwaves, v_nxt);
+    for (int64_t tile = t0; tile < n_tiles; tile += nwaves) {
+        const Stage nxt = fetch(i_nxt, v_nxt);
+        i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
+        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, K, max_depth, W, scan,
+                                               wc)) {
             // list the tile's intervals (one lane each) for the full-stack re-run
-            const bool mine = cur.v && ((lane & ((1 << k) - 1)) == (1 << k) - 1);
+            const bool mine = cur.v && ((lane & ((1 << K) - 1)) == (1 << K) - 1);
             const uint64_t m = ballot(mine);
             unsigned long long base = 0;
             if (lane == 0) base = atomicAdd(B.rerun_count, (unsigned long long)popc64(m));
@@ -560,6 +548,23 @@ __global__ __launch_bounds__(256) void k_subdivide_bins(Ev ev, SubdivArgs<Dt> A,
         }
         cur = nxt;
     }
+}
+
+template <typename Dt, int NGL, class Ev>
+__global__ __launch_bounds__(256) void k_subdivide_bins(Ev ev, SubdivArgs<Dt> A, BinRun B) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wib = uniform_int((int)(threadIdx.x >> 6));
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const SubdivLds<Dt> W(smem + (size_t)wib * subdiv_lds_per_wave<Dt>(A.stack_cap), A);
+    WaveCounters wc;
+    int64_t t_begin = 0;
+    subdivide_bin<0, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
+    subdivide_bin<1, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
+    subdivide_bin<2, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
+    subdivide_bin<3, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
+    subdivide_bin<4, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
+    subdivide_bin<5, Dt, NGL>(ev, A, B, W, gwave, nwaves, t_begin, wc);
     wc.flush(A.stats);
 }
 
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(64) void k_subdivide_rerun(Ev ev, SubdivArgs<Dt> A,
         const typename Ev::Ctx ctx = ev.load(i);
         const double sc = ev.scale(i);
         const LdsSegScan scan(W.scan, k);
-        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, ctx, i, lane < (1 << k), (int64_t)i << 6, sc, k,
+        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, ctx, i, lane < (1 << k), load_s(A.root_off + i), sc, k,
                                                effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc))
             wc.overflow = 1;  // unreachable: the stack holds 64 (depth + 2) entries
     }
@@ -753,47 +758,45 @@ static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B
 }
 
 // One interval's coefficient block (P * D values, into whichever of the FP64
-// and FP32 tables the build keeps). With kFeatures, also the learned seed's
-// feature row and safety floor of the interval.
-template <int P, int D, bool kFeatures>
-__device__ __forceinline__ void coef_one(int64_t i, int s, const double* T, const double* Pc, int pstride,
-                                         double isc, double* c64, float* c32, double* u0, double* h,
-                                         const FeatSetup& F, int nint) {
+// and FP32 tables the build keeps), its parameter origin and width; the FP32
+// coefficients and the width are also returned (learned-seed features).
+template <int P, int D>
+__device__ __forceinline__ void coef_block(int s, const double* T, const double* Pc, int pstride, double isc,
+                                           double* c64, float* c32, double* u0, double* h, float* cf,
+                                           double* h_out) {
     double out[P * D];
     double uu, hh;
     span_poly<P, D>(s, T, Pc, pstride, isc, out, &uu, &hh);
     *u0 = uu;
     *h = hh;
-    float cf[P * D];
+    *h_out = hh;
 #pragma unroll
     for (int j = 0; j < P * D; j++) {
         cf[j] = (float)out[j];
         if (c64) c64[j] = out[j];
         if (c32) c32[j] = cf[j];
     }
-    if (kFeatures) setup_features<P, D>(F, i, cf, hh, nint, Pc, pstride, s);
 }
 
 // Coefficient pass of one evaluator class (degree P, dimension D), one thread
 // per interval; `list` holds the class's intervals in a mixed batch (null:
 // every interval is of this class).
-template <int P, int D, bool kFeatures>
+template <int P, int D>
 __global__ __launch_bounds__(256) void k_bspline_coef(BSplineDev B, const int32_t* list, int64_t n_slots,
                                                       const int64_t* ioff, const int32_t* int_curve,
                                                       const double* int_scale, double* coef64, float* coef32,
-                                                      int cstride, double* u0, double* h, FeatSetup F) {
+                                                      int cstride, double* u0, double* h) {
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n_slots) return;
     const int64_t i = list ? (int64_t)list[slot] : slot;
     const int c = int_curve[i];
-    const int64_t i0 = ioff[c];
-    const int nint = kFeatures ? (int)(ioff[c + 1] - i0) : 0;
     const double* T = B.knots + B.knot_off[c];
     const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
-    coef_one<P, D, kFeatures>(i, (int)(i - i0) + P, T, Pc, B.ctrl_stride, rcp_f64(int_scale[i]),
-                              coef64 ? coef64 + i * cstride : nullptr, coef32 ? coef32 + i * cstride : nullptr,
-                              &u0[i], &h[i],
-                              F, nint);
+    float cf[P * D];
+    double hh;
+    coef_block<P, D>((int)(i - ioff[c]) + P, T, Pc, B.ctrl_stride, rcp_f64(int_scale[i]),
+                     coef64 ? coef64 + i * cstride : nullptr, coef32 ? coef32 + i * cstride : nullptr, &u0[i], &h[i],
+                     cf, &hh);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,8 +813,9 @@ struct SampleArgs {
     const double* int_cum;      // [n_int] cumulative interval lengths (curve units)
     const double* curve_len;    // [n_curve]
     const double* root_prefix;  // inclusive root prefix per interval
-    const int8_t* root_k;       // seeded builds: log2 root count per interval, root block at i << 6;
+    const int8_t* root_k;       // seeded builds: log2 root count per interval, root block at root_off[i];
                                 // else (null) 2^log2_nsub roots at i << log2_nsub
+    const int64_t* root_off;    // seeded builds: first root of interval i
     const uint64_t* deep;       // [n_int]
     int log2_nsub;
     double eps0;                // level-0 tolerance per root (same value as the build)
@@ -1316,7 +1320,7 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
     int ls = A.log2_nsub;
     int64_t ro = (int64_t)i << ls;
     if (mode_seeded<M>(A)) {
-        ro = (int64_t)i << 6;
+        ro = A.root_off[i];
         ls = A.root_k[i];
     }
     int off = 0, span = 0;
@@ -1492,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 int64_t ro = cn << A.log2_nsub;
                 N.ls = A.log2_nsub;
                 if constexpr (kSeeded) {
-                    ro = cn << 6;
+                    ro = A.root_off[cn];
                     N.ls = A.root_k[cn];
                 }
                 const double v = lane < (1 << N.ls) ? A.root_prefix[ro + lane] : 0.0;

@@ -19,6 +19,7 @@
 #include "curves.h"
 #include "seeded.h"
 #include "features.h"
+#include "kernels.h"
 
 namespace gcurval {
 namespace dev {
@@ -213,6 +214,83 @@ __device__ __forceinline__ void mlp_forward16(const MlpFrags& W, bf16x4 x, float
     o1 = o[1];
 }
 
+// The network's operand fragments in LDS for the inference kernels, in the
+// order the forward pass reads them: lane l's fragment of each operand is
+// contiguous (conflict-free 8- and 16-byte reads; the bias fragments repeat
+// over lane groups and are read as broadcasts). Reading them per pass keeps
+// the weights out of the registers the interval work needs.
+struct MlpLds {
+    bf16x4 w1[4][64];
+    bf16x8 w2[8][64];
+    bf16x8 w3[2][64];
+    f32x4 b1[4][64];
+    f32x4 b2[4][64];
+    float b3[4];
+};
+
+// Stage the fragments: every thread of the block calls this, each copying
+// a share of the elements (element j of lane l's fragment of output tile mt,
+// as load_frags assembles it: A[m][k] = W[k][m], m = 16 mt + l % 16,
+// k = 4 (l / 16) + j within each 16-wide k block).
+__device__ __forceinline__ void stage_mlp(const MlpParams& P, MlpLds* L) {
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+    uint16_t* w1 = (uint16_t*)L->w1;
+    for (int e = tid; e < 4 * 64 * 4; e += nt) {  // [mt][l][j]
+        const int j = e & 3, l = (e >> 2) & 63, mt = e >> 8;
+        w1[e] = P.w1b[(4 * (l >> 4) + j) * kHid + mt * 16 + (l & 15)];
+    }
+    uint16_t* w2 = (uint16_t*)L->w2;
+    for (int e = tid; e < 8 * 64 * 8; e += nt) {  // [c * 4 + mt][l][j], j < 4: k block 2c, else 2c + 1
+        const int j = e & 7, l = (e >> 3) & 63, q = e >> 9, c = q >> 2, mt = q & 3;
+        const int k = (2 * c + (j >> 2)) * 16 + 4 * (l >> 4) + (j & 3);
+        w2[e] = P.w2b[k * kHid + mt * 16 + (l & 15)];
+    }
+    uint16_t* w3 = (uint16_t*)L->w3;
+    for (int e = tid; e < 2 * 64 * 8; e += nt) {  // [c][l][j]
+        const int j = e & 7, l = (e >> 3) & 63, c = e >> 9;
+        const int k = (2 * c + (j >> 2)) * 16 + 4 * (l >> 4) + (j & 3);
+        w3[e] = P.w3b[k * 16 + (l & 15)];
+    }
+    float* b1 = (float*)L->b1;
+    float* b2 = (float*)L->b2;
+    for (int e = tid; e < 4 * 64 * 4; e += nt) {  // [mt][l][j]: unit 16 mt + 4 (l / 16) + j
+        const int j = e & 3, l = (e >> 2) & 63, mt = e >> 8;
+        b1[e] = P.p[off_b1() + mt * 16 + 4 * (l >> 4) + j];
+        b2[e] = P.p[off_b2() + mt * 16 + 4 * (l >> 4) + j];
+    }
+    if (tid < kOut) L->b3[tid] = P.p[off_b3() + tid];
+    __syncthreads();
+}
+
+// mlp_forward16 with the operands read from LDS as each product needs them
+// (scheduling barriers between the layers keep the reads of one layer from
+// being hoisted above the products of the one before, which would hold all
+// the fragments in registers at once).
+__device__ __forceinline__ void mlp_forward16_lds(const MlpLds* L, bf16x4 x, float& o0, float& o1) {
+    const int l = lane_id();
+    bf16x4 a1[4], a2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) a1[mt] = relu_bf16(mfma16(L->w1[mt][l], x, L->b1[mt][l]));
+    __builtin_amdgcn_sched_barrier(0);
+    f32x4 h2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) h2[mt] = L->b2[mt][l];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const bf16x8 b = cat8(a1[2 * c], a1[2 * c + 1]);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) h2[mt] = mfma32(L->w2[c * 4 + mt][l], b, h2[mt]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) a2[mt] = relu_bf16(h2[mt]);
+    f32x4 o = {L->b3[0], L->b3[1], 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 2; c++) o = mfma32(L->w3[c][l], cat8(a2[2 * c], a2[2 * c + 1]), o);
+    o0 = o[0];
+    o1 = o[1];
+}
+
 // B operand X^T from a row-major BF16 row block (rows r0.., 16 features).
 __device__ __forceinline__ bf16x4 load_xt(const uint16_t* rows, int64_t r0, int64_t n) {
     const int l = lane_id();
@@ -301,87 +379,148 @@ static __global__ __launch_bounds__(256) void k_features(FeatArgs F, float* out)
 
 // ---------------------------------------------------------------------------
 // Inference: features -> standardise -> MLP -> k = clamp(rint(out0)) (+1 when
-// the pass head predicts a level-0 failure and hedge is set).
-//   k_mlp_predict       features computed here (ellipse tables); one lane per
-//                       interval, rows staged once through LDS
-//   k_mlp_predict_rows  rows precomputed by the B-spline setup pass
+// the pass head predicts a level-0 failure and hedge is set), with the
+// features computed where the interval's data already is, so no feature rows
+// go through memory:
+//   k_mlp_predict        ellipse tables: features from the axes
+//   k_bspline_coef_seed  B-spline tables: the coefficient pass of one class
+//                        (the same tables as k_bspline_coef), features from
+//                        the coefficients in registers
+// Each wave takes 64 intervals at a time: lane j standardises interval j's
+// features into an LDS row, and four 16-row forward passes read them back.
 // ---------------------------------------------------------------------------
 constexpr int kRowLd = kFeat + 4;  // LDS row stride (elements): 40 bytes, conflict-free 8-byte reads
 constexpr int kWaveLds = 64 * kRowLd * 2;
+constexpr int kMlpLdsBytes = (int)sizeof(MlpLds);
 
-static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParams P, int hedge, int8_t* kk,
-                                                            int8_t* kmin, float* raw_out) {
+// Where the predictions go: kk[i] (k of interval i), optional raw outputs,
+// the safety floor kmin[i].
+struct SeedOut {
+    int8_t* kk;
+    float* raw;     // optional [n][2]
+    int8_t* kmin;
+    int hedge;
+    int ls_plain;   // log2 n_subdivided (safety floor of the plain root count)
+    float tol0;     // level-0 tolerance (normalised units)
+};
+
+// Lane's standardised BF16 row into X[lane] (zeros for a lane without an
+// interval: its outputs are never stored).
+__device__ __forceinline__ void stage_row(uint16_t* X, const float* f, const MlpParams& P, bool valid) {
+    const int lane = lane_id();
+    uint32_t w[kFeat / 2];
+#pragma unroll
+    for (int j = 0; j < kFeat / 2; j++) {
+        const uint32_t lo = f2bf((f[2 * j] - P.feat_mean[2 * j]) * P.feat_istd[2 * j]);
+        const uint32_t hi = f2bf((f[2 * j + 1] - P.feat_mean[2 * j + 1]) * P.feat_istd[2 * j + 1]);
+        w[j] = valid ? (lo | (hi << 16)) : 0u;
+    }
+    uint2* row = (uint2*)(X + lane * kRowLd);
+#pragma unroll
+    for (int j = 0; j < kFeat / 4; j++) row[j] = make_uint2(w[2 * j], w[2 * j + 1]);
+}
+
+// Forward passes of the wave's 64 staged rows; row r belongs to lane r,
+// whose interval is `i` and which is valid when r < n_valid.
+__device__ __forceinline__ void predict_rows(const MlpLds* Wl, const uint16_t* X, int64_t i, int n_valid,
+                                             const SeedOut& O) {
+    const int lane = lane_id();
+    wave_lds_sync();
+    // an offset the compiler cannot see through keeps the fragment reads in
+    // each pass (hoisted out of the caller's loop they would hold 48 or more
+    // registers through the interval work)
+    int opaque = 0;
+    asm volatile("" : "+s"(opaque));
+    Wl = (const MlpLds*)((const char*)Wl + opaque);
+#pragma unroll 1
+    for (int nt = 0; nt < 4; nt++) {
+        const uint2 q = *(const uint2*)(X + (nt * 16 + (lane & 15)) * kRowLd + 4 * (lane >> 4));
+        bf16x4 x;
+        x[0] = (short)(q.x & 0xFFFFu);
+        x[1] = (short)(q.x >> 16);
+        x[2] = (short)(q.y & 0xFFFFu);
+        x[3] = (short)(q.y >> 16);
+        float o0, o1;
+        mlp_forward16_lds(Wl, x, o0, o1);
+        const int r = nt * 16 + (lane & 15);
+        const int64_t ir = (int64_t)__shfl((long long)i, r, 64);
+        if (lane < 16 && r < n_valid) {
+            O.kk[ir] = k_from_outputs(o0, o1, O.hedge);
+            if (O.raw) {
+                O.raw[ir * kOut] = o0;
+                O.raw[ir * kOut + 1] = o1;
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+static __global__ __launch_bounds__(256) void k_mlp_predict(FeatArgs F, MlpParams P, SeedOut O) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;
-    uint16_t* X = (uint16_t*)(smem + (size_t)w * kWaveLds);  // [64][kRowLd]
+    MlpLds* Wl = (MlpLds*)smem;
+    uint16_t* X = (uint16_t*)(smem + sizeof(MlpLds) + (size_t)w * kWaveLds);  // [64][kRowLd]
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-    if (gw * 64 >= F.n) return;
-    const MlpFrags Wf = load_frags(P);
-    float mean[kFeat], istd[kFeat];
-#pragma unroll
-    for (int j = 0; j < kFeat; j++) {
-        mean[j] = P.feat_mean[j];
-        istd[j] = P.feat_istd[j];
-    }
+    stage_mlp(P, Wl);
     for (int64_t t0 = gw * 64; t0 < F.n; t0 += nw * 64) {
         const int64_t i = t0 + lane;
+        const bool valid = i < F.n;
         float f[kFeat], r2 = 1.0f;
-        if (i < F.n) interval_features(F, i, f, &r2);
+        if (valid) interval_features(F, i, f, &r2);
         else
             for (int j = 0; j < kFeat; j++) f[j] = 0.f;
-        if (i < F.n && kmin) kmin[i] = floor_from_ratio(r2, F.ls_plain);
-#pragma unroll
-        for (int j = 0; j < kFeat; j++) X[lane * kRowLd + j] = f2bf((f[j] - mean[j]) * istd[j]);
-        wave_lds_sync();
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const uint2 q = *(const uint2*)(X + (nt * 16 + (lane & 15)) * kRowLd + 4 * (lane >> 4));
-            bf16x4 x;
-            x[0] = (short)(q.x & 0xFFFFu);
-            x[1] = (short)(q.x >> 16);
-            x[2] = (short)(q.y & 0xFFFFu);
-            x[3] = (short)(q.y >> 16);
-            float o0, o1;
-            mlp_forward16(Wf, x, o0, o1);
-            const int64_t r = t0 + nt * 16 + lane;
-            if (lane < 16 && r < F.n) {
-                kk[r] = k_from_outputs(o0, o1, hedge);
-                if (raw_out) {
-                    raw_out[r * kOut] = o0;
-                    raw_out[r * kOut + 1] = o1;
-                }
-            }
-        }
-        wave_lds_sync();
+        if (valid && O.kmin) O.kmin[i] = floor_from_ratio(r2, O.ls_plain);
+        stage_row(X, f, P, valid);
+        predict_rows(Wl, X, i, (int)(F.n - t0 < 64 ? F.n - t0 : 64), O);
     }
 }
 
-static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_predict_rows(const uint16_t* xb, int64_t n, MlpParams P,
-                                                                 int hedge, int8_t* kk, float* raw_out) {
+// Coefficient pass of one B-spline class (degree P, dimension D) fused with
+// the learned seed: writes what k_bspline_coef writes, and the interval's k
+// and safety floor. Grid-stride over 64-slot wave groups (the network's
+// weights are loaded once per wave).
+template <int P, int D>
+__global__ __launch_bounds__(256) void k_bspline_coef_seed(BSplineDev B, const int32_t* list, int64_t n_slots,
+                                                           const int64_t* ioff, const int32_t* int_curve,
+                                                           const double* int_scale, double* coef64, float* coef32,
+                                                           int cstride, double* u0, double* h, MlpParams Pm,
+                                                           SeedOut O) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int lane = lane_id();
-    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6;
+    MlpLds* Wl = (MlpLds*)smem;
+    uint16_t* X = (uint16_t*)(smem + sizeof(MlpLds) + (size_t)w * kWaveLds);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-    if (gw * 64 >= n) return;
-    const MlpFrags Wf = load_frags(P);
-    for (int64_t t0 = gw * 64; t0 < n; t0 += nw * 64) {
-        bf16x4 x[4];
+    stage_mlp(Pm, Wl);
+    for (int64_t t0 = gw * 64; t0 < n_slots; t0 += nw * 64) {
+        const int64_t slot = t0 + lane;
+        const bool valid = slot < n_slots;
+        int64_t i = 0;
+        float f[kFeat], r2 = 1.0f;
+        if (valid) {
+            i = list ? (int64_t)list[slot] : slot;
+            const int c = int_curve[i];
+            const int64_t i0 = ioff[c];
+            const int nint = (int)(ioff[c + 1] - i0);
+            const int s = (int)(i - i0) + P;
+            const double* T = B.knots + B.knot_off[c];
+            const double* Pc = B.ctrl + B.ctrl_off[c] * B.ctrl_stride;
+            float cf[P * D];
+            double hh;
+            coef_block<P, D>(s, T, Pc, B.ctrl_stride, rcp_f64(int_scale[i]), coef64 ? coef64 + i * cstride : nullptr,
+                             coef32 ? coef32 + i * cstride : nullptr, &u0[i], &h[i], cf, &hh);
+            // control-polygon features are ratios, so raw control points serve
+            features_bspline<P, D>(cf, hh, nint, Pc, B.ctrl_stride, s, O.tol0, f, &r2);
+            O.kmin[i] = floor_from_ratio(r2, O.ls_plain);
+        } else {
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) x[nt] = load_xt(xb, t0 + nt * 16, n);
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            float o0, o1;
-            mlp_forward16(Wf, x[nt], o0, o1);
-            const int64_t r = t0 + nt * 16 + lane;
-            if (lane < 16 && r < n) {
-                kk[r] = k_from_outputs(o0, o1, hedge);
-                if (raw_out) {
-                    raw_out[r * kOut] = o0;
-                    raw_out[r * kOut + 1] = o1;
-                }
-            }
+            for (int j = 0; j < kFeat; j++) f[j] = 0.f;
         }
+        stage_row(X, f, Pm, valid);
+        predict_rows(Wl, X, i, (int)(n_slots - t0 < 64 ? n_slots - t0 : 64), O);
     }
 }
 

@@ -2,8 +2,9 @@
 // 2^k_i roots (k_i in 0..6, predicted per interval) instead of a fixed n_sub:
 // the safety floor that raises a seed where the speed nearly vanishes, and
 // the bucketing of intervals by (evaluator class, k) (a stable counting
-// sort, wave-aggregated) so every subdivision tile has one shape. Interval
-// i's roots live at i * 64 whatever k_i is, so no root offsets are needed.
+// sort, wave-aggregated) so every subdivision tile has one shape. The roots
+// are stored densely in interval order; the bucketing passes also write each
+// interval's first root (root_off[i] = sum of 2^k_j over j < i).
 #pragma once
 
 #include "device.h"
@@ -162,13 +163,15 @@ __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* s_wave, i
 // in a second register), and each 64-interval group is ranked with one
 // ballot per distinct bin, so there are no LDS atomics and no block barriers.
 //   k_bucket_hist   bins8[i] = bin of i, root_k[i] = k_i; per-wave counts, bin-major:
-//                   hist[b * nw + w]
+//                   hist[b * nw + w]; roots of the chunk: root_chunk[w]
 //   k_bucket_totals block per bin: total count of the bin
 //   k_bucket_rows   block per bin: exclusive scan of the bin's row in place,
 //                   offset by the totals of the bins before it, and
-//                   ranges[2b] = first slot, ranges[2b + 1] = count of bin b
+//                   ranges[2b] = first slot, ranges[2b + 1] = count of bin b;
+//                   one more block: exclusive scan of root_chunk in place
 //   k_bucket_place  list[hist[b * nw + w] + rank] = i, rank = order of i
-//                   among the wave chunk's intervals of bin b
+//                   among the wave chunk's intervals of bin b, and
+//                   root_off[i] = root_chunk[w] + roots of the chunk before i
 // The list is ordered by (bin, interval), so a wave tile takes neighbouring
 // intervals of one curve.
 constexpr int kBucketChunk = 1024;      // intervals per wave chunk (more when capped)
@@ -207,13 +210,14 @@ __device__ __forceinline__ int group_rank(bool valid, int b, BinCounts& cnt) {
 }
 
 static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B, int n_bins, int64_t wchunk, int nw,
-                                                            int8_t* bins8, int32_t* hist) {
+                                                            int8_t* bins8, int32_t* hist, int64_t* root_chunk) {
     const int lane = lane_id();
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= nw) return;
     const int64_t b0 = w * wchunk;
     const int64_t b1 = b0 + wchunk < n ? b0 + wchunk : n;
     BinCounts cnt;
+    int64_t roots = 0;  // this lane's intervals' roots
     for (int64_t g0 = b0; g0 < b1; g0 += 64 * kBucketUnroll) {
         int bb[kBucketUnroll];
 #pragma unroll
@@ -224,6 +228,7 @@ static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B
                 k = k_eff(B.kk, B.kmin, i);
                 if (B.cls_idx) c = B.cls_idx[B.int_curve ? B.int_curve[i] : i];
                 B.root_k[i] = (int8_t)k;
+                roots += 1 << k;
             }
             bb[u] = kSeedBins * c + k;
         }
@@ -237,6 +242,8 @@ static __global__ __launch_bounds__(256) void k_bucket_hist(int64_t n, BinArgs B
     }
     if (lane < n_bins) hist[(int64_t)lane * nw + w] = cnt.lo;
     if (lane + 64 < n_bins) hist[(int64_t)(lane + 64) * nw + w] = cnt.hi;
+    for (int o = 32; o > 0; o >>= 1) roots += __shfl_xor(roots, o, 64);
+    if (lane == 0) root_chunk[w] = roots;
 }
 
 static __global__ __launch_bounds__(1024) void k_bucket_totals(int nw, const int32_t* hist, int32_t* totals) {
@@ -254,11 +261,24 @@ static __global__ __launch_bounds__(1024) void k_bucket_totals(int nw, const int
     }
 }
 
-static __global__ __launch_bounds__(1024) void k_bucket_rows(int nw, int32_t* hist, const int32_t* totals,
-                                                             int64_t* ranges) {
+static __global__ __launch_bounds__(1024) void k_bucket_rows(int nw, int n_bins, int32_t* hist,
+                                                             const int32_t* totals, int64_t* ranges,
+                                                             int64_t* root_chunk) {
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_start;
     const int b = blockIdx.x;
+    if (b == n_bins) {  // the last block: chunk root counts -> chunk root offsets
+        int64_t carry = 0;
+        for (int base = 0; base < nw; base += blockDim.x) {
+            const int j = base + (int)threadIdx.x;
+            const int64_t v = j < nw ? root_chunk[j] : 0;
+            int64_t tot;
+            const int64_t ex = block_scan_excl(v, s_wave, &tot);
+            if (j < nw) root_chunk[j] = carry + ex;
+            carry += tot;
+        }
+        return;
+    }
     if (threadIdx.x < 64) {  // start of bin b: totals of bins 0 .. b-1 (b < kMaxBins <= 128)
         const int l = (int)threadIdx.x;
         int64_t v = (l < b ? totals[l] : 0) + (l + 64 < b ? totals[l + 64] : 0);
@@ -285,7 +305,8 @@ static __global__ __launch_bounds__(1024) void k_bucket_rows(int nw, int32_t* hi
 
 static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, int n_bins, int64_t wchunk, int nw,
                                                              const int8_t* bins8, const int32_t* hist,
-                                                             int32_t* list) {
+                                                             const int64_t* root_chunk, int32_t* list,
+                                                             int64_t* root_off) {
     const int lane = lane_id();
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= nw) return;
@@ -294,6 +315,7 @@ static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, int n_bi
     BinCounts base;
     base.lo = lane < n_bins ? hist[(int64_t)lane * nw + w] : 0;
     base.hi = lane + 64 < n_bins ? hist[(int64_t)(lane + 64) * nw + w] : 0;
+    int64_t roots = root_chunk[w];  // first root of the group (wave-uniform)
     for (int64_t g0 = b0; g0 < b1; g0 += 64 * kBucketUnroll) {
         int bb[kBucketUnroll];
 #pragma unroll
@@ -306,7 +328,19 @@ static __global__ __launch_bounds__(256) void k_bucket_place(int64_t n, int n_bi
             const int64_t i = g0 + 64 * u + lane;
             const bool valid = i < b1;
             const int pos = group_rank(valid, bb[u], base);
-            if (valid) list[pos] = (int32_t)i;
+            // roots of the group's intervals in interval order (<= 64 x 64: int)
+            const int r = valid ? 1 << (bb[u] % kSeedBins) : 0;
+            int incl = r;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            if (valid) {
+                list[pos] = (int32_t)i;
+                root_off[i] = roots + (incl - r);
+            }
+            roots += __shfl(incl, 63, 64);
         }
     }
 }

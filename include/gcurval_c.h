@@ -152,9 +152,12 @@ int gcv_table_seed_floor(gcv_table* t, const int8_t** d_floor);
 
 /* Root layout of the last build: seeded flag, roots per interval (log2) of
  * a uniform build (interval i's roots at i << log2_roots in the root-prefix
- * view), and for a seeded build the device log2 root count of every interval
- * (int8 [n_intervals]; interval i's roots at i * 64). */
-int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int8_t** d_root_k);
+ * view), and for a seeded build the device log2 root count k_i of every
+ * interval (int8 [n_intervals]) and the device first root of every interval
+ * (int64 [n_intervals]; the roots are dense in interval order, interval i's
+ * 2^k_i roots at root_off[i]). Any output may be null. */
+int gcv_table_root_layout(gcv_table* t, int* seeded, int* log2_roots, const int8_t** d_root_k,
+                          const int64_t** d_root_off);
 
 int gcv_table_destroy(gcv_table* t);
 int gcv_table_info(const gcv_table* t, int* n_curve, int64_t* n_intervals, int* n_subdivided);
@@ -173,11 +176,13 @@ int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** 
 /* Build statistics of the last build: quadratures evaluated, refined roots,
  * maximum task-stack depth, stack overflow flag (must be 0), refinement tasks,
  * leaves that still failed the error test when the depth cap stopped them,
- * and seeded-build tiles whose refinement outgrew the small task stack of
- * the bins kernel (their intervals are re-run with the full stack). Any
- * pointer may be NULL. */
+ * seeded-build tiles whose refinement outgrew the small task stack of the
+ * bins kernel (their intervals are re-run with the full stack), 64-root wave
+ * tiles, and refinement passes (wave passes over popped tasks). Any pointer
+ * may be NULL. */
 int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
-                    uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns);
+                    uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns,
+                    uint64_t* tiles, uint64_t* passes);
 
 /* Arc-length sampling: arclengthReverse / kn_initSearchTask +
  * kn_arclengthReverseDfsFast_Paper (gcurval/detail/cuda/arclength_calculate.h:224-231,

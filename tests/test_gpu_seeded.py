@@ -66,7 +66,7 @@ def test_seeded_random_k_matches_truth(ctx, cubic):
         d_k = ctx.array(k)
         d_cost = ctx.empty(b.n_intervals, np.int32)
         eng.createArclengthTableSeeded(d_k, d_cost)
-        seeded, _, _ = eng.root_layout()
+        seeded = eng.root_layout()[0]
         assert seeded
         Lg = eng.getAllCurveArclengths()
         assert np.max(np.abs(Lg - L) / L) <= 1e-6
@@ -76,6 +76,9 @@ def test_seeded_random_k_matches_truth(ctx, cubic):
         assert int(cost.sum()) == eng.stats()["quadratures"]
         _, rp, _, _, _ = eng.tables()
         assert rp.size == int(np.sum(1 << keff))
+        # dense roots: interval i's block starts at the sum of 2^k_j over j < i
+        np.testing.assert_array_equal(eng.root_counts(), keff)
+        np.testing.assert_array_equal(eng.root_offsets(), np.concatenate([[0], np.cumsum(1 << keff)[:-1]]))
         check_samples(eng, b, counts, ut, segt, L, cum)
         d_k.free()
         d_cost.free()
