@@ -837,7 +837,9 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedP
     dev::BSplineDev B{t->n_curve, t->bat.ctrl_stride, t->bat.degree, t->bat.dim, t->bat.knot_off,
                       t->bat.ctrl_off, t->bat.knots, t->bat.ctrl};
     if (t->n_curve > 0) {
-        const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 8);
+        // a wave per curve: the per-curve loops are latency chains, so every
+        // curve gets its own wave (up to 64 Ki blocks, then grid-stride)
+        const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, 65536);
         LAUNCH_OR_RETURN("k_bspline_curve_setup", dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0,
                          c->stream, B, t->ioff, t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
     }
@@ -881,7 +883,7 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedP
 static hipError_t build_scan(gcv_table* t) {
     gcv_ctx* c = t->ctx;
     if (t->kind == KIND_ELLIPSE || t->n_curve == 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, (int64_t)c->n_cu * 16);
+    const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, 65536);  // a wave per curve
     LAUNCH_OR_RETURN("k_curve_scan", dev::k_curve_scan, dim3((unsigned)blocks), dim3(256), 0, c->stream, t->n_curve,
                      t->ioff, t->int_cum, t->curve_len);
     return hipSuccess;

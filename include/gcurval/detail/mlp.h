@@ -223,8 +223,8 @@ struct MlpLds {
     bf16x4 w1[4][64];
     bf16x8 w2[8][64];
     bf16x8 w3[2][64];
-    f32x4 b1[4][64];
-    f32x4 b2[4][64];
+    f32x4 b1[4][4];  // [mt][l / 16]
+    f32x4 b2[4][4];
     float b3[4];
 };
 
@@ -253,10 +253,9 @@ __device__ __forceinline__ void stage_mlp(const MlpParams& P, MlpLds* L) {
     }
     float* b1 = (float*)L->b1;
     float* b2 = (float*)L->b2;
-    for (int e = tid; e < 4 * 64 * 4; e += nt) {  // [mt][l][j]: unit 16 mt + 4 (l / 16) + j
-        const int j = e & 3, l = (e >> 2) & 63, mt = e >> 8;
-        b1[e] = P.p[off_b1() + mt * 16 + 4 * (l >> 4) + j];
-        b2[e] = P.p[off_b2() + mt * 16 + 4 * (l >> 4) + j];
+    for (int e = tid; e < 4 * 4 * 4; e += nt) {  // [mt][g][j]: unit 16 mt + 4 g + j (g = l / 16)
+        b1[e] = P.p[off_b1() + e];
+        b2[e] = P.p[off_b2() + e];
     }
     if (tid < kOut) L->b3[tid] = P.p[off_b3() + tid];
     __syncthreads();
@@ -270,11 +269,11 @@ __device__ __forceinline__ void mlp_forward16_lds(const MlpLds* L, bf16x4 x, flo
     const int l = lane_id();
     bf16x4 a1[4], a2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) a1[mt] = relu_bf16(mfma16(L->w1[mt][l], x, L->b1[mt][l]));
+    for (int mt = 0; mt < 4; mt++) a1[mt] = relu_bf16(mfma16(L->w1[mt][l], x, L->b1[mt][l >> 4]));
     __builtin_amdgcn_sched_barrier(0);
     f32x4 h2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) h2[mt] = L->b2[mt][l];
+    for (int mt = 0; mt < 4; mt++) h2[mt] = L->b2[mt][l >> 4];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         const bf16x8 b = cat8(a1[2 * c], a1[2 * c + 1]);
@@ -408,12 +407,16 @@ struct SeedOut {
 // interval: its outputs are never stored).
 __device__ __forceinline__ void stage_row(uint16_t* X, const float* f, const MlpParams& P, bool valid) {
     const int lane = lane_id();
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
     uint32_t w[kFeat / 2];
 #pragma unroll
     for (int j = 0; j < kFeat / 2; j++) {
-        const uint32_t lo = f2bf((f[2 * j] - P.feat_mean[2 * j]) * P.feat_istd[2 * j]);
-        const uint32_t hi = f2bf((f[2 * j + 1] - P.feat_mean[2 * j + 1]) * P.feat_istd[2 * j + 1]);
-        w[j] = valid ? (lo | (hi << 16)) : 0u;
+        // round to nearest even, two values per conversion (finite features:
+        // the same bits as f2bf)
+        const f32x2_t v = {(f[2 * j] - P.feat_mean[2 * j]) * P.feat_istd[2 * j],
+                           (f[2 * j + 1] - P.feat_mean[2 * j + 1]) * P.feat_istd[2 * j + 1]};
+        w[j] = valid ? __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t)) : 0u;
     }
     uint2* row = (uint2*)(X + lane * kRowLd);
 #pragma unroll
