@@ -14,8 +14,11 @@ the other build path is timed on the same inputs ("seed_comparison").
 Default workload: 1,000,000 ellipses, a ~ U[0.5, 10], b = a U[0.01, 1], one
 knot interval each, FP32 quadrature with FP64 scan, 64..4096 samples per curve
 (log-uniform), uniform positions j/(N-1). The same line carries the learned vs
-plain comparison on ~4M knot intervals of cubic 3-D B-splines
-("c3_seed_comparison").
+plain comparison on the three B-spline workloads ("c2_seed_comparison": 20,000
+cubic 3-D curves, ~7.3M intervals; "c3_seed_comparison": the same distribution
+at ~4M intervals; "c4_seed_comparison": 50,000 mixed-degree 2-D/3-D curves,
+~12M intervals), the latency of the single-ellipse case ("c0_latency"), and the
+reference CPU algorithm on one thread and on all host cores ("cpu_baseline").
 """
 from __future__ import annotations
 
@@ -46,7 +49,7 @@ def f_node_bspline(p, D):
 
 # kernels reported per step; a name is a prefix of the launch names it sums
 KERNELS = ("k_subdivide", "k_curve_scan", "k_bspline_curve_setup", "k_bspline_coef", "k_sample", "k_fill_stats",
-           "k_seed_floor", "k_bucket", "k_root", "k_scan_block_sums", "k_mlp_predict", "k_block_curve")
+           "k_seed_floor", "k_bucket", "k_mlp_predict", "k_block_curve")
 
 
 def log(msg):
@@ -124,6 +127,75 @@ def cpu_baseline(batch, counts, threads, budget_s=20.0):
         "build_s": t_build, "sample_s": t_rev,
         "intervals_per_s": n_int / t, "samples_per_s": n_smp / t,
     }
+
+
+def cpu_info():
+    """(logical CPUs of the host, CPU model name)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def c0_latency(G, ctx, reps=20, warmup=3):
+    """Single ellipse a = 3, b = 1 as one knot interval, FP64, GL-5, absolute
+    tolerance 1e-8, then 1,000 arc-length-uniform samples at j / N: the
+    latency of one build + sample call pair (median of `reps` event-timed
+    runs, and of as many replays of the pair captured in a graph)."""
+    from gcurval_amd import EllipseBatch
+    b = EllipseBatch(np.array([3.0]), np.array([1.0]))
+    eng = G.ArclengthCalculateGpu(ctx, b, precision=G.GCV_FP64, gl_order=5, n_subdivided=64, tolerance=1e-8,
+                                  normalize=False)
+    n = 1000
+    d_off = ctx.array(np.array([0, n], np.int64))
+    d_out = ctx.empty(n, np.float64)
+
+    def step():
+        eng.createArclengthTable()
+        eng.sample_device(d_off, n, None, True, d_out, None, uniform_closed=False)
+
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    times = []
+    for _ in range(reps):
+        ctx.timer_start()
+        step()
+        times.append(ctx.timer_stop())
+    ctx.graph_begin()
+    step()
+    graph = ctx.graph_end()
+    graph.launch()
+    ctx.sync()
+    gtimes = []
+    for _ in range(reps):
+        ctx.timer_start()
+        graph.launch()
+        gtimes.append(ctx.timer_stop())
+    graph.close()
+    L = float(eng.getAllCurveArclengths()[0])
+    u = d_out.download()
+    rec = {"workload": "c0_single_ellipse", "a": 3.0, "b": 1.0, "precision": "fp64", "gl_order": 5,
+           "tolerance": 1e-8, "n_samples": n, "positions": "j/N",
+           "median_ms": float(np.median(times)), "min_ms": float(np.min(times)),
+           "graph_median_ms": float(np.median(gtimes)), "length": L,
+           "samples_monotone": bool(np.all(np.diff(u) > 0))}
+    try:
+        from scipy.special import ellipe
+        exact = 4 * 3.0 * float(ellipe(1.0 - 1.0 / 9.0))
+        rec["length_rel_err"] = abs(L - exact) / exact
+    except ImportError:
+        pass
+    eng.close()
+    d_off.free()
+    d_out.free()
+    return rec
 
 
 def copy_calibration(G, mib, steps):
@@ -313,20 +385,32 @@ def roofline_of(run, batch, counts, cfg_name, steps):
     else:
         dom, flop, dur = "k_subdivide", quad_flop, sub_ms
     achieved = flop / (dur * 1e-3) / 1e12 if dur > 0 else 0.0
-    traffic = None
+    traffic, traffic_source = None, None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         rec = json.load(open(tf)).get(cfg_name, {}).get("kernels", {}).get(dom)
         if rec:
             traffic = rec["read_bytes"] + rec["write_bytes"]
-    # compute roof: the published FP32 peak; the work runs on the vector ALU (non-packed v_fma_f32 issues
-    # at that rate, bench.py --valu-calibration), with its FP64 part at half of it
-    return {"kernel": dom, "bound": "mfma", "compute_unit": "valu (FP32 and FP64 vector ALU, no matrix cores)",
-            "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "flop_per_step": flop,
-            "sample_flop_cubic_path": cubic, "sample_flop_quintic_path": quintic, "quintic_sample_fraction": frac_q,
-            "kernel_ms_per_step": dur,
-            "hbm_gbs_of_traffic": (traffic / (dur * 1e-3) / 1e9) if (traffic and dur > 0) else None}
+            traffic_source = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
+                              "workload, scaled by the copy-kernel calibration (not measured in this run)")
+    # the roof that binds: the quadratures and the sampler's Newton steps run
+    # on the FP32 vector ALU (FP64 parts at half rate; non-packed v_fma_f32
+    # issues at the published FP32 rate, bench.py --valu-calibration); no
+    # matrix cores are involved
+    out = {"kernel": dom, "bound": "valu", "compute_unit": "FP32/FP64 vector ALU (no matrix cores)",
+           "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+           "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_source,
+           "flop_per_step": flop, "sample_flop_cubic_path": cubic, "sample_flop_quintic_path": quintic,
+           "quintic_sample_fraction": frac_q, "kernel_ms_per_step": dur,
+           "hbm_gbs_of_traffic": (traffic / (dur * 1e-3) / 1e9) if (traffic and dur > 0) else None}
+    if dom == "k_sample" and dur > 0:
+        # the sampler's HBM side: 8 B per sample (FP64 parameter out) + 4 B per
+        # knot interval (count / offset), the algorithmic output bytes
+        n_int = int(batch.n_intervals)
+        hbm_bytes = 8 * n_total + 4 * n_int
+        out["hbm"] = {"algorithmic_bytes": hbm_bytes, "achieved_gbs": hbm_bytes / (dur * 1e-3) / 1e9,
+                      "peak_gbs": PEAK_HBM_GBS, "frac": hbm_bytes / (dur * 1e-3) / 1e9 / PEAK_HBM_GBS}
+    return out
 
 
 def main():
@@ -342,8 +426,9 @@ def main():
                          "plain for the 1M-ellipse workload, which is defined on the plain method, "
                          "learned for the B-spline workloads")
     ap.add_argument("--no-compare", action="store_true", help="time only --path")
-    ap.add_argument("--no-c3", action="store_true",
-                    help="skip the learned-vs-plain check on the 4M-interval cubic B-spline workload")
+    ap.add_argument("--no-c3", "--no-seed-comparisons", dest="no_seed_comparisons", action="store_true",
+                    help="skip the learned-vs-plain comparisons on the B-spline workloads (c2, c3, c4)")
+    ap.add_argument("--no-c0", action="store_true", help="skip the single-ellipse latency entry")
     ap.add_argument("--seed-weights", default=None, help="learned-seed weights (.npz); default: the packaged ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time the library's streaming copy kernel over MIB mebibytes")
@@ -413,28 +498,51 @@ def main():
         result["seed_comparison"] = comparison(runs, n_int, n_total)
     del runs, main_run
 
-    if not args.no_c3 and args.config != "c3":
-        b3, c3, cfg3 = workload("c3")
-        r3 = measure(G, ctx, b3, c3, 3, 1, ["plain", "learned"], args.seed_weights)
-        comp = comparison(r3, b3.n_intervals, int(np.sum(c3)))
-        comp["workload"] = cfg3
-        result["c3_seed_comparison"] = comp
-        log(f"[bench] c3 learned vs plain: build {comp['build_speedup']:.2f}x, step {comp['step_speedup']:.2f}x")
+    if not args.no_seed_comparisons:
+        speedups = {}
+        for name in ("c2", "c3", "c4"):
+            if name == args.config:
+                if "seed_comparison" in result:
+                    speedups[name] = result["seed_comparison"]["build_speedup"]
+                continue
+            bk, ck, cfgk = workload(name)
+            rk = measure(G, ctx, bk, ck, 3, 1, ["plain", "learned"], args.seed_weights)
+            comp = comparison(rk, bk.n_intervals, int(np.sum(ck)))
+            comp["workload"] = cfgk
+            comp["n_intervals"] = int(bk.n_intervals)
+            result[f"{name}_seed_comparison"] = comp
+            speedups[name] = comp["build_speedup"]
+            log(f"[bench] {name} learned vs plain: build {comp['build_speedup']:.2f}x, "
+                f"step {comp['step_speedup']:.2f}x")
+            del rk, bk, ck
+        result["seed_build_speedups"] = speedups
+
+    if not args.no_c0:
+        result["c0_latency"] = c0_latency(G, ctx)
+        log(f"[bench] c0 latency {result['c0_latency']['median_ms']:.3f} ms "
+            f"(graph {result['c0_latency']['graph_median_ms']:.3f} ms)")
 
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nproc, model = cpu_info()
+        threads = args.cpu_threads or min(16, nproc or 1)
+
+        def describe(cb, th):
+            return (f"first {cb['n_curve']} curves of the same workload ({cb['n_intervals']} intervals, "
+                    f"{cb['n_samples']} samples), oracle restatement of ArclengthCalculateCpu, "
+                    f"build {cb['build_s']:.2f}s + sampling {cb['sample_s']:.2f}s on {th} thread(s)")
         try:
-            cb = cpu_baseline(batch, counts, threads)
+            cb1 = cpu_baseline(batch, counts, 1, budget_s=12.0)
+            cbn = cpu_baseline(batch, counts, threads, budget_s=15.0) if threads > 1 else cb1
             result["cpu_baseline"] = {
-                "value": cb["intervals_per_s"], "unit": "curve-intervals/s", "cores": threads,
-                "kind": "port", "samples_per_s": cb["samples_per_s"],
-                "sample": f"first {cb['n_curve']} curves of the same workload ({cb['n_intervals']} intervals, "
-                          f"{cb['n_samples']} samples), oracle restatement of ArclengthCalculateCpu, "
-                          f"build {cb['build_s']:.2f}s + sampling {cb['sample_s']:.2f}s on {threads} threads",
+                "value": cbn["intervals_per_s"], "unit": "curve-intervals/s", "cores": threads,
+                "kind": "port", "samples_per_s": cbn["samples_per_s"], "sample": describe(cbn, threads),
+                "single_thread": {"value": cb1["intervals_per_s"], "samples_per_s": cb1["samples_per_s"],
+                                  "cores": 1, "sample": describe(cb1, 1)},
+                "nproc": nproc, "cpu_model": model,
             }
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"value": None, "unit": "curve-intervals/s", "cores": 0, "kind": "port",
-                                      "sample": f"unavailable: {e}"}
+                                      "sample": f"unavailable: {e}", "nproc": nproc, "cpu_model": model}
     print(json.dumps(result), flush=True)
     ctx.close()
 
