@@ -33,7 +33,7 @@ gcurvalDevice, gcurvalHost, gcurvalReverseReturnUs, gcurvalReverseReturnPoints =
 GCV_FP32, GCV_FP64 = 0, 1
 
 _STATUS = {0: "GCV_OK", 1: "GCV_ERR_INVALID", 2: "GCV_ERR_OOM", 3: "GCV_ERR_HIP",
-           4: "GCV_ERR_UNSUPPORTED", 5: "GCV_ERR_STATE"}
+           4: "GCV_ERR_UNSUPPORTED", 5: "GCV_ERR_STATE", 6: "GCV_ERR_DATA"}
 
 
 class GcvError(RuntimeError):
@@ -408,18 +408,26 @@ class ArclengthCalculateGpu:
         self.opts = make_opts(precision, gl_order, n_subdivided, max_depth, tolerance, normalize, newton_iters,
                               fp64_step)
         self._bufs = []
+        self.h = None
         h = C.c_void_p()
-        if batch.kind == "ellipse":
-            da, db = ctx.array(batch.a), ctx.array(batch.b)
-            self._bufs += [da, db]
-            ctx._check(lib().gcv_table_create_ellipse(ctx.h, batch.n_curve, da.ptr, db.ptr, C.byref(self.opts),
-                                                      C.byref(h)))
-        else:
-            bufs = [ctx.array(batch.degree), ctx.array(batch.dim), ctx.array(batch.knot_off),
-                    ctx.array(batch.ctrl_off), ctx.array(batch.knots), ctx.array(batch.ctrl.reshape(-1))]
-            self._bufs += bufs
-            bc = _BSplineBatchC(batch.n_curve, batch.ctrl_stride, *[b.ptr for b in bufs])
-            ctx._check(lib().gcv_table_create_bspline(ctx.h, C.byref(bc), C.byref(self.opts), C.byref(h)))
+        try:
+            if batch.kind == "ellipse":
+                da, db = ctx.array(batch.a), ctx.array(batch.b)
+                self._bufs += [da, db]
+                ctx._check(lib().gcv_table_create_ellipse(ctx.h, batch.n_curve, da.ptr, db.ptr,
+                                                          C.byref(self.opts), C.byref(h)))
+            else:
+                bufs = [ctx.array(batch.degree), ctx.array(batch.dim), ctx.array(batch.knot_off),
+                        ctx.array(batch.ctrl_off), ctx.array(batch.knots), ctx.array(batch.ctrl.reshape(-1))]
+                self._bufs += bufs
+                bc = _BSplineBatchC(batch.n_curve, batch.ctrl_stride, *[b.ptr for b in bufs])
+                ctx._check(lib().gcv_table_create_bspline(ctx.h, C.byref(bc), C.byref(self.opts), C.byref(h)))
+        except Exception:
+            # a rejected batch (bad options or data) leaves no device buffers behind
+            for buf in self._bufs:
+                buf.free()
+            self._bufs = []
+            raise
         self.h = h.value
         nc, ni, ns = C.c_int(), C.c_int64(), C.c_int()
         ctx._check(lib().gcv_table_info(self.h, C.byref(nc), C.byref(ni), C.byref(ns)))

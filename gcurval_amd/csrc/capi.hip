@@ -647,6 +647,28 @@ static int table_alloc_common(gcv_table* t) {
     return GCV_OK;
 }
 
+// Runs one input-check kernel (k_check_*) and reports its finding: the first
+// bad curve and what is wrong with it, or GCV_OK.
+template <class Launch>
+static int check_input(gcv_ctx* c, const char* family, Launch launch) {
+    void* dbad = nullptr;
+    int rc = gcv_alloc(c, sizeof(unsigned long long), &dbad);
+    if (rc) return rc;
+    unsigned long long bad = ~0ull;
+    if ((rc = gcv_copy_h2d(c, dbad, &bad, sizeof(bad)))) { gcv_free(c, dbad); return rc; }
+    launch((unsigned long long*)dbad);
+    if (hipGetLastError() != hipSuccess) { gcv_free(c, dbad); return set_err(c, GCV_ERR_HIP, "input check launch failed"); }
+    rc = gcv_copy_d2h(c, &bad, dbad, sizeof(bad));
+    gcv_free(c, dbad);
+    if (rc) return rc;
+    if (bad == ~0ull) return GCV_OK;
+    static const char* bspline_why[4] = {"", "a knot is not finite", "its knots decrease",
+                                         "a control point is not finite"};
+    const int why = (int)(bad & 3);
+    return set_err(c, GCV_ERR_DATA, "%s curve %llu: %s", family, bad >> 2,
+                   family[0] == 'e' ? "a semi-axis is not finite" : bspline_why[why]);
+}
+
 extern "C" int gcv_table_create_ellipse(gcv_ctx* c, int n_curve, const double* d_a, const double* d_b,
                                         const gcv_opts* opts, gcv_table** out) {
     use_device(c);
@@ -658,6 +680,13 @@ extern "C" int gcv_table_create_ellipse(gcv_ctx* c, int n_curve, const double* d
     t->kind = KIND_ELLIPSE;
     int rc = resolve_opts(c, opts, &t->o);
     if (rc) { delete t; return rc; }
+    if (n_curve > 0 && (rc = check_input(c, "ellipse", [&](unsigned long long* bad) {
+            hipLaunchKernelGGL(dev::k_check_ellipse, dim3((unsigned)((n_curve + 255) / 256)), dim3(256), 0,
+                               c->stream, n_curve, d_a, d_b, bad);
+        }))) {
+        delete t;
+        return rc;
+    }
     t->n_curve = n_curve;
     t->n_int = n_curve;
     t->log2_nsub = log2i(t->o.n_subdivided);
@@ -717,6 +746,14 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
     }
     t->n_int = ioff[n];
     if (t->n_int > INT32_MAX) { delete t; return set_err(c, GCV_ERR_UNSUPPORTED, "more than 2^31-1 intervals"); }
+    if (n > 0) {
+        const dev::BSplineDev BD{n, B->ctrl_stride, B->degree, B->dim, B->knot_off, B->ctrl_off, B->knots, B->ctrl};
+        rc = check_input(c, "B-spline", [&](unsigned long long* bad) {
+            const unsigned blocks = (unsigned)std::min<int64_t>((n + 3) / 4, 65536);
+            hipLaunchKernelGGL(dev::k_check_bspline, dim3(blocks), dim3(256), 0, c->stream, BD, bad);
+        });
+        if (rc) { delete t; return rc; }
+    }
     if ((rc = table_alloc_common(t))) { table_free(t); return rc; }
     if ((rc = gcv_copy_h2d(c, t->ioff, ioff.data(), sizeof(int64_t) * (n + 1)))) { table_free(t); return rc; }
     if ((rc = talloc(t, &t->curve_scale, n)) || (rc = talloc(t, &t->int_scale, t->n_int)) ||

@@ -695,6 +695,38 @@ struct BSplineDev {
     const double* ctrl;
 };
 
+// Input check of a batch before any table is built: B-spline knots finite and
+// non-decreasing within each curve, control points finite (the coordinates a
+// curve uses); ellipse semi-axes finite. The first bad curve and the reason
+// land in *bad = curve * 4 + reason (atomic minimum, *bad starts at ~0).
+enum BadInput { kBadKnotValue = 1, kBadKnotOrder = 2, kBadCtrl = 3, kBadAxis = 1 };
+static __global__ __launch_bounds__(256) void k_check_bspline(BSplineDev B, unsigned long long* bad) {
+    const int lane = lane_id();
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t c = gwave; c < B.n_curve; c += nwaves) {
+        const int64_t k0 = B.knot_off[c], k1 = B.knot_off[c + 1];
+        int reason = 0;
+        for (int64_t j = k0 + lane; j < k1; j += 64) {
+            const double t = B.knots[j];
+            if (!__builtin_isfinite(t)) reason = reason > kBadKnotValue ? reason : kBadKnotValue;
+            else if (j + 1 < k1 && B.knots[j + 1] < t) reason = reason > kBadKnotOrder ? reason : kBadKnotOrder;
+        }
+        const int D = B.dim[c], st = B.ctrl_stride;
+        const int64_t p0 = B.ctrl_off[c] * st, p1 = B.ctrl_off[c + 1] * st;
+        for (int64_t j = p0 + lane; j < p1; j += 64)
+            if ((int)((j - p0) % st) < D && !__builtin_isfinite(B.ctrl[j])) reason = kBadCtrl;
+        for (int o = 32; o > 0; o >>= 1) reason = max(reason, __shfl_xor(reason, o, 64));
+        if (lane == 0 && reason) atomicMin(bad, (unsigned long long)c * 4ull + (unsigned long long)reason);
+    }
+}
+static __global__ __launch_bounds__(256) void k_check_ellipse(int n, const double* a, const double* b,
+                                                              unsigned long long* bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(__builtin_isfinite(a[i]) && __builtin_isfinite(b[i])))
+        atomicMin(bad, (unsigned long long)i * 4ull + kBadAxis);
+}
+
 static __global__ __launch_bounds__(256) void k_bspline_curve_setup(BSplineDev B, const int64_t* ioff, int normalize,
                                                              double* curve_scale, double* int_scale,
                                                              int32_t* int_curve) {

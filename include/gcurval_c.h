@@ -25,7 +25,9 @@ typedef enum gcv_status {
     GCV_ERR_OOM = 2,         /* arena exhausted */
     GCV_ERR_HIP = 3,         /* HIP runtime error */
     GCV_ERR_UNSUPPORTED = 4, /* option combination not built */
-    GCV_ERR_STATE = 5        /* call order (e.g. sampling before a build) */
+    GCV_ERR_STATE = 5,       /* call order (e.g. sampling before a build) */
+    GCV_ERR_DATA = 6         /* batch data: a non-finite knot, control point or semi-axis, or
+                                knots that decrease within a curve (checked at table creation) */
 } gcv_status;
 
 /* Mirrors GcurvalEnum (gcurval/base/util.h:71-77). */
