@@ -196,10 +196,15 @@ class DeviceArray:
         self.ptr = None
 
 
-class Context:
-    """One HIP device, one stream, one arena (arena_bytes=0: hipMalloc per allocation)."""
+ARENA_POOL = (1 << 64) - 1  # GCV_ARENA_POOL: an arena that grows by hipMalloc chunks
 
-    def __init__(self, device: int = 0, arena_bytes: int = 0):
+
+class Context:
+    """One HIP device, one stream, one arena: by default a pool that grows by
+    hipMalloc chunks (ARENA_POOL); arena_bytes=0: hipMalloc per allocation;
+    any other value: one fixed chunk of that size."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = ARENA_POOL):
         L = lib()
         h = C.c_void_p()
         rc = L.gcv_ctx_create(device, arena_bytes, C.byref(h))

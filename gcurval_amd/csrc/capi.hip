@@ -75,10 +75,12 @@ struct gcv_ctx {
     hipEvent_t fork_ev = nullptr;
     hipEvent_t join_ev[kAuxStreams] = {};
     std::string err;
-    // arena: one hipMalloc block, first-fit with coalescing (256-byte aligned)
-    char* arena = nullptr;
+    // arena: hipMalloc chunks, first-fit with coalescing inside a chunk
+    // (256-byte aligned); one fixed chunk, or a pool that adds chunks
+    bool pooled = false, grow = false;
+    std::map<uintptr_t, size_t> chunks;             // chunk base -> size
     size_t cap = 0, used = 0;
-    std::map<size_t, size_t> free_blocks;           // offset -> size
+    std::map<uintptr_t, size_t> free_blocks;        // address -> size
     std::unordered_map<void*, size_t> live;         // pointer -> size
     // timing
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -224,6 +226,18 @@ void gcv_opts_default(gcv_opts* o) {
     o->newton_iters = 0;
 }
 
+// One more arena chunk (hipMalloc, 256-byte multiple) on the free list.
+constexpr size_t kPoolChunk = (size_t)256 << 20;  // growth step of a pool context
+static int add_chunk(gcv_ctx* c, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return set_err(c, GCV_ERR_OOM, "hipMalloc of a %zu-byte chunk failed", bytes);
+    c->chunks[(uintptr_t)p] = bytes;
+    c->free_blocks[(uintptr_t)p] = bytes;
+    c->cap += bytes;
+    return GCV_OK;
+}
+
 int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
     if (!out) return GCV_ERR_INVALID;
     *out = nullptr;
@@ -250,14 +264,14 @@ int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out) {
         ctx_release(c);
         return GCV_ERR_HIP;
     }
-    if (arena_bytes > 0) {
-        arena_bytes = (arena_bytes + 255) & ~(size_t)255;
-        if (hipMalloc((void**)&c->arena, arena_bytes) != hipSuccess) {
+    if (arena_bytes == GCV_ARENA_POOL) {
+        c->pooled = c->grow = true;  // chunks are added on demand
+    } else if (arena_bytes > 0) {
+        c->pooled = true;
+        if (add_chunk(c, arena_bytes) != GCV_OK) {
             ctx_release(c);
             return GCV_ERR_OOM;
         }
-        c->cap = arena_bytes;
-        c->free_blocks[0] = arena_bytes;
     }
     *out = c;
     return GCV_OK;
@@ -283,10 +297,11 @@ int gcv_ctx_destroy(gcv_ctx* c) {
     harvest(c);
     for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
     c->event_pool.clear();
-    if (c->arena) hipFree(c->arena);
+    if (c->pooled)
+        for (auto& kv : c->chunks) hipFree((void*)kv.first);
     else
         for (auto& kv : c->live) hipFree(kv.first);
-    c->arena = nullptr;
+    c->chunks.clear();
     ctx_release(c);
     return GCV_OK;
 }
@@ -299,7 +314,7 @@ int gcv_alloc(gcv_ctx* c, size_t bytes, void** dptr) {
     *dptr = nullptr;
     if (bytes == 0) bytes = 1;
     bytes = (bytes + 255) & ~(size_t)255;
-    if (!c->arena) {
+    if (!c->pooled) {
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess)
             return set_err(c, GCV_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
@@ -308,17 +323,25 @@ int gcv_alloc(gcv_ctx* c, size_t bytes, void** dptr) {
         *dptr = p;
         return GCV_OK;
     }
-    for (auto it = c->free_blocks.begin(); it != c->free_blocks.end(); ++it) {
-        if (it->second >= bytes) {
-            size_t off = it->first, sz = it->second;
-            c->free_blocks.erase(it);
-            if (sz > bytes) c->free_blocks[off + bytes] = sz - bytes;
-            void* p = c->arena + off;
-            c->live[p] = bytes;
-            c->used += bytes;
-            *dptr = p;
-            return GCV_OK;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        for (auto it = c->free_blocks.begin(); it != c->free_blocks.end(); ++it) {
+            if (it->second >= bytes) {
+                const uintptr_t at = it->first;
+                const size_t sz = it->second;
+                c->free_blocks.erase(it);
+                if (sz > bytes) c->free_blocks[at + bytes] = sz - bytes;
+                void* p = (void*)at;
+                c->live[p] = bytes;
+                c->used += bytes;
+                *dptr = p;
+                return GCV_OK;
+            }
         }
+        if (!c->grow || attempt > 0) break;
+        // a growing pool adds a chunk of at least kPoolChunk bytes (a
+        // hipMalloc: refused while the stream is captured)
+        if (int rc = capture_guard(c, "gcv_alloc growing the pool")) return rc;
+        if (int rc = add_chunk(c, std::max(bytes, kPoolChunk))) return rc;
     }
     return set_err(c, GCV_ERR_OOM, "arena exhausted: %zu bytes requested, %zu of %zu in use", bytes, c->used,
                    c->cap);
@@ -333,7 +356,7 @@ int gcv_free(gcv_ctx* c, void* p) {
     size_t sz = it->second;
     c->live.erase(it);
     c->used -= sz;
-    if (!c->arena) {
+    if (!c->pooled) {
         if (c->capturing) {  // keep the bookkeeping consistent: refuse before forgetting the block
             c->live[p] = sz;
             c->used += sz;
@@ -343,16 +366,20 @@ int gcv_free(gcv_ctx* c, void* p) {
         HIPCHK(c, hipFree(p));
         return GCV_OK;
     }
-    // stream-ordered reuse: later work on the same stream may take the block
-    size_t off = (size_t)((char*)p - c->arena);
+    // stream-ordered reuse: later work on the same stream may take the
+    // block; neighbours merge only inside the block's chunk (separate
+    // hipMalloc chunks may happen to be adjacent)
+    const uintptr_t off = (uintptr_t)p;
+    auto ch = std::prev(c->chunks.upper_bound(off));
+    const uintptr_t lo = ch->first, hi = ch->first + ch->second;
     auto nx = c->free_blocks.lower_bound(off);
-    if (nx != c->free_blocks.end() && nx->first == off + sz) {
+    if (nx != c->free_blocks.end() && nx->first == off + sz && nx->first < hi) {
         sz += nx->second;
         nx = c->free_blocks.erase(nx);
     }
     if (nx != c->free_blocks.begin()) {
         auto pv = std::prev(nx);
-        if (pv->first + pv->second == off) {
+        if (pv->first >= lo && pv->first + pv->second == off) {
             pv->second += sz;
             return GCV_OK;
         }

@@ -38,7 +38,7 @@ inline void check(int rc, const char* what, gcv_ctx* c = nullptr) {
 inline gcv_ctx* default_context() {
     static gcv_ctx* ctx = [] {
         gcv_ctx* c = nullptr;
-        check(gcv_ctx_create(0, 0, &c), "gcv_ctx_create");
+        check(gcv_ctx_create(0, GCV_ARENA_POOL, &c), "gcv_ctx_create");
         return c;
     }();
     return ctx;

@@ -62,7 +62,14 @@ void gcv_opts_default(gcv_opts* o);
 
 /* ---- context, arena, copies (replaces gMalloc/gFree/gFree_s over RMM or
  * cudaMalloc, gcurval/detail/cuda/util.h:41-71, and the cc()/cet() checks
- * that exit(-1), util.h:14-39) ---- */
+ * that exit(-1), util.h:14-39) ----
+ * arena_bytes: 0 = one hipMalloc per gcv_alloc (gcv_free synchronises the
+ * stream and frees); GCV_ARENA_POOL = a pool that grows by hipMalloc chunks of
+ * at least 256 MiB, kept until the context is destroyed (the RMM pool of the
+ * reference's default build); any other value = one fixed chunk of that size.
+ * In both arena forms gcv_free returns the block for reuse by later work on
+ * the context's stream, without a synchronisation. */
+#define GCV_ARENA_POOL ((size_t)-1)
 int gcv_ctx_create(int device, size_t arena_bytes, gcv_ctx** out);
 int gcv_ctx_destroy(gcv_ctx* ctx);
 const char* gcv_last_error(const gcv_ctx* ctx);

@@ -365,6 +365,36 @@ def test_max_depth_one_refines_once(ctx):
         assert st["max_stack"] <= 128
 
 
+def test_pool_context_grows_and_reuses():
+    # the default context is a pool: chunks of >= 256 MiB are added on demand,
+    # freed blocks are reused, neighbours coalesce inside their chunk only
+    MiB = 1 << 20
+    with G.Context(0) as c:
+        used0, cap0 = c.arena_stats()
+        assert used0 == 0 and cap0 == 0
+        a = c.empty(100 * MiB // 8, np.float64)
+        used, cap = c.arena_stats()
+        assert used == 100 * MiB and cap == 256 * MiB
+        big = c.empty(300 * MiB // 8, np.float64)  # does not fit the first chunk: a second one
+        used, cap = c.arena_stats()
+        assert used == 400 * MiB and cap == 256 * MiB + 300 * MiB
+        x = np.arange(1000, dtype=np.float64)
+        small = c.array(x)
+        np.testing.assert_array_equal(small.download(), x)
+        a.free()
+        small.free()
+        b2 = c.empty(200 * MiB // 8, np.float64)  # the first chunk, coalesced again, takes it
+        used, cap = c.arena_stats()
+        assert used == 500 * MiB and cap == 256 * MiB + 300 * MiB
+        for buf in (big, b2):
+            buf.free()
+        assert c.arena_stats()[0] == 0
+        b = W.bspline_batch(10, seed=4, n_ctrl_range=(8, 40))
+        with build(c, b, normalize=True) as eng:
+            Lt, _ = O.Truth(b).lengths()
+            assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
+
+
 def test_arena_context(ctx):
     with G.Context(0, arena_bytes=64 << 20) as c2:
         b = W.bspline_batch(10, seed=4, n_ctrl_range=(8, 40))
