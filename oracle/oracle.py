@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# GCV_ORACLE_LIB selects another build of the same source (the sanitized one)
+LIB_PATH = os.environ.get("GCV_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OrcBatch(C.Structure):
