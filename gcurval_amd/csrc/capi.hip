@@ -556,6 +556,11 @@ struct gcv_table {
     double* curve_scale = nullptr;
     double* int_scale = nullptr;
     int32_t* int_curve = nullptr;
+    // cumulative scan tiles (B-spline tables): status words, aggregates, prefixes
+    unsigned long long* scan_status = nullptr;
+    double* scan_agg = nullptr;
+    double* scan_prefix = nullptr;
+    unsigned scan_epoch = 0;
     double* coef64 = nullptr;             // FP64 builds: [n_int * cstride]
     int cstride = 1;                      // coefficient values per interval (max p * D over the batch)
     float* coef32 = nullptr;
@@ -783,11 +788,21 @@ extern "C" int gcv_table_create_bspline(gcv_ctx* c, const gcv_bspline_batch* B, 
     }
     if ((rc = table_alloc_common(t))) { table_free(t); return rc; }
     if ((rc = gcv_copy_h2d(c, t->ioff, ioff.data(), sizeof(int64_t) * (n + 1)))) { table_free(t); return rc; }
+    const int64_t n_tiles = (t->n_int + dev::kScanTile - 1) / dev::kScanTile;
     if ((rc = talloc(t, &t->curve_scale, n)) || (rc = talloc(t, &t->int_scale, t->n_int)) ||
         (rc = talloc(t, &t->int_curve, t->n_int)) || (rc = talloc(t, &t->u0, t->n_int)) ||
-        (rc = talloc(t, &t->h, t->n_int))) {
+        (rc = talloc(t, &t->h, t->n_int)) || (rc = talloc(t, &t->scan_status, std::max<int64_t>(n_tiles, 1))) ||
+        (rc = talloc(t, &t->scan_agg, std::max<int64_t>(n_tiles, 1))) ||
+        (rc = talloc(t, &t->scan_prefix, std::max<int64_t>(n_tiles, 1)))) {
         table_free(t);
         return rc;
+    }
+    // status words start at epoch 0, which no build uses
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3((unsigned)((std::max<int64_t>(n_tiles, 1) + 255) / 256)), dim3(256), 0,
+                       c->stream, t->scan_status, (int)std::max<int64_t>(n_tiles, 1), 0ull);
+    if (hipGetLastError() != hipSuccess) {
+        table_free(t);
+        return set_err(c, GCV_ERR_HIP, "scan status fill failed");
     }
     for (auto& kv : by_class) t->cstride = std::max(t->cstride, (kv.first / 10) * (kv.first % 10));
     // FP32 builds keep FP32 coefficients only (the sampler's FP64 step evaluates them in FP64)
@@ -944,13 +959,28 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedP
     return hipSuccess;
 }
 
+// Cumulative scan of the interval lengths (single pass, decoupled look-back;
+// time set by the interval count whatever the curve lengths).
 static hipError_t build_scan(gcv_table* t) {
     gcv_ctx* c = t->ctx;
-    if (t->kind == KIND_ELLIPSE || t->n_curve == 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((t->n_curve + 3) / 4, 65536);  // a wave per curve
-    LAUNCH_OR_RETURN("k_curve_scan", dev::k_curve_scan, dim3((unsigned)blocks), dim3(256), 0, c->stream, t->n_curve,
-                     t->ioff, t->int_cum, t->curve_len);
+    if (t->kind == KIND_ELLIPSE || t->n_curve == 0 || t->n_int == 0) return hipSuccess;
+    const int64_t n_tiles = (t->n_int + dev::kScanTile - 1) / dev::kScanTile;
+    if (++t->scan_epoch == 0) t->scan_epoch = 1;  // epoch 0 marks never-written status words
+    LAUNCH_OR_RETURN("k_fill_stats", dev::k_fill_u64, dim3(1), dim3(64), 0, c->stream, t->stats + dev::kScanTileWord,
+                     2, 0ull);
+    const dev::ScanTiles T{t->scan_status, t->scan_agg, t->scan_prefix};
+    LAUNCH_OR_RETURN("k_seg_scan", dev::k_seg_scan, dim3((unsigned)n_tiles), dim3(256), 0, c->stream, t->n_int,
+                     (const int32_t*)t->int_curve, t->int_cum, t->curve_len, T, t->scan_epoch, t->stats);
     return hipSuccess;
+}
+
+// A scan whose look-back gave up (it never does when every block runs to
+// completion) leaves the flag word set: the table's lengths are then wrong.
+static int scan_status(gcv_table* t) {
+    if (t->kind == KIND_ELLIPSE) return GCV_OK;
+    unsigned long long fail = 0;
+    if (int rc = gcv_copy_d2h(t->ctx, &fail, t->stats + dev::kScanFailWord, sizeof(fail))) return rc;
+    return fail ? set_err(t->ctx, GCV_ERR_HIP, "cumulative scan stalled waiting for a tile") : GCV_OK;
 }
 
 // Level-0 tolerance of the plain error test, tol / n_subdivided (the device
@@ -1201,7 +1231,8 @@ extern "C" int gcv_table_lengths(gcv_table* t, double* host_out) {
     if (t) use_device(t->ctx);
     if (!t || (!host_out && t->n_curve > 0)) return GCV_ERR_INVALID;
     if (!t->built) return set_err(t->ctx, GCV_ERR_STATE, "table not built");
-    return gcv_copy_d2h(t->ctx, host_out, t->curve_len, sizeof(double) * t->n_curve);
+    if (int rc = gcv_copy_d2h(t->ctx, host_out, t->curve_len, sizeof(double) * t->n_curve)) return rc;
+    return scan_status(t);
 }
 
 extern "C" int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** root_prefix,

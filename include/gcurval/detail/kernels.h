@@ -45,7 +45,10 @@ constexpr int kStatWords = kStatSlots * kStatStride;
 // whose tile outgrew the small task stack (re-run with the full one).
 constexpr int kQueueStride = 2;
 constexpr int kQueueClasses = 16;
-constexpr int kStatBufWords = kStatWords + kQueueStride * kQueueClasses;
+// then the cumulative scan's tile counter and its stall flag
+constexpr int kScanTileWord = kStatWords + kQueueStride * kQueueClasses;
+constexpr int kScanFailWord = kScanTileWord + 1;
+constexpr int kStatBufWords = kScanFailWord + 1;
 
 template <typename Dt>
 struct SubdivArgs {
@@ -632,6 +635,179 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
             }
         }
         if (lane == 0) curve_len[c] = carry;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cumulative interval lengths of every curve in one pass over all intervals
+// (a segmented inclusive FP64 scan, curves as segments), in place on
+// int_cum, and each curve's length from its last interval. Its time depends
+// on the interval count only, not on how the intervals split into curves.
+// Single pass with decoupled look-back: a block takes the next tile of
+// kScanTile intervals from a counter (so every tile it waits for belongs to
+// a block that is already running), scans it, publishes its aggregate (the
+// sum since its last curve start) and then, once the carry from the tiles
+// before it is known, its inclusive prefix; the carry is the sum of
+// predecessor aggregates back to the first one that holds a curve start or
+// already has its prefix, added left to right (the prefixes are sums in that
+// same order, so the result does not depend on which tiles had published
+// their prefix yet: builds are bit-reproducible). Publication is a value
+// store followed by a release
+// store of the tile's status word {epoch, kind, has-start}; the epoch (one
+// per build) tells this build's words from stale ones, so the status array
+// needs no clearing. A wait gives up after kScanMaxSpin polls and raises the
+// stall flag (the host then reports the build as failed) instead of hanging.
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 8;                      // intervals per thread
+constexpr int kScanTile = 256 * kScanItems;        // intervals per block tile
+constexpr unsigned kScanMaxSpin = 1u << 20;
+enum : uint64_t { kScanAgg = 1, kScanPrefix = 2, kScanHead = 4 };
+
+struct ScanTiles {
+    unsigned long long* status;  // [n_tiles]: epoch << 32 | kind | has-start
+    double* agg;                 // [n_tiles] sum since the tile's last curve start
+    double* prefix;              // [n_tiles] inclusive prefix of the tile's open curve
+};
+
+// (has a curve start, value) pairs under the segmented sum
+struct SegVal {
+    int f;
+    double v;
+};
+__device__ __forceinline__ SegVal seg_add(SegVal a, SegVal b) { return SegVal{a.f | b.f, b.f ? b.v : a.v + b.v}; }
+
+__device__ __forceinline__ SegVal wave_seg_scan_incl(SegVal x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int f = __shfl_up(x.f, o, 64);
+        const double v = shfl_up_d(x.v, o);
+        if (lane >= o) x = seg_add(SegVal{f, v}, x);
+    }
+    return x;
+}
+
+static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_t* int_curve, double* int_cum,
+                                                         double* curve_len, ScanTiles T, unsigned epoch,
+                                                         unsigned long long* words) {
+    __shared__ SegVal s_wave[4];
+    __shared__ int64_t s_tile;
+    __shared__ double s_carry;
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(words + kScanTileWord, 1ull);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t i0 = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    // this thread's intervals: lengths, curve starts, and curve ends
+    double x[kScanItems];
+    int head[kScanItems];
+    int32_t cur[kScanItems + 2];  // curves of i0 - 1 .. i0 + kScanItems
+#pragma unroll
+    for (int k = -1; k <= kScanItems; k++) {
+        const int64_t i = i0 + k;
+        cur[k + 1] = (i >= 0 && i < n) ? int_curve[i] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = i0 + k;
+        x[k] = i < n ? int_cum[i] : 0.0;
+        head[k] = i < n && cur[k + 1] != cur[k];
+    }
+    // serial scan of the thread's items
+    SegVal t{0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        t = seg_add(t, SegVal{head[k], x[k]});
+        x[k] = t.v;
+    }
+    // block scan of the thread totals
+    const SegVal incl = wave_seg_scan_incl(t);
+    const int fx = __shfl_up(incl.f, 1, 64);
+    const double vx = shfl_up_d(incl.v, 1);
+    SegVal excl = lane ? SegVal{fx, vx} : SegVal{0, 0.0};  // within the wave
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    SegVal before{0, 0.0}, tile_agg{0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q < w) before = seg_add(before, s_wave[q]);
+        tile_agg = seg_add(tile_agg, s_wave[q]);
+    }
+    excl = seg_add(before, excl);  // from the tile start to this thread
+    // publish, look back (wave 0: 64 predecessors per poll), publish the prefix
+    if (w == 0) {
+        const uint64_t ep = (uint64_t)epoch << 32, hf = tile_agg.f ? kScanHead : 0;
+        if (lane == 0) {
+            T.agg[tile] = tile_agg.v;
+            if (tile_agg.f) T.prefix[tile] = tile_agg.v;
+            __hip_atomic_store(T.status + tile, ep | hf | (tile_agg.f ? kScanPrefix : kScanAgg), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // back to the nearest tile whose prefix is out (a tile with a curve
+        // start publishes it at once; tile 0 starts curve 0), with every tile
+        // after it at least aggregated ...
+        int64_t hi = tile - 1, j = -1;  // window: tiles hi - 63 .. hi, lane l polls hi - l
+        bool stalled = false;
+        unsigned spin = 0;
+        while (hi >= 0) {
+            const int64_t q = hi - lane;
+            unsigned long long st = 0;
+            if (q >= 0) st = __hip_atomic_load(T.status + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const bool mine = (st >> 32) == epoch;
+            const uint64_t mp = ballot(q >= 0 && mine && (st & kScanPrefix));
+            const uint64_t mr = ballot(q < 0 || (mine && (st & (kScanAgg | kScanPrefix))));
+            if (mp) {
+                const int l = __ffsll((unsigned long long)mp) - 1;  // the nearest such tile
+                const uint64_t after = (1ull << l) - 1ull;
+                if ((mr & after) == after) {
+                    j = hi - l;
+                    break;
+                }
+            } else if (mr == ~0ull) {
+                hi -= 64;  // all aggregated, no prefix yet: further back
+                spin = 0;
+                continue;
+            }
+            if (++spin >= kScanMaxSpin) {  // stalled: flag the build, leave the carry short
+                if (lane == 0) atomicMax(words + kScanFailWord, 1ull);
+                stalled = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // ... then forward from it: the carry is the left-to-right sum of the
+        // aggregates whatever the timing, so every build rounds it the same way
+        double carry = 0.0;
+        if (!stalled && j >= 0) {
+            carry = T.prefix[j];
+            for (int64_t b0 = j + 1; b0 < tile; b0 += 64) {
+                const int64_t q = b0 + lane;
+                const double v = q < tile ? T.agg[q] : 0.0;
+                const int cnt = (int)(tile - b0 < 64 ? tile - b0 : 64);
+                for (int u = 0; u < cnt; u++) carry += readlane_d(v, u);
+            }
+        }
+        if (lane == 0) {
+            if (!tile_agg.f) {
+                T.prefix[tile] = carry + tile_agg.v;
+                __hip_atomic_store(T.status + tile, ep | kScanPrefix, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_carry = carry;
+        }
+    }
+    __syncthreads();
+    const SegVal base = seg_add(SegVal{0, s_carry}, excl);
+    int seen = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = i0 + k;
+        seen |= head[k];
+        const double v = seen ? x[k] : base.v + x[k];
+        if (i < n) {
+            int_cum[i] = v;
+            if (cur[k + 2] != cur[k + 1]) curve_len[cur[k + 1]] = v;  // last interval of its curve
+        }
     }
 }
 
