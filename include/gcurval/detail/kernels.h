@@ -647,16 +647,21 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
 // kScanTile intervals from a counter (so every tile it waits for belongs to
 // a block that is already running), scans it, publishes its aggregate (the
 // sum since its last curve start) and then, once the carry from the tiles
-// before it is known, its inclusive prefix; the carry is the sum of
-// predecessor aggregates back to the first one that holds a curve start or
-// already has its prefix, added left to right (the prefixes are sums in that
-// same order, so the result does not depend on which tiles had published
-// their prefix yet: builds are bit-reproducible). Publication is a value
-// store followed by a release
-// store of the tile's status word {epoch, kind, has-start}; the epoch (one
-// per build) tells this build's words from stale ones, so the status array
-// needs no clearing. A wait gives up after kScanMaxSpin polls and raises the
-// stall flag (the host then reports the build as failed) instead of hanging.
+// before it is known, its inclusive prefix; the carry is the prefix of the
+// nearest tile that has one (a tile holding a curve start has it at once)
+// plus the aggregates after it, added left to right (the prefixes are sums
+// in that same order, so the result does not depend on which tiles had
+// published yet: builds are bit-reproducible).
+// Tile words are device-coherent atomics (relaxed, no cache maintenance):
+// a value is stored, the store is waited for, then the tile's status word
+// {epoch, kind, has-start} is stored; a reader loads the status, then the
+// value. The epoch (one per build) tells this build's words from stale
+// ones, so the status array needs no clearing. A wait gives up after
+// kScanMaxSpin polls and raises the stall flag (the host then reports the
+// build as failed) instead of hanging.
+// The tile moves through LDS twice: coalesced loads in, each thread's
+// kScanItems consecutive intervals out of LDS for its serial scan, and the
+// same way back for coalesced stores.
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 8;                      // intervals per thread
 constexpr int kScanTile = 256 * kScanItems;        // intervals per block tile
@@ -687,31 +692,63 @@ __device__ __forceinline__ SegVal wave_seg_scan_incl(SegVal x) {
     return x;
 }
 
+__device__ __forceinline__ void scan_put(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double scan_get(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// the value stores before it have completed when the status store issues
+__device__ __forceinline__ void scan_publish(unsigned long long* st, unsigned long long v) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __hip_atomic_store(st, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS slot of tile element e: one pad word per 32 keeps the blocked reads
+// (thread t, elements 8t .. 8t + 7) free of bank conflicts
+__device__ __forceinline__ int scan_slot(int e) { return e + (e >> 5); }
+
 static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_t* int_curve, double* int_cum,
                                                          double* curve_len, ScanTiles T, unsigned epoch,
                                                          unsigned long long* words) {
+    __shared__ double s_x[kScanTile + kScanTile / 32];
+    __shared__ int32_t s_c[kScanTile + kScanTile / 32];
     __shared__ SegVal s_wave[4];
     __shared__ int64_t s_tile;
     __shared__ double s_carry;
-    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
-    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(words + kScanTileWord, 1ull);
+    __shared__ int32_t s_edge[2];  // curves of the intervals before and after the tile
+    const int tid = (int)threadIdx.x, lane = lane_id(), w = tid >> 6;
+    if (tid == 0) s_tile = (int64_t)atomicAdd(words + kScanTileWord, 1ull);
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t i0 = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    // this thread's intervals: lengths, curve starts, and curve ends
-    double x[kScanItems];
-    int head[kScanItems];
-    int32_t cur[kScanItems + 2];  // curves of i0 - 1 .. i0 + kScanItems
-#pragma unroll
-    for (int k = -1; k <= kScanItems; k++) {
-        const int64_t i = i0 + k;
-        cur[k + 1] = (i >= 0 && i < n) ? int_curve[i] : -1;
-    }
+    const int64_t t0 = tile * kScanTile;
+    // coalesced loads into LDS
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
-        const int64_t i = i0 + k;
-        x[k] = i < n ? int_cum[i] : 0.0;
-        head[k] = i < n && cur[k + 1] != cur[k];
+        const int e = k * 256 + tid;
+        const int64_t i = t0 + e;
+        s_x[scan_slot(e)] = i < n ? int_cum[i] : 0.0;
+        s_c[scan_slot(e)] = i < n ? int_curve[i] : -1;
+    }
+    if (tid == 0) s_edge[0] = t0 > 0 ? int_curve[t0 - 1] : -1;
+    if (tid == 1) s_edge[1] = t0 + kScanTile < n ? int_curve[t0 + kScanTile] : -1;
+    __syncthreads();
+    // this thread's intervals: lengths, curve starts, and curve ends
+    double x[kScanItems];
+    int head[kScanItems], tail[kScanItems];
+    {
+        const int e0 = tid * kScanItems;
+        int32_t prev = e0 > 0 ? s_c[scan_slot(e0 - 1)] : s_edge[0];
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            const int e = e0 + k;
+            const int32_t c = s_c[scan_slot(e)];
+            const int32_t nx = e + 1 < kScanTile ? s_c[scan_slot(e + 1)] : s_edge[1];
+            x[k] = s_x[scan_slot(e)];
+            head[k] = c >= 0 && c != prev;
+            tail[k] = c >= 0 && c != nx;
+            prev = c;
+        }
     }
     // serial scan of the thread's items
     SegVal t{0, 0.0};
@@ -738,21 +775,19 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
     if (w == 0) {
         const uint64_t ep = (uint64_t)epoch << 32, hf = tile_agg.f ? kScanHead : 0;
         if (lane == 0) {
-            T.agg[tile] = tile_agg.v;
-            if (tile_agg.f) T.prefix[tile] = tile_agg.v;
-            __hip_atomic_store(T.status + tile, ep | hf | (tile_agg.f ? kScanPrefix : kScanAgg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            scan_put(T.agg + tile, tile_agg.v);
+            if (tile_agg.f) scan_put(T.prefix + tile, tile_agg.v);
+            scan_publish(T.status + tile, ep | hf | (tile_agg.f ? kScanPrefix : kScanAgg));
         }
-        // back to the nearest tile whose prefix is out (a tile with a curve
-        // start publishes it at once; tile 0 starts curve 0), with every tile
-        // after it at least aggregated ...
+        // back to the nearest tile whose prefix is out (tile 0 starts curve
+        // 0, so there is one), with every tile after it aggregated ...
         int64_t hi = tile - 1, j = -1;  // window: tiles hi - 63 .. hi, lane l polls hi - l
         bool stalled = false;
         unsigned spin = 0;
         while (hi >= 0) {
             const int64_t q = hi - lane;
             unsigned long long st = 0;
-            if (q >= 0) st = __hip_atomic_load(T.status + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (q >= 0) st = __hip_atomic_load(T.status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const bool mine = (st >> 32) == epoch;
             const uint64_t mp = ballot(q >= 0 && mine && (st & kScanPrefix));
             const uint64_t mr = ballot(q < 0 || (mine && (st & (kScanAgg | kScanPrefix))));
@@ -775,23 +810,21 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // ... then forward from it: the carry is the left-to-right sum of the
-        // aggregates whatever the timing, so every build rounds it the same way
+        // ... then forward from it, left to right
         double carry = 0.0;
         if (!stalled && j >= 0) {
-            carry = T.prefix[j];
+            carry = scan_get(T.prefix + j);
             for (int64_t b0 = j + 1; b0 < tile; b0 += 64) {
                 const int64_t q = b0 + lane;
-                const double v = q < tile ? T.agg[q] : 0.0;
+                const double v = q < tile ? scan_get(T.agg + q) : 0.0;
                 const int cnt = (int)(tile - b0 < 64 ? tile - b0 : 64);
                 for (int u = 0; u < cnt; u++) carry += readlane_d(v, u);
             }
         }
         if (lane == 0) {
             if (!tile_agg.f) {
-                T.prefix[tile] = carry + tile_agg.v;
-                __hip_atomic_store(T.status + tile, ep | kScanPrefix, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                scan_put(T.prefix + tile, carry + tile_agg.v);
+                scan_publish(T.status + tile, ep | kScanPrefix);
             }
             s_carry = carry;
         }
@@ -799,15 +832,22 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
     __syncthreads();
     const SegVal base = seg_add(SegVal{0, s_carry}, excl);
     int seen = 0;
+    {
+        const int e0 = tid * kScanItems;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            seen |= head[k];
+            const double v = seen ? x[k] : base.v + x[k];
+            s_x[scan_slot(e0 + k)] = v;
+            if (tail[k]) curve_len[s_c[scan_slot(e0 + k)]] = v;  // last interval of its curve
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
-        const int64_t i = i0 + k;
-        seen |= head[k];
-        const double v = seen ? x[k] : base.v + x[k];
-        if (i < n) {
-            int_cum[i] = v;
-            if (cur[k + 2] != cur[k + 1]) curve_len[cur[k + 1]] = v;  // last interval of its curve
-        }
+        const int e = k * 256 + tid;
+        const int64_t i = t0 + e;
+        if (i < n) int_cum[i] = s_x[scan_slot(e)];
     }
 }
 
