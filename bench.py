@@ -48,7 +48,7 @@ def f_node_bspline(p, D):
 
 
 # kernels reported per step; a name is a prefix of the launch names it sums
-KERNELS = ("k_subdivide", "k_seg_scan", "k_bspline_curve_setup", "k_bspline_coef", "k_sample", "k_fill_stats",
+KERNELS = ("k_subdivide", "k_curve_scan", "k_seg_scan", "k_bspline_curve_setup", "k_bspline_coef", "k_sample", "k_fill_stats",
            "k_seed_floor", "k_bucket", "k_mlp_predict", "k_block_curve")
 
 

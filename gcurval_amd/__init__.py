@@ -120,6 +120,7 @@ _SIGS = [
     ("gcv_graph_destroy", C.c_int, [C.c_void_p]),
     ("gcv_table_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("gcv_table_set_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("gcv_table_set_scan", C.c_int, [C.c_void_p, C.c_int]),
     ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
     ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -496,6 +497,10 @@ class ArclengthCalculateGpu:
 
     def set_launch(self, waves_per_block: int = 0, grid_pct: int = 100, bin_streams: int = 0):
         self.ctx._check(lib().gcv_table_set_launch(self.h, waves_per_block, grid_pct, bin_streams))
+
+    def set_scan(self, mode: int = 0):
+        """Cumulative-scan kernel: 0 by curve length, 1 a wave per curve, 2 the look-back scan."""
+        self.ctx._check(lib().gcv_table_set_scan(self.h, mode))
 
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)

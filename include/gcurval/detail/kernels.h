@@ -617,7 +617,7 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t c = gwave; c < n_curve; c += nwaves) {
         const int64_t b = ioff[c], e = ioff[c + 1];
-        double carry = 0.0;
+        double carry = 0.0, last = 0.0;
         // 4 x 64 lengths in flight per iteration, then scanned in order
         for (int64_t base = b; base < e; base += 256) {
             double v[4];
@@ -632,9 +632,13 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
                 const double w = seg_scan_incl(v[u], 6) + carry;
                 if (j < e) int_cum[j] = w;
                 carry = shfl_d(w, 63);
+                // the curve length is its last interval's value itself (lanes
+                // past the end sum the same terms in another order)
+                const int64_t k = e - 1 - (base + 64 * u);
+                if (k >= 0 && k < 64) last = shfl_d(w, (int)k);
             }
         }
-        if (lane == 0) curve_len[c] = carry;
+        if (lane == 0) curve_len[c] = last;
     }
 }
 
@@ -664,7 +668,8 @@ static __global__ __launch_bounds__(256) void k_curve_scan(int n_curve, const in
 // same way back for coalesced stores.
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 8;                      // intervals per thread
-constexpr int kScanTile = 256 * kScanItems;        // intervals per block tile
+constexpr int kScanThreads = 512;
+constexpr int kScanTile = kScanThreads * kScanItems;  // intervals per block tile
 constexpr unsigned kScanMaxSpin = 1u << 20;
 enum : uint64_t { kScanAgg = 1, kScanPrefix = 2, kScanHead = 4 };
 
@@ -708,12 +713,12 @@ __device__ __forceinline__ void scan_publish(unsigned long long* st, unsigned lo
 // (thread t, elements 8t .. 8t + 7) free of bank conflicts
 __device__ __forceinline__ int scan_slot(int e) { return e + (e >> 5); }
 
-static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_t* int_curve, double* int_cum,
+static __global__ __launch_bounds__(kScanThreads) void k_seg_scan(int64_t n, const int32_t* int_curve, double* int_cum,
                                                          double* curve_len, ScanTiles T, unsigned epoch,
                                                          unsigned long long* words) {
     __shared__ double s_x[kScanTile + kScanTile / 32];
     __shared__ int32_t s_c[kScanTile + kScanTile / 32];
-    __shared__ SegVal s_wave[4];
+    __shared__ SegVal s_wave[kScanThreads / 64];
     __shared__ int64_t s_tile;
     __shared__ double s_carry;
     __shared__ int32_t s_edge[2];  // curves of the intervals before and after the tile
@@ -725,7 +730,7 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
     // coalesced loads into LDS
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
-        const int e = k * 256 + tid;
+        const int e = k * kScanThreads + tid;
         const int64_t i = t0 + e;
         s_x[scan_slot(e)] = i < n ? int_cum[i] : 0.0;
         s_c[scan_slot(e)] = i < n ? int_curve[i] : -1;
@@ -766,7 +771,7 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
     __syncthreads();
     SegVal before{0, 0.0}, tile_agg{0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < kScanThreads / 64; q++) {
         if (q < w) before = seg_add(before, s_wave[q]);
         tile_agg = seg_add(tile_agg, s_wave[q]);
     }
@@ -845,7 +850,7 @@ static __global__ __launch_bounds__(256) void k_seg_scan(int64_t n, const int32_
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
-        const int e = k * 256 + tid;
+        const int e = k * kScanThreads + tid;
         const int64_t i = t0 + e;
         if (i < n) int_cum[i] = s_x[scan_slot(e)];
     }

@@ -305,6 +305,14 @@ int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* out);
    spread over that many streams and joined before the build returns, 0 = the
    default (3 for B-spline batches of several degree/dimension classes, else 1). */
 int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct, int bin_streams);
+/* Cumulative-scan kernel of B-spline builds (no counterpart in the reference,
+   which scans per curve, kernel/generic.inl.cuh:117-137): 0 = by curve length
+   (a wave per curve when every curve has at most 8,192 intervals and they
+   average at least 64, else the single-pass look-back scan over all
+   intervals), 1 = a wave per curve,
+   2 = the look-back scan. Same results either way up to FP64 rounding; each
+   is bit-reproducible from build to build. */
+int gcv_table_set_scan(gcv_table* t, int mode);
 
 /* Host-side helpers that run without a GPU (used by the CPU test suite). */
 /* Power-basis derivative coefficients of one B-spline span, as computed by

@@ -1,7 +1,10 @@
-"""The cumulative scan of interval lengths (k_seg_scan: one pass over all
-intervals with decoupled look-back, curves as segments) against closed forms
-and the oracle, on one curve of more than a million intervals and on many
-short ones, and its time against how the intervals split into curves.
+"""The cumulative scan of interval lengths against closed forms and the
+oracle, on one curve of more than a million intervals and on many short
+ones, and its time against how the intervals split into curves. Batches of
+curves of 64..8,192 intervals on average scan a curve per wave
+(k_curve_scan); the others take one pass over all intervals with decoupled
+look-back (k_seg_scan, curves as segments); the oracle test runs both on one
+batch.
 
 The per-curve scan it replaces (gcurval/detail/cuda/kernel/generic.inl.cuh:
 117-137 scans each curve's intervals) costs one serial chain per curve; here
@@ -55,18 +58,21 @@ def test_one_curve_of_1_2m_intervals(ctx):
     assert np.max(np.abs(interior - interior.mean())) <= 1e-9 * interior.mean()
 
 
-def test_many_curves_against_the_oracle(ctx):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_many_curves_against_the_oracle(ctx, mode):
+    # both scan kernels on the same batch (a wave per curve; the look-back pass)
     b = W.bspline_batch(300, seed=71, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(8, 3000), hairpin_frac=0.2)
-    print(f"[scan] 300 curves, {b.n_intervals} intervals", flush=True)
+    print(f"[scan] mode {mode}: 300 curves, {b.n_intervals} intervals", flush=True)
     ref = O.RefGpuPlain(b, O.opts(1, 3, normalize=True))
     ref.build()
     Lr, cumr, _, _ = ref.tables()
     with G.ArclengthCalculateGpu(ctx, b, precision=G.GCV_FP64, normalize=True) as eng:
+        eng.set_scan(mode)
         eng.createArclengthTable()
         cum, _, _, Lg, io = eng.tables()
         eng.createArclengthTable()
         cum2 = eng.tables()[0]
-    np.testing.assert_array_equal(cum, cum2)  # the look-back does not change the rounding from build to build
+    np.testing.assert_array_equal(cum, cum2)  # the same rounding from build to build
     assert np.max(np.abs(Lg - Lr) / Lr) <= 1e-10
     scale = np.repeat(Lr, np.diff(io))
     assert np.max(np.abs(cum - cumr) / scale) <= 1e-10
@@ -87,7 +93,9 @@ def test_time_follows_the_interval_count(ctx):
                 ctx.profile_reset()
                 eng.createArclengthTable()
                 ctx.sync()
-                t, n = ctx.profile_query("k_seg_scan")
+                t1, n1 = ctx.profile_query("k_seg_scan")
+                t2, n2 = ctx.profile_query("k_curve_scan")
+                t, n = t1 + t2, n1 + n2
                 assert n == 1
                 best = min(best, t)
             ctx.profile(False)
