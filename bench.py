@@ -325,6 +325,29 @@ def measure(G, ctx, batch, counts, steps, warmup, paths, weights=None):
         out[path] = {"ms": ms, "ms_per_step": ms / steps, "graph_ms_per_step": graph_ms / steps,
                      "kernels": kernels, "stats": st, "build_ms": build_ms,
                      "sample_ms": t_smp / steps, "lengths": lengths}
+        if batch.kind != "ellipse":
+            # sampling alone on this table under each Newton start (the
+            # step above used the table's default, the plain start)
+            by_start = {}
+            modes = (("plain", G.GCV_START_PLAIN), ("learned", G.GCV_START_LEARNED),
+                     ("quintic", G.GCV_START_QUINTIC))
+            for name, mode in modes:
+                if path != "learned" and name == "learned":
+                    continue
+                eng.set_newton_start(mode)
+                eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
+                ctx.sync()
+                ctx.profile(True)
+                ctx.profile_reset()
+                for _ in range(steps):
+                    eng.sample_device(d_off, n_total, None, True, d_out, None, uniform_closed=True)
+                ctx.sync()
+                t, n = ctx.profile_query("k_sample")
+                ctx.profile(False)
+                by_start[name] = t / n if n else None
+            eng.set_newton_start(G.GCV_START_PLAIN)
+            out[path]["sample_ms_by_newton_start"] = by_start
+            log(f"[bench] {path}: k_sample ms by Newton start {by_start}")
     eng.close()
     d_off.free()
     d_out.free()
@@ -343,7 +366,8 @@ def comparison(runs, n_int, n_total):
                 "build_intervals_per_sec": n_int / (r["build_ms"] * 1e-3),
                 "step_intervals_per_sec": n_int / (r["ms_per_step"] * 1e-3),
                 "step_samples_per_sec": n_total / (r["ms_per_step"] * 1e-3),
-                "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r["kernels"].items()}}
+                "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r["kernels"].items()},
+                "sample_ms_by_newton_start": r.get("sample_ms_by_newton_start")}
 
     return {"plain": side(pl), "learned": side(le),
             "build_speedup": pl["build_ms"] / le["build_ms"],

@@ -22,7 +22,7 @@ __all__ = [
     "lib", "load_library", "GcvError", "Opts", "Context", "DeviceArray",
     "EllipseBatch", "BSplineBatch", "ArclengthCalculateGpu",
     "gcurvalDevice", "gcurvalHost", "gcurvalReverseReturnUs", "gcurvalReverseReturnPoints",
-    "GCV_FP32", "GCV_FP64",
+    "GCV_FP32", "GCV_FP64", "GCV_START_PLAIN", "GCV_START_LEARNED", "GCV_START_QUINTIC",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -31,6 +31,8 @@ LIB_PATH = os.path.join(HERE, "libgcurval_amd.so")
 # GcurvalEnum (gcurval/base/util.h:71-77)
 gcurvalDevice, gcurvalHost, gcurvalReverseReturnUs, gcurvalReverseReturnPoints = 0, 1, 2, 3
 GCV_FP32, GCV_FP64 = 0, 1
+# Newton start of the B-spline sampler (gcv_table_set_newton_start)
+GCV_START_PLAIN, GCV_START_LEARNED, GCV_START_QUINTIC = 0, 1, 2
 
 _STATUS = {0: "GCV_OK", 1: "GCV_ERR_INVALID", 2: "GCV_ERR_OOM", 3: "GCV_ERR_HIP",
            4: "GCV_ERR_UNSUPPORTED", 5: "GCV_ERR_STATE", 6: "GCV_ERR_DATA"}
@@ -121,6 +123,9 @@ _SIGS = [
     ("gcv_table_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("gcv_table_set_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("gcv_table_set_scan", C.c_int, [C.c_void_p, C.c_int]),
+    ("gcv_table_set_newton_start", C.c_int, [C.c_void_p, C.c_int]),
+    ("gcv_table_start_labels", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gcv_table_set_iters_out", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gcv_mlp_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("gcv_mlp_destroy", C.c_int, [C.c_void_p]),
     ("gcv_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -501,6 +506,29 @@ class ArclengthCalculateGpu:
     def set_scan(self, mode: int = 0):
         """Cumulative-scan kernel: 0 by curve length, 1 a wave per curve, 2 the look-back scan."""
         self.ctx._check(lib().gcv_table_set_scan(self.h, mode))
+
+    def set_newton_start(self, mode: int = GCV_START_LEARNED):
+        """Newton start of B-spline samples: GCV_START_PLAIN (cubic Hermite, FP32
+        steps, FP64 step; the table's default), GCV_START_LEARNED (the
+        predictor's per-interval choice of the quintic start on learned
+        builds) or GCV_START_QUINTIC."""
+        self.ctx._check(lib().gcv_table_set_newton_start(self.h, mode))
+
+    def start_labels(self) -> np.ndarray:
+        """Per interval of the built table: log2 of the worst quintic-start error
+        over its roots (relative to root width)."""
+        n = self.n_intervals
+        d = self.ctx.empty(max(n, 1), np.float32)
+        try:
+            self.ctx._check(lib().gcv_table_start_labels(self.h, d.ptr))
+            return d.download()[:n]
+        finally:
+            d.free()
+
+    def set_iters_out(self, d_iters: "DeviceArray | None"):
+        """Per-sample Newton step counts of the following samplings (FP32 steps
+        in the low nibble, FP64 steps in the high one); None turns it off."""
+        self.ctx._check(lib().gcv_table_set_iters_out(self.h, d_iters.ptr if d_iters is not None else None))
 
     def getAllCurveArclengths(self) -> np.ndarray:
         out = np.empty(self.n_curve, dtype=np.float64)

@@ -588,6 +588,10 @@ struct gcv_table {
     int32_t* rerun_list = nullptr;        // [n_int] intervals of seeded tiles that outgrew the small stack
     int seed_stack_cap = 0;               // task-stack entries per wave of the seeded bins kernel
     bool seeded = false;                  // last build used per-interval root counts
+    int8_t* start_ok = nullptr;           // [n_int] learned Newton-start decision (output 3 of the predictor)
+    bool has_start = false;               // start_ok holds the predictions of the last (learned) build
+    int newton_start = GCV_START_PLAIN;   // sampler Newton start (gcv_table_set_newton_start)
+    int8_t* iters_out = nullptr;          // optional per-sample Newton step counts (gcv_table_set_iters_out)
     int built_ls = 6;                     // log2 roots per interval of the last uniform build
     unsigned long long* stats = nullptr;
     std::vector<void*> owned;
@@ -665,6 +669,7 @@ static int table_alloc_common(gcv_table* t) {
     if ((rc = talloc(t, &t->stats, dev::kStatBufWords))) return rc;
     // seeded-build arrays (allocated once, so builds never allocate)
     if ((rc = talloc(t, &t->seed_k, t->n_int)) || (rc = talloc(t, &t->seed_floor, t->n_int)) ||
+        (rc = talloc(t, &t->start_ok, t->n_int)) ||
         (rc = talloc(t, &t->root_k, t->n_int)) || (rc = talloc(t, &t->root_off, t->n_int)) ||
         (rc = talloc(t, &t->bucket_root_chunk, dev::kBucketMaxWaves)) || (rc = talloc(t, &t->bucket_list, t->n_int)) ||
         (rc = talloc(t, &t->bucket_hist, dev::kMaxBins * dev::kBucketMaxWaves)) ||
@@ -1429,6 +1434,42 @@ extern "C" int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int s
     return GCV_OK;
 }
 
+extern "C" int gcv_table_set_newton_start(gcv_table* t, int mode) {
+    if (t) use_device(t->ctx);
+    if (!t) return GCV_ERR_INVALID;
+    if (mode < GCV_START_PLAIN || mode > GCV_START_QUINTIC)
+        return set_err(t->ctx, GCV_ERR_INVALID, "Newton start must be GCV_START_PLAIN, _LEARNED or _QUINTIC");
+    t->newton_start = mode;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_set_iters_out(gcv_table* t, int8_t* d_iters) {
+    if (t) use_device(t->ctx);
+    if (!t) return GCV_ERR_INVALID;
+    t->iters_out = d_iters;
+    return GCV_OK;
+}
+
+extern "C" int gcv_table_start_labels(gcv_table* t, float* d_out) {
+    if (t) use_device(t->ctx);
+    if (!t || (!d_out && t->n_int > 0)) return GCV_ERR_INVALID;
+    gcv_ctx* c = t->ctx;
+    if (!t->built) return set_err(c, GCV_ERR_STATE, "table not built");
+    if (t->kind != KIND_BSPLINE || t->o.precision != GCV_FP32 || t->o.gl_order != 3)
+        return set_err(c, GCV_ERR_INVALID, "start labels serve FP32 GL-3 B-spline tables");
+    dev::SampleArgs A{};
+    A.n_int = t->n_int;
+    A.root_prefix = t->root_prefix;
+    A.root_k = t->seeded ? t->root_k : nullptr;
+    A.root_off = t->seeded ? t->root_off : nullptr;
+    A.deep = t->deep;
+    A.log2_nsub = t->built_ls;
+    A.class_of_curve = t->class_of_curve;
+    hipError_t e = Launch<float, 3>::start_labels(cfg_of(c), t->uniform_class, bspline_dev(t), t->int_curve, A, d_out);
+    if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "start-label launch failed: %s", hipGetErrorString(e));
+    return GCV_OK;
+}
+
 extern "C" int gcv_table_set_scan(gcv_table* t, int mode) {
     if (t) use_device(t->ctx);
     if (!t) return GCV_ERR_INVALID;
@@ -1483,6 +1524,11 @@ static int sample_check(gcv_table* t, const int64_t* d_soff, int64_t n_total, do
     return GCV_OK;
 }
 
+// The quintic Newton start serves FP32 GL-3 builds that take the FP64 step.
+static bool f32_direct_ok(const gcv_table* t) {
+    return t->o.precision == GCV_FP32 && t->o.gl_order == 3 && !t->o.skip_fp64_step;
+}
+
 // Sampling with the block maps in blk_curve / blk_int (n_total / kSampleBlock
 // entries each, rounded up): the kernels of gcv_sample. `profiled`: the
 // launches are timed when the context profiles (event creation included).
@@ -1512,6 +1558,14 @@ static int sample_launch(gcv_table* t, const int64_t* d_soff, int64_t n_total, c
     A.max_depth = t->o.max_depth;
     A.newton_iters = t->o.newton_iters;
     A.fp64_step = t->o.skip_fp64_step ? 0 : 1;
+    // Newton start of B-spline samples (FP32 builds with the FP64 step)
+    A.start_mode = GCV_START_PLAIN;
+    if (t->kind == KIND_BSPLINE && f32_direct_ok(t)) {
+        if (t->newton_start == GCV_START_QUINTIC) A.start_mode = GCV_START_QUINTIC;
+        else if (t->newton_start == GCV_START_LEARNED && t->has_start) A.start_mode = GCV_START_LEARNED;
+    }
+    A.start_ok = t->start_ok;
+    A.out_iters = t->iters_out;
     A.out_u = d_out_u;
     A.out_seg = d_out_seg;
     A.class_of_curve = t->class_of_curve;
@@ -1758,11 +1812,11 @@ extern "C" int gcv_mlp_features(gcv_table* t, float* d_out) {
 // rows and safety floor from the setup kernel and stream them through the
 // network; ellipses compute features inside the prediction kernel.
 static hipError_t setup_and_predict(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw,
-                                    bool reset_stats) {
+                                    bool reset_stats, int8_t* d_start = nullptr) {
     gcv_ctx* c = t->ctx;
     const int64_t n = t->n_int;
     const dev::SeedOut O{d_k, d_raw, t->seed_floor, hedge, t->log2_nsub,
-                         (float)(t->o.tolerance / (double)(1 << t->log2_nsub))};
+                         (float)(t->o.tolerance / (double)(1 << t->log2_nsub)), d_start};
     if (t->kind == KIND_BSPLINE) {
         const SeedPredict sp{mlp_dev(m), O};
         return dispatch_build<SetupWithSeed>(t, reset_stats, &sp);
@@ -1794,10 +1848,12 @@ extern "C" int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t*
     if (!t || !m) return GCV_ERR_INVALID;
     gcv_ctx* c = t->ctx;
     if (m->ctx != c) return set_err(c, GCV_ERR_INVALID, "predictor and table belong to different contexts");
-    hipError_t e = setup_and_predict(m, t, hedge, t->seed_k, nullptr, true);
+    t->has_start = false;
+    hipError_t e = setup_and_predict(m, t, hedge, t->seed_k, nullptr, true, t->start_ok);
     if (e == hipSuccess) e = dispatch_build<SeededAfterSetup>(t, (const int8_t*)t->seed_k, d_cost);
     if (e != hipSuccess) return set_err(c, GCV_ERR_HIP, "learned-seed build failed: %s", hipGetErrorString(e));
     t->built = true;
+    t->has_start = true;
     return GCV_OK;
 }
 

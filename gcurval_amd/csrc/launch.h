@@ -62,6 +62,10 @@ struct Launch {
     // uniform_class when that is null.
     static hipError_t sample_bspline(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
                                      const dev::SampleArgs& A, const int32_t* blk_curve);
+    // Training labels of the learned Newton start (FP32 GL-3 builds only):
+    // per interval, log2 of the worst quintic-start error over its roots.
+    static hipError_t start_labels(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
+                                   const int32_t* int_curve, const dev::SampleArgs& A, float* out);
     // LDS per block and waves per block of the subdivision kernel.
     static void subdivide_shape(int stack_cap, int want_wpb, int* waves_per_block, size_t* lds_bytes);
 

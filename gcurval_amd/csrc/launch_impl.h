@@ -278,7 +278,7 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
 // Mixed-class B-spline sampler: the class is uniform within a curve, so a
 // wave diverges only where its samples straddle curves of different classes.
 template <typename Dt, int NGL, class M>
-__global__ __launch_bounds__(256) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A,
+__global__ __launch_bounds__(256, 8) void k_sample_bspline(BSplineTableDev T, int uniform_class, SampleArgs A,
                                                         const int32_t* blk_curve) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.n_total) return;
@@ -321,22 +321,137 @@ hipError_t Launch<Dt, NGL>::sample_bspline(const LaunchCfg& c, int uniform_class
     const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_bspline");
     const bool pos = A.pos != nullptr, seeded = A.root_k != nullptr;
-#define GCV_BSPLINE_SAMPLE(P, S)                                                                        \
-    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL, StaticMode<P, S>>), dim3((unsigned)blocks),          \
+#define GCV_BSPLINE_SAMPLE(P, S, ST)                                                                    \
+    hipLaunchKernelGGL((k_sample_bspline<Dt, NGL, StaticMode<P, S, ST>>), dim3((unsigned)blocks),      \
                        dim3(kSampleBlock), 0, c.stream, T, uniform_class, A, blk_curve)
+    if constexpr (sizeof(Dt) == sizeof(float)) {
+        if (A.start_mode != kStartPlain) {
+            // the learned / quintic Newton start has kernels of its own
+            if (pos && seeded) {
+                GCV_BSPLINE_SAMPLE(true, true, true);
+            } else if (pos) {
+                GCV_BSPLINE_SAMPLE(true, false, true);
+            } else if (seeded) {
+                GCV_BSPLINE_SAMPLE(false, true, true);
+            } else {
+                GCV_BSPLINE_SAMPLE(false, false, true);
+            }
+            hipError_t e = hipGetLastError();
+            prof_end(c.owner);
+            return e;
+        }
+    }
     if (pos && seeded) {
-        GCV_BSPLINE_SAMPLE(true, true);
+        GCV_BSPLINE_SAMPLE(true, true, false);
     } else if (pos) {
-        GCV_BSPLINE_SAMPLE(true, false);
+        GCV_BSPLINE_SAMPLE(true, false, false);
     } else if (seeded) {
-        GCV_BSPLINE_SAMPLE(false, true);
+        GCV_BSPLINE_SAMPLE(false, true, false);
     } else {
-        GCV_BSPLINE_SAMPLE(false, false);
+        GCV_BSPLINE_SAMPLE(false, false, false);
     }
 #undef GCV_BSPLINE_SAMPLE
     hipError_t e = hipGetLastError();
     prof_end(c.owner);
     return e;
+}
+
+// Training labels of the learned Newton start: per knot interval, log2 of
+// the worst error of the quintic Hermite start (relative to root width) at
+// the targets 1/4, 1/2 and 3/4 of each root's length, against the converged
+// FP64 Newton solution of the same single-rule partial integral the
+// sampler's FP64 step evaluates; a refined root counts as error 1 (its
+// samples never take the quintic start).
+template <int NGL, class Ev, class Ev64>
+__device__ float start_label(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int64_t i) {
+    int ls = A.log2_nsub;
+    int64_t ro = i << ls;
+    if (A.root_k) {
+        ls = A.root_k[i];
+        ro = A.root_off[i];
+    }
+    const uint64_t deep = A.deep[i];
+    const typename Ev::Ctx ctx = ev.load((int)i, 1.0);
+    const typename Ev64::Ctx cw = Ev64::widen(ctx);
+    float worst = 0.0f;
+    double prev = 0.0;
+    for (int k = 0; k < (1 << ls); k++) {
+        const double pk = A.root_prefix[ro + k];
+        const double len = pk - prev;
+        prev = pk;
+        if ((deep >> k) & 1ull) {
+            worst = 1.0f;
+            continue;
+        }
+        float a, xm, b;
+        task_bounds<float>(k, 0, 0u, ls, a, xm, b);
+        const LeafEnds<float> E = leaf_ends_slopes(ev, ctx, a, b);
+        const double wd = (double)b - (double)a;
+        for (int j = 1; j <= 3; j++) {
+            const double tau = 0.25 * j * len;
+            float vs;
+            const float d0 = invert_leaf<NGL>(ev, ctx, a, b, (float)tau, (float)len, 0, 1, &vs, true, E, true);
+            double d = d0, lo = 0.0, hi = wd;
+            for (int it = 0; it < 16; it++) {
+                const double F = quad_from<NGL>(ev64, cw, (double)a, d) - tau;
+                if (F > 0.0) hi = d; else lo = d;
+                const double v = ev64.speed(cw, (double)a + d);
+                double dn = v > 0.0 ? d - F / v : 0.5 * (lo + hi);
+                if (!(dn >= lo && dn <= hi)) dn = 0.5 * (lo + hi);
+                const bool done = fabs(dn - d) <= wd * 0x1p-45;
+                d = dn;
+                if (done) break;
+            }
+            const float e = (float)(fabs((double)d0 - d) / wd);
+            if (!(e <= worst)) worst = e == e ? e : 1.0f;  // a NaN start counts as error 1
+        }
+    }
+    return log2f(worst > 0x1p-40f ? worst : 0x1p-40f);
+}
+
+static __global__ __launch_bounds__(256) void k_start_labels(BSplineTableDev T, int uniform_class, const int32_t* int_curve,
+                                                      SampleArgs A, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_int) return;
+    const int c = int_curve[i];
+    const int cls = A.class_of_curve ? A.class_of_curve[c] : uniform_class;
+    float r = 0.0f;
+#define GCV_LABEL_CASE(PP, DD)                                                                                \
+    case 10 * PP + DD:                                                                                        \
+        r = start_label<3>(MakeEv<float, PP, DD>::get(T), MakeEv64<float, PP, DD>::get(T), A, i);             \
+        break;
+    switch (cls) {
+        GCV_LABEL_CASE(1, 2)
+        GCV_LABEL_CASE(1, 3)
+        GCV_LABEL_CASE(2, 2)
+        GCV_LABEL_CASE(2, 3)
+        GCV_LABEL_CASE(3, 2)
+        GCV_LABEL_CASE(3, 3)
+        GCV_LABEL_CASE(4, 2)
+        GCV_LABEL_CASE(4, 3)
+        GCV_LABEL_CASE(5, 2)
+        GCV_LABEL_CASE(5, 3)
+        default:
+            break;
+    }
+#undef GCV_LABEL_CASE
+    out[i] = r;
+}
+
+template <typename Dt, int NGL>
+hipError_t Launch<Dt, NGL>::start_labels(const LaunchCfg& c, int uniform_class, const BSplineTableDev& T,
+                                         const int32_t* int_curve, const SampleArgs& A, float* out) {
+    if (A.n_int <= 0) return hipSuccess;
+    if constexpr (sizeof(Dt) != sizeof(float) || NGL != 3) {
+        return hipErrorInvalidValue;  // the quintic start serves FP32 GL-3 builds
+    } else {
+        prof_begin(c.owner, "k_start_labels");
+        hipLaunchKernelGGL(k_start_labels, dim3((unsigned)((A.n_int + 255) / 256)), dim3(256), 0, c.stream, T,
+                           uniform_class, int_curve, A, out);
+        hipError_t e = hipGetLastError();
+        prof_end(c.owner);
+        return e;
+    }
 }
 
 }  // namespace rt

@@ -9,13 +9,21 @@ build (gcv_table_build_seeded) starts interval i from 2^k_i roots, k_i in
 so accuracy is that of the adaptive test whatever k_i is; the seed only moves
 the work.
 
-This module drives the device predictor of k_i: a 16-64-64-2 perceptron on
+This module drives the device predictor of k_i: a 16-64-64-4 perceptron on
 BF16 matrix cores (gcv_mlp_*). Training labels come from the library itself:
 per-interval quadrature counts of uniform-k builds for k = 0..6; the label of
 an interval is the smallest k at or above its safety floor whose roots all
 pass the level-0 test (or, with the "cheapest" policy, the k of fewest
 quadratures), and the second output learns whether that seed needs no
 refinement.
+
+The same network predicts the sampler's Newton start (outputs 2 and 3): on
+the uniform-k table of the interval's label k, the library measures the
+worst error of the quintic Hermite start of the inverse map over the
+interval's roots against the converged Newton solution
+(gcv_table_start_labels); output 2 learns its scaled log2 and output 3
+whether it is at most 2^-13 of a root, in which case the sampler takes the
+FP64 step straight from that start (gcv_table_set_newton_start).
 
 ``python -m gcurval_amd.seed --train`` regenerates ``seed_weights.npz``.
 """
@@ -34,9 +42,12 @@ from . import (GCV_FP32, ArclengthCalculateGpu, Context, DeviceArray, _ptr, lib)
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "seed_weights.npz")
 
-N_FEAT, N_HID, N_OUT = 16, 64, 2
+N_FEAT, N_HID, N_OUT = 16, 64, 4
 N_PARAMS = N_FEAT * N_HID + N_HID + N_HID * N_HID + N_HID + N_HID * N_OUT + N_OUT
 MAX_K = 6
+# direct-start label: worst quintic-start error at most 2^-13 of a root (the
+# sampler keeps an FP64 step that moves at most 2^-12)
+START_LOG2_OK = -13.0
 
 FEATURE_NAMES = [
     "log2 speed x=0", "log2 speed x=1/4", "log2 speed x=1/2", "log2 speed x=3/4", "log2 speed x=1",
@@ -84,7 +95,7 @@ class SeedMLP:
         m = np.ascontiguousarray(mean, np.float32)
         s = np.ascontiguousarray(istd, np.float32)
         if p.size != N_PARAMS or m.size != N_FEAT or s.size != N_FEAT:
-            raise ValueError("parameter sizes must be 5378, 16, 16")
+            raise ValueError(f"parameter sizes must be {N_PARAMS}, 16, 16")
         self.ctx._check(lib().gcv_mlp_set_params(self.h, _ptr(p), _ptr(m), _ptr(s)))
 
     def get_params(self):
@@ -108,7 +119,7 @@ class SeedMLP:
             d.free()
 
     def predict(self, eng: ArclengthCalculateGpu, hedge: bool = False):
-        """(k int8 [n_intervals], raw outputs float [n_intervals][2])."""
+        """(k int8 [n_intervals], raw outputs float [n_intervals][4])."""
         n = eng.n_intervals
         dk = self.ctx.empty(n, np.int8)
         dr = self.ctx.empty(n * N_OUT, np.float32)
@@ -158,22 +169,28 @@ def _dptr(x) -> int:
 # ---------------------------------------------------------------------------
 # Labels and training
 # ---------------------------------------------------------------------------
-def cost_matrix(ctx: Context, batch, precision=GCV_FP32, **opts):
+def cost_matrix(ctx: Context, batch, precision=GCV_FP32, with_start: bool = False, **opts):
     """Per-interval quadrature counts of uniform-k builds, k = 0..6, and the
-    safety floor: (cost int64 [7][n_intervals], floor int8 [n_intervals])."""
+    safety floor: (cost int64 [7][n_intervals], floor int8 [n_intervals]);
+    with_start adds the log2 quintic-start errors of those builds (float
+    [7][n_intervals]; NaN for ellipses, whose sampler picks its start
+    analytically)."""
     with ArclengthCalculateGpu(ctx, batch, precision=precision, normalize=True, **opts) as eng:
         n = eng.n_intervals
         d_cost = ctx.empty(n, np.int32)
         cost = np.empty((MAX_K + 1, n), np.int64)
+        start = np.full((MAX_K + 1, n), np.nan, np.float32)
         for k in range(MAX_K + 1):
             eng.createArclengthTableUniform(k, d_cost)
             cost[k] = d_cost.download()
+            if with_start and batch.kind != "ellipse":
+                start[k] = eng.start_labels()
         d_k = ctx.array(np.zeros(n, np.int8))
         eng.createArclengthTableSeeded(d_k)
         floor = eng.seed_floor()
         d_k.free()
         d_cost.free()
-    return cost, floor
+    return (cost, floor, start) if with_start else (cost, floor)
 
 
 def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "clean") -> np.ndarray:
@@ -198,6 +215,15 @@ def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "clean"
     return np.stack([k.astype(np.float32), (c == 3 * (1 << k)).astype(np.float32)], axis=1)
 
 
+def start_targets(log2_err: np.ndarray) -> np.ndarray:
+    """Targets of outputs 2 and 3 from log2 quintic-start errors: the error
+    mapped to about [-2.5, 2.5] and the direct-start flag (NaN stays NaN)."""
+    e = np.asarray(log2_err, np.float32)
+    y2 = (e + 20.0) / 8.0
+    y3 = np.where(np.isnan(e), np.nan, (e <= START_LOG2_OK).astype(np.float32))
+    return np.stack([y2, y3], axis=1).astype(np.float32)
+
+
 def training_batches(seed: int = 100):
     """Synthetic batches spanning the classes the predictor serves (about 2.2M
     intervals): cubic 3-D random walks with near-cusps, mixed degrees 2/3/5 in
@@ -218,12 +244,16 @@ def training_batches(seed: int = 100):
 
 
 def make_dataset(ctx: Context, batches, mlp: SeedMLP, policy: str = "clean"):
+    """Feature rows and targets [n][4]: the seed labels, then the start targets
+    measured on the uniform-k table of each interval's label k."""
     X, Y = [], []
     for b in batches:
-        cost, floor = cost_matrix(ctx, b)
+        cost, floor, start = cost_matrix(ctx, b, with_start=True)
         with ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
             X.append(mlp.features(eng))
-        Y.append(labels_from_costs(cost, floor, policy))
+        y = labels_from_costs(cost, floor, policy)
+        k = y[:, 0].astype(np.int64)
+        Y.append(np.concatenate([y, start_targets(start[k, np.arange(k.size)])], axis=1))
     return np.concatenate(X), np.concatenate(Y)
 
 
@@ -280,7 +310,8 @@ def _main():
         X, Y = make_dataset(ctx, training_batches(), probe, a.policy)
         probe.close()
         print(f"dataset {X.shape[0]} intervals in {time.time() - t0:.1f}s; label histogram "
-              f"{np.bincount(Y[:, 0].astype(int), minlength=7).tolist()}", flush=True)
+              f"{np.bincount(Y[:, 0].astype(int), minlength=7).tolist()}; direct-start share "
+              f"{np.nanmean(Y[:, 3]):.3f} of {int(np.sum(~np.isnan(Y[:, 3])))} B-spline intervals", flush=True)
         mlp = train(ctx, X, Y, epochs=a.epochs)
         mlp.save(a.out, n_rows=int(X.shape[0]), epochs=a.epochs, policy=a.policy)
         print(f"saved {a.out}", flush=True)

@@ -336,6 +336,27 @@ struct BSplineEval {
         }
         return sq(s);
     }
+    // speed and its slope d|C'|/dx = (C' . C'') / |C'| (0 where the speed
+    // vanishes), C' and C'' from one Horner pass per component
+    template <typename T>
+    __device__ __forceinline__ void speed_and_slope(const Ctx& k, T x, T& v, T& vp) const {
+        T s = T(0), sd = T(0);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T c = T(k.c[d * P + P - 1]), dc = T(0);
+#pragma unroll
+            for (int j = P - 2; j >= 0; j--) {
+                dc = dc * x + c;
+                c = c * x + T(k.c[d * P + j]);
+            }
+            s += c * c;
+            sd += c * dc;
+        }
+        v = sq(s);
+        vp = v > T(0) ? sd * rcp(v) : T(0);
+    }
+    __device__ __forceinline__ static float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+    __device__ __forceinline__ static double rcp(double v) { return rcp_f64(v); }
     __device__ __forceinline__ static float sq(float v) { return __builtin_amdgcn_sqrtf(v); }
     __device__ __forceinline__ static double sq(double v) { return sqrt_pos(v); }
     __device__ __forceinline__ double to_u(int i, double x) const { return u0[i] + h[i] * x; }

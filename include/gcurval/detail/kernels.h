@@ -1081,7 +1081,14 @@ struct SampleArgs {
     const int32_t* class_of_curve;  // optional (mixed batches): evaluator class per curve
     const int32_t* blk_int;     // optional (uniform positions): interval of each sample block's first sample
     int64_t n_int;              // knot intervals of the batch (0: no interval windows)
+    // Newton start of B-spline samples in FP32 builds (kStart*): the plain
+    // cubic start with FP32 steps, the learned per-interval choice of the
+    // quintic start (start_ok[i] != 0) or the quintic start everywhere
+    int start_mode;
+    const int8_t* start_ok;     // [n_int] learned decisions (kStartLearned)
+    int8_t* out_iters;          // optional per sample: FP32 steps | FP64 steps << 4
 };
+constexpr int kStartPlain = 0, kStartLearned = 1, kStartQuintic = 2;
 
 // first index in [lo, hi) with a[idx] > v (hi if none)
 __device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t lo, int64_t hi, int64_t v) {
@@ -1239,13 +1246,41 @@ __device__ __forceinline__ LeafEnds<Dt> leaf_ends_quintic(const Ev& ev, const ty
     }
     return e;
 }
+// Evaluators that give the speed and its slope in one pass (B-splines).
+template <class Ev, class = void>
+struct HasSpeedAndSlope : std::false_type {};
+template <class Ev>
+struct HasSpeedAndSlope<Ev, decltype((void)std::declval<const Ev&>().speed_and_slope(
+                                std::declval<const typename Ev::Ctx&>(), 0.0f, std::declval<float&>(),
+                                std::declval<float&>()))> : std::true_type {};
+template <class Ev, typename Dt>
+__device__ __forceinline__ LeafEnds<Dt> leaf_ends_slopes(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b) {
+    if constexpr (HasSpeedAndSlope<Ev>::value) {
+        LeafEnds<Dt> e;
+        Dt sa, sb;
+        ev.speed_and_slope(c, a, e.va, sa);
+        ev.speed_and_slope(c, b, e.vb, sb);
+        e.rva = nrcp(e.va);
+        e.rvb = nrcp(e.vb);
+        e.xa2 = Dt(0);
+        e.xb2 = Dt(0);
+        if (e.va > Dt(0) && e.vb > Dt(0)) {
+            e.xa2 = -sa * e.rva * e.rva * e.rva;
+            e.xb2 = -sb * e.rvb * e.rvb * e.rvb;
+        }
+        return e;
+    } else {
+        return leaf_ends_quintic(ev, c, a, b);
+    }
+}
 // With `quintic` (ends known, second derivatives included) the start is the
 // quintic Hermite interpolant of x(s) and no Newton step follows here: the
 // caller takes the FP64 step from it (*v_out = the speed at the start).
 template <int NGL, class Ev, typename Dt>
 __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& c, Dt a, Dt b, Dt tau, Dt len,
                                           int iters, int pieces, Dt* v_out, bool ends_known = false,
-                                          const LeafEnds<Dt>& ends_in = LeafEnds<Dt>{}, bool quintic = false) {
+                                          const LeafEnds<Dt>& ends_in = LeafEnds<Dt>{}, bool quintic = false,
+                                          int* n_steps = nullptr) {
     const Dt w = b - a;
     Dt t = len > Dt(0) ? tau * nrcp(len) : Dt(0);
     t = t < Dt(0) ? Dt(0) : (t > Dt(1) ? Dt(1) : t);
@@ -1288,7 +1323,10 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
         Dt dn = v > Dt(0) ? d - F * nrcp(v) : Dt(0.5) * (lo + hi);
         if (!(dn >= lo && dn <= hi)) dn = Dt(0.5) * (lo + hi);
         const bool past = it + 1 >= iters;
-        if (past && (dabs(dn - d) <= conv || it + 1 >= iters + kExtraNewton)) return dn;
+        if (past && (dabs(dn - d) <= conv || it + 1 >= iters + kExtraNewton)) {
+            if (n_steps) *n_steps = it + 1;
+            return dn;
+        }
         const Dt vn = ev.speed(c, a + dn);
         if (past)
             F = quad_from_n<NGL>(ev, c, a, dn, pieces) - tau;
@@ -1364,11 +1402,13 @@ constexpr int kSampleBlock = 256;
 // read from the arguments at run time, or fixed at compile time so that a
 // kernel has no branch between its loads.
 struct RuntimeMode {
-    static constexpr bool kStatic = false, kPos = false, kSeeded = false;
+    static constexpr bool kStatic = false, kPos = false, kSeeded = false, kStart = false;
 };
-template <bool P, bool S>
+// kStart: the kernel carries the learned / quintic Newton start (kept out of
+// the plain kernels, whose registers and code it would otherwise grow)
+template <bool P, bool S, bool St = false>
 struct StaticMode {
-    static constexpr bool kStatic = true, kPos = P, kSeeded = S;
+    static constexpr bool kStatic = true, kPos = P, kSeeded = S, kStart = St;
 };
 template <class M>
 __device__ __forceinline__ bool mode_pos(const SampleArgs& A) {
@@ -1496,13 +1536,13 @@ __device__ __forceinline__ int64_t locate_interval(const SampleArgs& A, const Ta
 // then the FP64 step. c64 is the FP64 context for evaluators that load it
 // (read before any wave barrier); evaluators that widen their Dt context do
 // so here.
-template <typename Dt, int NGL, class Ev, class Ev64>
+template <typename Dt, int NGL, class Ev, class Ev64, bool kStart = false>
 __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, const SampleArgs& A, int i,
                                                 const typename Ev::Ctx& ctx, const typename Ev64::Ctx& c64,
                                                 uint64_t deep, int ls, int k, double sn, double pre, double rlen,
                                                 bool root_ends_known = false,
                                                 const LeafEnds<Dt>& root_ends = LeafEnds<Dt>{},
-                                                bool quintic = false) {
+                                                bool quintic = false, int* iters = nullptr) {
     const Dt eps0 = (Dt)A.eps0;
     double tau = sn - pre;
     tau = tau < 0.0 ? 0.0 : (tau > rlen ? rlen : tau);
@@ -1519,27 +1559,68 @@ __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, 
     }
     Dt v_last;
     const int pieces = 1;
-    // the quintic start replaces the Dt Newton steps only where the FP64 step follows
-    const bool q = quintic && ends_known && sizeof(Dt) < sizeof(double) && A.fp64_step;
-    const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last, ends_known,
-                                  root_ends, q);
-    double dd = (double)d;
-    if (sizeof(Dt) < sizeof(double) && A.fp64_step) {
-        // one FP64 Newton step on the same leaf: the residual is evaluated in
-        // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
-        // only scales a residual that is already at FP32 rounding level)
+    constexpr bool f32 = sizeof(Dt) < sizeof(double);
+    const bool step64 = f32 && A.fp64_step;
+    // one FP64 Newton step on the same leaf: the residual is evaluated in
+    // FP64; the derivative is the last FP32 speed (its 1e-7 relative error
+    // only scales a residual that is already small)
+    auto newton64 = [&](double d0, Dt v) -> double {
         typename Ev64::Ctx cw;
         if constexpr (HasWiden<Ev64, typename Ev::Ctx>::value)
             cw = Ev64::widen(ctx);
         else
             cw = c64;
         const double ad = (double)a, wd = (double)b - (double)a;
-        const double F = quad_from_n<NGL>(ev64, cw, ad, dd, pieces) - tau;
-        if (v_last > Dt(0)) {
-            const double dn = dd - F * (double)nrcp(v_last);
-            dd = dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
+        const double F = quad_from_n<NGL>(ev64, cw, ad, d0, pieces) - tau;
+        if (!(v > Dt(0))) return d0;
+        const double dn = d0 - F * (double)nrcp(v);
+        return dn < 0.0 ? 0.0 : (dn > wd ? wd : dn);
+    };
+    int n32 = 0, n64 = 0;
+    double dd;
+    if constexpr (kStart && f32 && HasSpeedAndSlope<Ev>::value) {
+        // learned (or forced) Newton start, decided per wave so that the
+        // wave's lanes run one path: when every lane's interval is marked
+        // (and no lane sits on a refined root), the quintic Hermite start of
+        // the inverse map goes straight to the FP64 step, which a lane keeps
+        // when it moved the start by at most 2^-12 of the leaf w (Newton's
+        // error after it is then about (w |v'|/v) 2^-25 w); a lane whose step
+        // moved more falls back to the plain start and steps below
+        const bool want = step64 && !((deep >> k) & 1ull) &&
+                          (A.start_mode == kStartQuintic || (A.start_mode == kStartLearned && A.start_ok[i] != 0));
+        bool plain = true;
+        dd = 0.0;
+        LeafEnds<Dt> E{};
+        if (__all(want)) {
+            E = leaf_ends_slopes(ev, ctx, a, b);
+            Dt vs;
+            const Dt d0 = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, 0, pieces, &vs, true, E, true);
+            dd = newton64((double)d0, vs);
+            n64 = 1;
+            plain = !(dabs(dd - (double)d0) <= ((double)b - (double)a) * 0x1p-12);
+            ends_known = true;
+        }
+        if (plain) {
+            const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last, ends_known,
+                                          ends_known ? E : root_ends, false, &n32);
+            dd = (double)d;
+            if (step64) {
+                dd = newton64(dd, v_last);
+                n64++;
+            }
+        }
+    } else {
+        // the quintic start replaces the Dt Newton steps only where the FP64 step follows
+        const bool q = quintic && ends_known && step64;
+        const Dt d = invert_leaf<NGL>(ev, ctx, a, b, (Dt)tau, len, A.newton_iters, pieces, &v_last, ends_known,
+                                      root_ends, q, &n32);
+        dd = (double)d;
+        if (step64) {
+            dd = newton64(dd, v_last);
+            n64 = 1;
         }
     }
+    if (iters) *iters = n32 | (n64 << 4);
     return ev.to_u(i, (double)a + dd);
 }
 
@@ -1594,7 +1675,11 @@ __device__ __forceinline__ double sample_one(const Ev& ev, const Ev64& ev64, con
         find_root((const lds_double*)(win + off), ls, sn, k, pre, rlen);
     else
         find_root(A.root_prefix + ro, ls, sn, k, pre, rlen);
-    return finish_sample<Dt, NGL>(ev, ev64, A, (int)i, ctx, c64, deep, ls, k, sn, pre, rlen);
+    int its = 0;
+    const double u = finish_sample<Dt, NGL, Ev, Ev64, M::kStart>(ev, ev64, A, (int)i, ctx, c64, deep, ls, k, sn,
+                                                                 pre, rlen, false, LeafEnds<Dt>{}, false, &its);
+    if (A.out_iters) A.out_iters[g] = (int8_t)its;
+    return u;
 }
 
 // Curve of the first sample of every block of kSampleBlock samples: thread per

@@ -1,7 +1,11 @@
-// Learned seed of the table build: a 16 -> 64 -> 64 -> 2 multilayer
-// perceptron (ReLU) that predicts, per knot interval, log2 of the initial
-// root count (output 0, regression) and the probability that every root of
-// that seed passes the level-0 error test (output 1, logit).
+// Learned seed of the table build and of the sampler's Newton start: a
+// 16 -> 64 -> 64 -> 4 multilayer perceptron (ReLU) that predicts, per knot
+// interval, log2 of the initial root count (output 0, regression), the
+// probability that every root of that seed passes the level-0 error test
+// (output 1, logit), the scaled log2 of the worst error of the quintic
+// Hermite start of the inverse map over the interval's roots (output 2,
+// regression) and the probability that this start is close enough for the
+// final FP64 Newton step to follow it directly (output 3, logit).
 //
 // Every GEMM runs on BF16 matrix cores with FP32 accumulation
 // (v_mfma_f32_16x16x16bf16_1k, wave64): a wave owns a tile of 64 rows
@@ -25,20 +29,20 @@ namespace gcurval {
 namespace dev {
 
 constexpr int kHid = 64;
-constexpr int kOut = 2;
-constexpr int kParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut;  // 5378
+constexpr int kOut = 4;
+constexpr int kParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut;  // 5508
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Parameter block (FP32 master copy + BF16 operands), offsets into one array.
 struct MlpParams {
-    float* p;          // [kParams] W1[16][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+    float* p;          // [kParams] W1[16][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4]
     float* feat_mean;  // [16]
     float* feat_istd;  // [16]
     uint16_t* w1b;     // [16][64]
     uint16_t* w2b;     // [64][64]
-    uint16_t* w3b;     // [64][16]  (columns >= 2 are zero)
+    uint16_t* w3b;     // [64][16]  (columns >= kOut are zero)
 };
 __host__ __device__ constexpr int off_W1() { return 0; }
 __host__ __device__ constexpr int off_b1() { return kFeat * kHid; }
@@ -190,9 +194,9 @@ __device__ __forceinline__ bf16x4 relu_bf16(f32x4 d) {
 }
 
 // Forward pass of 16 rows: x is the B operand X^T (lane l: row l % 16,
-// features 4 (l / 16) .. +3). Lanes 0..15 receive outputs 0 and 1 of row l.
+// features 4 (l / 16) .. +3). Lanes 0..15 receive outputs 0..3 of row l.
 // Each layer's accumulator starts from its bias.
-__device__ __forceinline__ void mlp_forward16(const MlpFrags& W, bf16x4 x, float& o0, float& o1) {
+__device__ __forceinline__ f32x4 mlp_forward16(const MlpFrags& W, bf16x4 x) {
     bf16x4 a1[4], a2[4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) a1[mt] = relu_bf16(mfma16(W.w1[mt], x, W.b1[mt]));
@@ -207,11 +211,10 @@ __device__ __forceinline__ void mlp_forward16(const MlpFrags& W, bf16x4 x, float
     }
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) a2[mt] = relu_bf16(h2[mt]);
-    f32x4 o = {W.b3[0], W.b3[1], 0.f, 0.f};
+    f32x4 o = {W.b3[0], W.b3[1], W.b3[2], W.b3[3]};
 #pragma unroll
     for (int c = 0; c < 2; c++) o = mfma32(W.w3[c], cat8(a2[2 * c], a2[2 * c + 1]), o);
-    o0 = o[0];
-    o1 = o[1];
+    return o;
 }
 
 // The network's operand fragments in LDS for the inference kernels, in the
@@ -265,7 +268,7 @@ __device__ __forceinline__ void stage_mlp(const MlpParams& P, MlpLds* L) {
 // (scheduling barriers between the layers keep the reads of one layer from
 // being hoisted above the products of the one before, which would hold all
 // the fragments in registers at once).
-__device__ __forceinline__ void mlp_forward16_lds(const MlpLds* L, bf16x4 x, float& o0, float& o1) {
+__device__ __forceinline__ f32x4 mlp_forward16_lds(const MlpLds* L, bf16x4 x) {
     const int l = lane_id();
     bf16x4 a1[4], a2[4];
 #pragma unroll
@@ -283,11 +286,10 @@ __device__ __forceinline__ void mlp_forward16_lds(const MlpLds* L, bf16x4 x, flo
     }
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) a2[mt] = relu_bf16(h2[mt]);
-    f32x4 o = {L->b3[0], L->b3[1], 0.f, 0.f};
+    f32x4 o = {L->b3[0], L->b3[1], L->b3[2], L->b3[3]};
 #pragma unroll
     for (int c = 0; c < 2; c++) o = mfma32(L->w3[c][l], cat8(a2[2 * c], a2[2 * c + 1]), o);
-    o0 = o[0];
-    o1 = o[1];
+    return o;
 }
 
 // B operand X^T from a row-major BF16 row block (rows r0.., 16 features).
@@ -298,6 +300,10 @@ __device__ __forceinline__ bf16x4 load_xt(const uint16_t* rows, int64_t r0, int6
     if (r < n) v = *(const bf16x4*)(rows + r * kFeat + 4 * (l >> 4));
     return v;
 }
+
+// Output 3 > 0: the quintic start of the interval's roots may go straight to
+// the FP64 Newton step (the sampler still checks each sample's step).
+__device__ __forceinline__ int8_t start_from_outputs(float o3) { return o3 > 0.f ? (int8_t)1 : (int8_t)0; }
 
 __device__ __forceinline__ int8_t k_from_outputs(float o0, float o1, int hedge) {
     int k = (int)rintf(o0);
@@ -396,11 +402,12 @@ constexpr int kMlpLdsBytes = (int)sizeof(MlpLds);
 // the safety floor kmin[i].
 struct SeedOut {
     int8_t* kk;
-    float* raw;     // optional [n][2]
+    float* raw;     // optional [n][4]
     int8_t* kmin;
     int hedge;
     int ls_plain;   // log2 n_subdivided (safety floor of the plain root count)
     float tol0;     // level-0 tolerance (normalised units)
+    int8_t* start_ok = nullptr;  // optional [n]: the Newton-start decision (output 3)
 };
 
 // Lane's standardised BF16 row into X[lane] (zeros for a lane without an
@@ -443,16 +450,13 @@ __device__ __forceinline__ void predict_rows(const MlpLds* Wl, const uint16_t* X
         x[1] = (short)(q.x >> 16);
         x[2] = (short)(q.y & 0xFFFFu);
         x[3] = (short)(q.y >> 16);
-        float o0, o1;
-        mlp_forward16_lds(Wl, x, o0, o1);
+        const f32x4 o = mlp_forward16_lds(Wl, x);
         const int r = nt * 16 + (lane & 15);
         const int64_t ir = (int64_t)__shfl((long long)i, r, 64);
         if (lane < 16 && r < n_valid) {
-            O.kk[ir] = k_from_outputs(o0, o1, O.hedge);
-            if (O.raw) {
-                O.raw[ir * kOut] = o0;
-                O.raw[ir * kOut + 1] = o1;
-            }
+            O.kk[ir] = k_from_outputs(o[0], o[1], O.hedge);
+            if (O.start_ok) O.start_ok[ir] = start_from_outputs(o[3]);
+            if (O.raw) *(f32x4*)(O.raw + ir * kOut) = o;
         }
     }
     wave_lds_sync();
@@ -529,8 +533,11 @@ __global__ __launch_bounds__(256) void k_bspline_coef_seed(BSplineDev B, const i
 
 // ---------------------------------------------------------------------------
 // Training: forward + backward over a batch of standardised features X (bf16
-// [n][16]) with targets y ([n][2]: log2 seed, pass flag). Loss per row:
-//   (out0 - y0)^2 + w_pass * BCE(sigmoid(out1), y1), averaged over n.
+// [n][16]) with targets y ([n][4]: log2 seed, pass flag, scaled log2 start
+// error, direct-start flag). Loss per row:
+//   (out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1)
+//     + (out2 - y2)^2 + w_pass BCE(sigmoid(out3), y3),  averaged over n;
+// a NaN target leaves its term (and gradient) out.
 // Gradients accumulate into grad[kParams] (FP32, zeroed beforehand); the
 // summed loss into loss[0].
 // ---------------------------------------------------------------------------
@@ -597,10 +604,10 @@ static __global__ __launch_bounds__(64) void k_mlp_train(const uint16_t* Xg, con
             const int m = 4 * (l >> 4) + j;
             const int64_t r = r0 + m;
             float g = 0.f;
-            if (r < n && col < kOut) {
+            const float y = r < n && col < kOut ? yg[r * kOut + col] : 0.f;
+            if (r < n && col < kOut && y == y) {
                 const float out = o[j] + b3[col];
-                const float y = yg[r * kOut + col];
-                if (col == 0) {
+                if ((col & 1) == 0) {
                     const float d = out - y;
                     lsum += d * d;
                     g = 2.f * d;
@@ -615,11 +622,11 @@ static __global__ __launch_bounds__(64) void k_mlp_train(const uint16_t* Xg, con
             if (col < kOut) gb3 += g;
         }
         wave_lds_sync();
-        // dW3 += H2^T dO  (M = 64 hidden in 4 tiles, K = 16 rows, N = 16 -> 2 used)
+        // dW3 += H2^T dO  (M = 64 hidden in 4 tiles, K = 16 rows, N = 16 -> 4 used)
         const bf16x4 bo = load_b_lds(dO, 16, 0);
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) gW3[mt] = mfma16(load_at(H2, kHid, mt * 16), bo, gW3[mt]);
-        // dH2 = (dO W3^T) * [H2 > 0]  (K = 16 with only 2 nonzero)
+        // dH2 = (dO W3^T) * [H2 > 0]  (K = 16 with only 4 nonzero)
         const bf16x4 ad = load_a(dO, 16, 0);
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
@@ -745,19 +752,15 @@ static __global__ void k_mlp_adam(MlpParams P, float* m, float* v, const float* 
     sync_bf16_one(P, i);
 }
 
-// Forward only over n standardised bf16 rows (tests and evaluation): out[n][2].
+// Forward only over n standardised bf16 rows (tests and evaluation): out[n][4].
 static __global__ __launch_bounds__(64) void k_mlp_forward(const uint16_t* Xg, int64_t n, MlpParams P, float* outg) {
     const int l = lane_id();
     const int64_t ntile = (n + 15) / 16;
     const MlpFrags Wf = load_frags(P);
     for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const int64_t r0 = tile * 16;
-        float o0, o1;
-        mlp_forward16(Wf, load_xt(Xg, r0, n), o0, o1);
-        if (l < 16 && r0 + l < n) {
-            outg[(r0 + l) * kOut] = o0;
-            outg[(r0 + l) * kOut + 1] = o1;
-        }
+        const f32x4 o = mlp_forward16(Wf, load_xt(Xg, r0, n));
+        if (l < 16 && r0 + l < n) *(f32x4*)(outg + (r0 + l) * kOut) = o;
     }
 }
 

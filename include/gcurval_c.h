@@ -233,16 +233,19 @@ int gcv_sample_noalloc(gcv_table* t, const int64_t* d_sample_off, int64_t n_tota
 int gcv_points(gcv_table* t, const int64_t* d_sample_off, int64_t n_total, const double* d_u,
                const int32_t* d_seg, int out_dim, double* d_out);
 
-/* ---- learned seed: a 16-64-64-2 MLP on BF16 matrix cores (FP32 accumulate)
- * predicting, per knot interval, log2 of the initial root count and whether
- * every root of that seed passes the level-0 error test. No counterpart in the
- * reference; the adaptive error test still verifies every root (see
- * gcv_table_build_seeded). ---- */
+/* ---- learned seed: a 16-64-64-4 MLP on BF16 matrix cores (FP32 accumulate)
+ * predicting, per knot interval, log2 of the initial root count (output 0),
+ * whether every root of that seed passes the level-0 error test (output 1),
+ * the scaled log2 of the worst quintic Newton-start error over its roots
+ * (output 2) and whether that start may go straight to the FP64 step
+ * (output 3). No counterpart in the reference; the adaptive error test still
+ * verifies every root (see gcv_table_build_seeded) and the sampler every
+ * FP64 step (see gcv_table_set_newton_start). ---- */
 typedef struct gcv_mlp gcv_mlp;
 int gcv_mlp_create(gcv_ctx* ctx, gcv_mlp** out);
 int gcv_mlp_destroy(gcv_mlp* m);
 /* Host FP32 parameters in the order W1[16][64], b1[64], W2[64][64], b2[64],
- * W3[64][2], b3[2] (5378 values) and the feature standardisation
+ * W3[64][4], b3[4] (5508 values) and the feature standardisation
  * (x - mean) * istd (16 + 16). Setting resets the optimiser state. */
 int gcv_mlp_set_params(gcv_mlp* m, const float* params, const float* mean, const float* istd);
 int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd);
@@ -251,19 +254,21 @@ int gcv_mlp_get_params(gcv_mlp* m, float* params, float* mean, float* istd);
  * statistics of gcv_table_stats are left as they are). */
 int gcv_mlp_features(gcv_table* t, float* d_out);
 /* Predicted k (device int8 [n_intervals]) and optionally the raw outputs
- * (device float [n_intervals][2]); hedge=1 adds one level where the pass head
+ * (device float [n_intervals][4]); hedge=1 adds one level where the pass head
  * predicts a level-0 failure. */
 int gcv_mlp_predict_k(gcv_mlp* m, gcv_table* t, int hedge, int8_t* d_k, float* d_raw);
-/* Setup + prediction + seeded build in one enqueue (no host sync). */
+/* Setup + prediction + seeded build in one enqueue (no host sync); also keeps
+ * the Newton-start decisions (output 3) for the table's sampling. */
 int gcv_table_build_mlp(gcv_table* t, gcv_mlp* m, int hedge, int32_t* d_cost);
 /* k predicted by the last gcv_table_build_mlp (device int8 [n_intervals]). */
 int gcv_table_seed_k(gcv_table* t, const int8_t** d_k);
 /* Forward pass over n standardised feature rows given as BF16 bits (device
- * uint16 [n][16]) into device float [n][2]. */
+ * uint16 [n][16]) into device float [n][4]. */
 int gcv_mlp_forward(gcv_mlp* m, const uint16_t* d_x, int64_t n, float* d_out);
-/* Loss and parameter gradients (host float [5378]) of
- * mean((out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1)) over n rows, targets
- * device float [n][2]. Synchronises. */
+/* Loss and parameter gradients (host float [5508]) of
+ * mean((out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1) + (out2 - y2)^2
+ *      + w_pass BCE(sigmoid(out3), y3)) over n rows, targets device float
+ * [n][4]; a NaN target leaves its term out. Synchronises. */
 int gcv_mlp_grad(gcv_mlp* m, const uint16_t* d_x, const float* d_y, int64_t n, float w_pass, float* host_grad,
                  float* host_loss);
 /* One Adam step (beta 0.9 / 0.999) on that loss; host_loss optional (syncs). */
@@ -313,6 +318,38 @@ int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct
    2 = the look-back scan. Same results either way up to FP64 rounding; each
    is bit-reproducible from build to build. */
 int gcv_table_set_scan(gcv_table* t, int mode);
+
+/* Newton start of the B-spline sampler in FP32 GL-3 builds that take the
+   final FP64 step (no counterpart in the reference, whose sampler bisects
+   down its tree, kernel/reverse.inl.cuh:216-341; other tables always use
+   GCV_START_PLAIN):
+     GCV_START_PLAIN    (default) cubic Hermite start of the inverse map,
+                        FP32 Newton steps, then the FP64 step;
+     GCV_START_LEARNED  on tables built by gcv_table_build_mlp, the
+                        intervals whose predictor output 3 is positive start
+                        from the quintic Hermite interpolant (end speeds and
+                        speed slopes) and take the FP64 step directly when
+                        every sample of a wavefront is on such an interval
+                        and on an unrefined root; a sample whose FP64 step
+                        moves more than 2^-12 of its root falls back to the
+                        plain start and steps. Tables built otherwise: as
+                        GCV_START_PLAIN;
+     GCV_START_QUINTIC  the quintic start with that check on every unrefined
+                        root. */
+#define GCV_START_PLAIN 0
+#define GCV_START_LEARNED 1
+#define GCV_START_QUINTIC 2
+int gcv_table_set_newton_start(gcv_table* t, int mode);
+/* Training labels of the Newton-start outputs on the table as built (FP32
+   GL-3 B-spline tables; no counterpart in the reference): d_out[i] = log2 of
+   the worst quintic-start error over interval i's roots, relative to root
+   width, at 1/4, 1/2 and 3/4 of each root's length, against the converged
+   FP64 Newton solution; a refined root counts as error 1. float [n_intervals]. */
+int gcv_table_start_labels(gcv_table* t, float* d_out);
+/* Diagnostic of the following gcv_sample calls (no counterpart in the
+   reference): per sample, FP32 Newton steps | FP64 steps << 4 (int8
+   [n_total]); null turns it off. */
+int gcv_table_set_iters_out(gcv_table* t, int8_t* d_iters);
 
 /* Host-side helpers that run without a GPU (used by the CPU test suite). */
 /* Power-basis derivative coefficients of one B-spline span, as computed by

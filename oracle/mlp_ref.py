@@ -1,16 +1,18 @@
 """Numpy restatement of the learned seed's network (test infrastructure only).
 
 The learned seed has no counterpart in the reference; this module states the
-network the GPU kernels compute so the tests can check them: a 16-64-64-2
-perceptron with ReLU, loss per row (out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1)
-averaged over the rows. With ``quant=True`` it mirrors the kernels' arithmetic
+network the GPU kernels compute so the tests can check them: a 16-64-64-4
+perceptron with ReLU, loss per row
+  (out0 - y0)^2 + w_pass BCE(sigmoid(out1), y1)
+    + (out2 - y2)^2 + w_pass BCE(sigmoid(out3), y3)
+averaged over the rows, a NaN target leaving its term out. With ``quant=True`` it mirrors the kernels' arithmetic
 (every matrix-product operand rounded to BF16, round to nearest even; FP32
 accumulation, biases and ReLU masks); with ``quant=False`` it is the exact
 FP64 network, whose gradients the tests pin by central differences.
 """
 import numpy as np
 
-N_FEAT, N_HID, N_OUT = 16, 64, 2
+N_FEAT, N_HID, N_OUT = 16, 64, 4
 N_PARAMS = N_FEAT * N_HID + N_HID + N_HID * N_HID + N_HID + N_HID * N_OUT + N_OUT
 
 
@@ -49,7 +51,7 @@ class _Num:
 
 
 def forward(p, X, quant=True):
-    """Outputs [n][2] for standardised features X (rounded to BF16 when quant)."""
+    """Outputs [n][4] for standardised features X (rounded to BF16 when quant)."""
     N = _Num(quant)
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, N.dt))
     Xq = N.q(X)
@@ -58,12 +60,27 @@ def forward(p, X, quant=True):
     return N.mm(N.q(H2), N.q(W3)) + b3
 
 
+def _terms(O, y, w_pass):
+    """Per-element loss terms and their derivatives (zero where y is NaN)."""
+    m = ~np.isnan(y)
+    yz = np.where(m, y, 0.0)
+    L = np.zeros_like(O, dtype=np.float64)
+    G = np.zeros_like(O, dtype=np.float64)
+    for c in range(N_OUT):
+        o = O[:, c].astype(np.float64)
+        if c % 2 == 0:
+            d = o - yz[:, c]
+            L[:, c], G[:, c] = d * d, 2 * d
+        else:
+            L[:, c] = w_pass * (np.maximum(o, 0) - o * yz[:, c] + np.log1p(np.exp(-np.abs(o))))
+            G[:, c] = w_pass * (1 / (1 + np.exp(-o)) - yz[:, c])
+    return np.where(m, L, 0.0), np.where(m, G, 0.0)
+
+
 def loss(p, X, y, w_pass, quant=True):
     O = forward(p, X, quant)
-    d0 = O[:, 0] - y[:, 0]
-    o1 = O[:, 1]
-    bce = np.maximum(o1, 0) - o1 * y[:, 1] + np.log1p(np.exp(-np.abs(o1)))
-    return (np.sum(d0 * d0) + w_pass * np.sum(bce)) / X.shape[0]
+    L, _ = _terms(O, y, w_pass)
+    return np.sum(L) / X.shape[0]
 
 
 def loss_and_grad(p, X, y, w_pass, quant=True):
@@ -77,10 +94,9 @@ def loss_and_grad(p, X, y, w_pass, quant=True):
     H2 = np.maximum(N.mm(H1q, N.q(W2)) + b2, 0)
     H2q = N.q(H2)
     O = N.mm(H2q, N.q(W3)) + b3
-    d0 = O[:, 0] - y[:, 0]
-    o1 = O[:, 1]
-    L = (np.sum(d0 * d0) + w_pass * np.sum(np.maximum(o1, 0) - o1 * y[:, 1] + np.log1p(np.exp(-np.abs(o1))))) / n
-    dO = (np.stack([2 * d0, w_pass * (1 / (1 + np.exp(-o1)) - y[:, 1])], axis=1) / n).astype(N.dt)
+    Lt, Gt = _terms(O, y, w_pass)
+    L = np.sum(Lt) / n
+    dO = (Gt / n).astype(N.dt)
     dOq = N.q(dO)
     gb3 = dO.sum(0)
     gW3 = N.mm(H2q.T, dOq)

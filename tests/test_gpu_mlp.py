@@ -30,7 +30,9 @@ def _params(seed):
 def _data(n, seed):
     rng = np.random.default_rng(seed)
     X = rng.normal(size=(n, 16)).astype(np.float32)
-    y = np.stack([rng.uniform(0, 6, n), rng.integers(0, 2, n)], axis=1).astype(np.float32)
+    y = np.stack([rng.uniform(0, 6, n), rng.integers(0, 2, n), rng.uniform(-3, 3, n), rng.integers(0, 2, n)],
+                 axis=1).astype(np.float32)
+    y[::5, 2:] = np.nan  # rows without start labels leave those terms out
     return X, y
 
 
@@ -68,19 +70,20 @@ def test_gradients_match_reference(ctx):
     assert abs(loss - Lr) <= 1e-3 * abs(Lr)
     assert abs(loss2 - loss) <= 1e-5 * abs(loss)
     np.testing.assert_allclose(g2, g, rtol=1e-4, atol=1e-7)
-    blocks = {"W1": (0, 1024), "b1": (1024, 1088), "W2": (1088, 5184), "b2": (5184, 5248), "W3": (5248, 5376),
-              "b3": (5376, 5378)}
+    blocks = {"W1": (0, 1024), "b1": (1024, 1088), "W2": (1088, 5184), "b2": (5184, 5248), "W3": (5248, 5504),
+              "b3": (5504, 5508)}
     for name, (a, b) in blocks.items():
         rel = np.linalg.norm(g[a:b] - gr[a:b]) / np.linalg.norm(gr[a:b])
         assert rel <= 2e-2, (name, rel)
 
 
 def test_training_reduces_loss(ctx):
-    # a learnable target: y0 = smooth function of the features, y1 = sign of one feature
+    # learnable targets: smooth functions of the features and signs of features
     rng = np.random.default_rng(11)
     n = 32768
     X = rng.normal(size=(n, 16)).astype(np.float32)
-    y = np.stack([3 + np.tanh(X[:, 0] - 0.5 * X[:, 3]) * 2, (X[:, 5] > 0)], axis=1).astype(np.float32)
+    y = np.stack([3 + np.tanh(X[:, 0] - 0.5 * X[:, 3]) * 2, (X[:, 5] > 0), np.tanh(X[:, 7] + X[:, 1]),
+                  (X[:, 9] - X[:, 2] > 0)], axis=1).astype(np.float32)
     with S.SeedMLP(ctx, S.he_init(2)) as m:
         d_x = ctx.array(S.bf16_bits(X))
         d_y = ctx.array(y)

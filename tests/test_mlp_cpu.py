@@ -22,8 +22,8 @@ def _params(seed, bias=0.1):
 
 
 def test_sizes():
-    assert R.N_PARAMS == S.N_PARAMS == 5378
-    assert S.he_init(0).shape == (5378,)
+    assert R.N_PARAMS == S.N_PARAMS == 5508
+    assert S.he_init(0).shape == (5508,)
     assert len(S.FEATURE_NAMES) == S.N_FEAT
 
 
@@ -39,18 +39,25 @@ def test_bf16_rounding_bits():
     assert np.all(np.abs(R.bf16(r) - r) <= np.abs(r) * 2.0 ** -8 + 1e-38)
 
 
+def _targets(rng, n):
+    y = np.stack([rng.uniform(0, 6, n), rng.integers(0, 2, n), rng.uniform(-3, 3, n), rng.integers(0, 2, n)],
+                 axis=1).astype(np.float64)
+    y[::3, 2:] = np.nan  # rows without start labels (ellipses)
+    return y
+
+
 def test_exact_gradient_matches_central_differences():
     rng = np.random.default_rng(3)
     p = _params(3)
     X = rng.normal(size=(9, 16))
-    y = np.stack([rng.uniform(0, 6, 9), rng.integers(0, 2, 9)], axis=1).astype(np.float64)
+    y = _targets(rng, 9)
     L, g = R.loss_and_grad(p, X, y, 0.7, quant=False)
     assert abs(L - R.loss(p, X, y, 0.7, quant=False)) <= 1e-12 * abs(L)
     idx = np.concatenate([rng.choice(np.arange(0, 1024), 25, replace=False),   # W1
                           np.arange(1024, 1088, 7),                            # b1
                           rng.choice(np.arange(1088, 5184), 25, replace=False), # W2
                           np.arange(5184, 5248, 7),                            # b2
-                          np.arange(5248, 5378)])                              # W3, b3
+                          np.arange(5248, 5508)])                              # W3, b3
     h = 1e-6
     for i in idx:
         e = np.zeros_like(p)
@@ -63,13 +70,38 @@ def test_bf16_network_close_to_exact():
     rng = np.random.default_rng(4)
     p = _params(4)
     X = rng.normal(size=(200, 16))
-    y = np.stack([rng.uniform(0, 6, 200), rng.integers(0, 2, 200)], axis=1)
+    y = _targets(rng, 200)
     Oq = R.forward(p, X, quant=True)
     Oe = R.forward(p, X, quant=False)
     assert np.max(np.abs(Oq - Oe)) <= 0.05 * np.max(np.abs(Oe))
     _, gq = R.loss_and_grad(p, X, y, 0.5, quant=True)
     _, ge = R.loss_and_grad(p, X, y, 0.5, quant=False)
     assert np.linalg.norm(gq - ge) <= 0.03 * np.linalg.norm(ge)
+
+
+def test_nan_targets_leave_their_terms_out():
+    rng = np.random.default_rng(5)
+    p = _params(5)
+    X = rng.normal(size=(12, 16))
+    y = _targets(rng, 12)
+    L, g = R.loss_and_grad(p, X, y, 0.5, quant=False)
+    # the same loss and gradient as the rows' seed terms plus the start terms of labelled rows only
+    lab = ~np.isnan(y[:, 2])
+    L_seed, g_seed = R.loss_and_grad(p, X, np.concatenate([y[:, :2], np.full((12, 2), np.nan)], 1), 0.5, quant=False)
+    L_start, g_start = R.loss_and_grad(p, X[lab], np.concatenate([np.full((lab.sum(), 2), np.nan), y[lab, 2:]], 1),
+                                       0.5, quant=False)
+    n_lab = lab.sum()
+    assert abs(L - (L_seed + L_start * n_lab / 12)) <= 1e-12 * abs(L)
+    np.testing.assert_allclose(g, g_seed + g_start * n_lab / 12, rtol=1e-10, atol=1e-14)
+    assert np.isfinite(L) and np.all(np.isfinite(g))
+
+
+def test_start_targets():
+    e = np.array([-30.0, -13.0, -12.9, 0.0, np.nan], np.float32)
+    y = S.start_targets(e)
+    np.testing.assert_allclose(y[:4, 0], (e[:4] + 20) / 8)
+    np.testing.assert_array_equal(y[:4, 1], [1, 1, 0, 0])
+    assert np.isnan(y[4]).all()
 
 
 def test_labels_from_costs():
@@ -93,6 +125,6 @@ def test_weights_file_shapes():
     if not os.path.exists(S.WEIGHTS):
         pytest.skip("no trained weights in the tree")
     with np.load(S.WEIGHTS, allow_pickle=False) as z:
-        assert z["params"].shape == (5378,) and z["params"].dtype == np.float32
+        assert z["params"].shape == (5508,) and z["params"].dtype == np.float32
         assert z["mean"].shape == (16,) and z["istd"].shape == (16,)
         assert np.all(np.isfinite(z["params"])) and np.all(z["istd"] > 0)
