@@ -930,23 +930,34 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedP
         LAUNCH_OR_RETURN("k_bspline_curve_setup", dev::k_bspline_curve_setup, dim3((unsigned)blocks), dim3(256), 0,
                          c->stream, B, t->ioff, t->o.normalize, t->curve_scale, t->int_scale, t->int_curve);
     }
+    // the classes' coefficient passes are independent and latency bound:
+    // with several classes they run on the auxiliary streams, so one pass's
+    // tail overlaps the next
+    const int n_streams = std::min(std::max(t->tune_streams, 1), kAuxStreams);
+    const bool multi = n_streams > 1 && t->classes.size() > 1;
+    hipError_t e = multi ? aux_fork(c) : hipSuccess;
+    int ci = 0;
     for (const ClassRange& r : t->classes) {
+        if (e != hipSuccess) break;
         if (r.count == 0) continue;
+        const hipStream_t st = multi ? c->aux[ci++ % n_streams] : c->stream;
         const int32_t* list = t->class_list ? t->class_list + r.offset : nullptr;
         const unsigned blocks = (unsigned)((r.count + 255) / 256);
         // fused pass: 4 waves of 64 intervals per block, grid-stride
         const unsigned sblocks = (unsigned)std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
+        const char* name = sp ? "k_bspline_coef_seed" : "k_bspline_coef";
+        prof_begin(c, name, st);
 #define GCV_COEF_CASE(PP, DD)                                                                                    \
     case 10 * PP + DD:                                                                                           \
         if (sp)                                                                                                  \
-            LAUNCH_OR_RETURN("k_bspline_coef_seed", (dev::k_bspline_coef_seed<PP, DD>), dim3(sblocks),          \
-                             dim3(256), dev::kMlpLdsBytes + 4 * dev::kWaveLds, c->stream, B, list, r.count,      \
-                             t->ioff, t->int_curve, t->int_scale, t->coef64, t->coef32, t->cstride, t->u0, t->h, \
-                             sp->P, sp->O);                                                                      \
+            hipLaunchKernelGGL((dev::k_bspline_coef_seed<PP, DD>), dim3(sblocks), dim3(256),                     \
+                               dev::kMlpLdsBytes + 4 * dev::kWaveLds, st, B, list, r.count, t->ioff,             \
+                               t->int_curve, t->int_scale, t->coef64, t->coef32, t->cstride, t->u0, t->h, sp->P, \
+                               sp->O);                                                                           \
         else                                                                                                     \
-            LAUNCH_OR_RETURN("k_bspline_coef", (dev::k_bspline_coef<PP, DD>), dim3(blocks), dim3(256), 0,       \
-                             c->stream, B, list, r.count, t->ioff, t->int_curve, t->int_scale, t->coef64,        \
-                             t->coef32, t->cstride, t->u0, t->h);                                                \
+            hipLaunchKernelGGL((dev::k_bspline_coef<PP, DD>), dim3(blocks), dim3(256), 0, st, B, list, r.count,  \
+                               t->ioff, t->int_curve, t->int_scale, t->coef64, t->coef32, t->cstride, t->u0,     \
+                               t->h);                                                                            \
         break;
         switch (bspline_class(r.p, r.D)) {
             GCV_COEF_CASE(1, 2)
@@ -960,11 +971,18 @@ static hipError_t build_setup(gcv_table* t, bool reset_stats = true, const SeedP
             GCV_COEF_CASE(5, 2)
             GCV_COEF_CASE(5, 3)
             default:
-                return hipErrorInvalidValue;
+                e = hipErrorInvalidValue;
+                break;
         }
 #undef GCV_COEF_CASE
+        if (e == hipSuccess) e = hipGetLastError();
+        prof_end(c, st);
     }
-    return hipSuccess;
+    if (multi) {
+        const hipError_t ej = aux_join(c);
+        if (e == hipSuccess) e = ej;
+    }
+    return e;
 }
 
 // Cumulative scan of the interval lengths. Batches whose curves all have at

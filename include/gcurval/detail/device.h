@@ -83,6 +83,42 @@ __device__ __forceinline__ Dt quad(const Ev& ev, const typename Ev::Ctx& c, Dt x
     return gl_sum<N>(ev, c, m, k, Dt(0)) * k;
 }
 
+// The same sum, also folding the node speeds into [mn, mx] (evaluators with
+// their own length rule expose no node speeds: mn and mx are left alone).
+template <int N, int J = 0, class Ev, typename Dt>
+__device__ __forceinline__ Dt gl_sum_mm(const Ev& ev, const typename Ev::Ctx& c, Dt m, Dt k, Dt acc, Dt& mn,
+                                        Dt& mx) {
+    if constexpr (J == N) {
+        return acc;
+    } else {
+        const Dt v = ev.speed(c, gl_node<N, J>(m, k));
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+        const Dt t = Dt(GL<N>::w[J]) * v;
+        return gl_sum_mm<N, J + 1>(ev, c, m, k, J == 0 ? t : acc + t, mn, mx);
+    }
+}
+template <int N, class Ev, typename Dt>
+__device__ __forceinline__ Dt quad_mm(const Ev& ev, const typename Ev::Ctx& c, Dt x0, Dt x1, Dt& mn, Dt& mx) {
+    if constexpr (HasLengthRule<Ev>::value) return ev.length(c, x0, x1);
+    const Dt k = (x1 - x0) * Dt(0.5);
+    const Dt m = (x1 + x0) * Dt(0.5);
+    return gl_sum_mm<N>(ev, c, m, k, Dt(0), mn, mx) * k;
+}
+
+// Dip guard of seeded builds: a segment of 2^-m of its knot interval that is
+// wider than a plain root (m < ls_plain) is not accepted as a leaf while the
+// speeds at its halves' nodes dip, min / max < 0.05, nor while
+// m < min(4, ls_plain) and min / max < 0.2: the halves-versus-whole test can
+// pass by coincidence where |C'| nearly vanishes inside a wide segment (a
+// kink of the integrand between nodes), and the plain method's 64-way
+// pre-split keeps such segments narrow.
+template <typename Dt>
+__device__ __forceinline__ bool dip_forces(int m, int ls_plain, Dt mn, Dt mx) {
+    const int mid = ls_plain < 4 ? ls_plain : 4;
+    return (m < ls_plain && mn < Dt(0.05) * mx) || (m < mid && mn < Dt(0.2) * mx);
+}
+
 // ---------------------------------------------------------------------------
 // wave64 primitives
 // ---------------------------------------------------------------------------

@@ -231,11 +231,14 @@ struct SubdivLds {
 // root r; `rb` the lane's root block, root k of it at rb + k. Returns true
 // (wave-uniform) if the refinement outgrew the stack: the tile's results are
 // then incomplete and the caller has it re-run with a larger stack.
-template <typename Dt, int NGL, class Ev, bool kWhole>
+// kGuard (seeded builds): segments wider than a plain root (2^-ls_plain of
+// the interval) whose halves' node speeds dip are refined whatever the test
+// says (dip_forces).
+template <typename Dt, int NGL, class Ev, bool kWhole, bool kGuard = false>
 __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt>& A, const typename Ev::Ctx& ctx,
                                                int i, bool valid, int64_t rb, double scale_i, int ls,
                                                int max_depth, const SubdivLds<Dt>& W, const LdsSegScan& scan,
-                                               WaveCounters& wc) {
+                                               WaveCounters& wc, int ls_plain = 6) {
     typedef typename KeyOf<Dt>::type Key;
     const int lane = lane_id();
     const int nsub = 1 << ls;
@@ -246,10 +249,19 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
     Dt x0, xm, x1;
     task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
     const Dt P = quad<NGL>(ev, ctx, x0, x1);
-    const Dt L = quad<NGL>(ev, ctx, x0, xm);
-    const Dt R = quad<NGL>(ev, ctx, xm, x1);
+    Dt L, R;
+    bool dip = false;
+    if constexpr (kGuard) {
+        Dt mn = Dt(3.0e38f), mx = Dt(0);
+        L = quad_mm<NGL>(ev, ctx, x0, xm, mn, mx);
+        R = quad_mm<NGL>(ev, ctx, xm, x1, mn, mx);
+        dip = dip_forces(ls, ls_plain, mn, mx);
+    } else {
+        L = quad<NGL>(ev, ctx, x0, xm);
+        R = quad<NGL>(ev, ctx, xm, x1);
+    }
     Dt len = L + R;
-    const bool fail = valid && max_depth > 0 && dabs(len - P) > A.eps0;
+    const bool fail = valid && max_depth > 0 && (dabs(len - P) > A.eps0 || dip);
     wc.n_quad += 3ull * popc64(ballot(valid));
     wc.n_tiles++;
     const uint64_t fmask = ballot(fail);
@@ -286,11 +298,20 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
             const typename Ev::Ctx cr = (kWhole || ipw_log2 == 0) ? ctx : shfl_pod(ctx, r);
             Dt a0, am, a1;
             task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
-            const Dt L2 = quad<NGL>(ev, cr, a0, am);
-            const Dt R2 = quad<NGL>(ev, cr, am, a1);
+            Dt L2, R2;
+            bool dip2 = false;
+            if constexpr (kGuard) {
+                Dt mn = Dt(3.0e38f), mx = Dt(0);
+                L2 = quad_mm<NGL>(ev, cr, a0, am, mn, mx);
+                R2 = quad_mm<NGL>(ev, cr, am, a1, mn, mx);
+                dip2 = dip_forces(ls + l, ls_plain, mn, mx);
+            } else {
+                L2 = quad<NGL>(ev, cr, a0, am);
+                R2 = quad<NGL>(ev, cr, am, a1);
+            }
             const Dt lr = L2 + R2;
             const bool over = dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
-            bool f2 = take && l < max_depth && over;
+            bool f2 = take && l < max_depth && (over || dip2);
             if (take && l >= max_depth && over) wc.n_cap++;  // a leaf that still fails: the depth cap
             const uint64_t tmask = ballot(take);
             wc.n_quad += 2ull * popc64(tmask);
@@ -538,8 +559,8 @@ __device__ __forceinline__ void subdivide_bin(const Ev& ev, const SubdivArgs<Dt>
     for (int64_t tile = t0; tile < n_tiles; tile += nwaves) {
         const Stage nxt = fetch(i_nxt, v_nxt);
         i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
-        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, K, max_depth, W, scan,
-                                               wc)) {
+        if (subdivide_tile<Dt, NGL, Ev, false, true>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, K, max_depth, W,
+                                                     scan, wc, B.ls_plain)) {
             // list the tile's intervals (one lane each) for the full-stack re-run
             const bool mine = cur.v && ((lane & ((1 << K) - 1)) == (1 << K) - 1);
             const uint64_t m = ballot(mine);
@@ -600,8 +621,9 @@ __global__ __launch_bounds__(64) void k_subdivide_rerun(Ev ev, SubdivArgs<Dt> A,
         const typename Ev::Ctx ctx = ev.load(i);
         const double sc = ev.scale(i);
         const LdsSegScan scan(W.scan, k);
-        if (subdivide_tile<Dt, NGL, Ev, false>(ev, A, ctx, i, lane < (1 << k), load_s(A.root_off + i), sc, k,
-                                               effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc))
+        if (subdivide_tile<Dt, NGL, Ev, false, true>(ev, A, ctx, i, lane < (1 << k), load_s(A.root_off + i), sc, k,
+                                                     effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc,
+                                                     ls_plain))
             wc.overflow = 1;  // unreachable: the stack holds 64 (depth + 2) entries
     }
     wc.flush(A.stats);
@@ -1341,9 +1363,12 @@ __device__ __forceinline__ Dt invert_leaf(const Ev& ev, const typename Ev::Ctx& 
 // Re-walk the adaptive tree of a refined root in the build's arithmetic and
 // return the build leaf [a, b] holding tau (root-local target), with tau
 // reduced to a leaf-local target and the leaf's length.
+// guard: the table is a seeded build, whose segments wider than a plain root
+// (2^-ls_plain) were also refined where their node speeds dip (dip_forces).
 template <typename Dt, int NGL, class Ev>
 __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int ls, Dt eps0, int max_depth,
-                             double tau, Dt& a, Dt& b, double& tau_leaf, Dt& leaf_len) {
+                             double tau, Dt& a, Dt& b, double& tau_leaf, Dt& leaf_len, bool guard = false,
+                             int ls_plain = 6) {
     Dt stackR[64];
     int l = 0;
     uint32_t idx = 0;
@@ -1353,10 +1378,19 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
     double acc = 0.0;
     for (int guard = 0; guard < (1 << 26); guard++) {
         task_bounds<Dt>(k, l, idx, ls, x0, xm, x1);
-        const Dt L = quad<NGL>(ev, c, x0, xm);
-        const Dt R = quad<NGL>(ev, c, xm, x1);
+        Dt L, R;
+        bool dip = false;
+        if (guard) {
+            Dt mn = Dt(3.0e38f), mx = Dt(0);
+            L = quad_mm<NGL>(ev, c, x0, xm, mn, mx);
+            R = quad_mm<NGL>(ev, c, xm, x1, mn, mx);
+            dip = dip_forces(ls + l, ls_plain, mn, mx);
+        } else {
+            L = quad<NGL>(ev, c, x0, xm);
+            R = quad<NGL>(ev, c, xm, x1);
+        }
         const Dt lr = L + R;
-        if (l < max_depth && dabs(lr - P) > eps0 * pow2m(Dt(0), l)) {
+        if (l < max_depth && (dabs(lr - P) > eps0 * pow2m(Dt(0), l) || dip)) {
             stackR[l + 1] = R;
             l++;
             idx = 2u * idx;
@@ -1553,7 +1587,7 @@ __device__ __forceinline__ double finish_sample(const Ev& ev, const Ev64& ev64, 
     if ((deep >> k) & 1ull) {
         double tl;
         const int depth = effective_depth(A.max_depth, A.ls_plain, ls, sizeof(Dt) == 4);
-        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len);
+        walk_to_leaf<Dt, NGL>(ev, ctx, k, ls, eps0, depth, tau, a, b, tl, len, A.root_k != nullptr, A.ls_plain);
         tau = tl;
         ends_known = false;  // the leaf is below the root
     }

@@ -39,17 +39,19 @@ __device__ __forceinline__ int k_eff(const int8_t* kk, const int8_t* kmin, int64
 // Safety floor of the seed. The halves-versus-whole test can pass by
 // coincidence where |C'| nearly vanishes inside a wide root (a kink of the
 // integrand that no node sees); the plain method's 64-way pre-split keeps
-// such roots narrow. Sampling |dC/dx|^2 at 17 points of the interval, a
-// minimum-to-maximum speed ratio below 0.05 forces the plain root count and
-// below 0.2 at least 16 roots.
+// such roots narrow. The seeded subdivision refines every segment wider than
+// a plain root whose halves' node speeds dip (dip_forces), so the floor only
+// keeps the roots of an interval whose 17-sample min / max speed ratio is
+// below 0.2 narrow enough (4 roots) for those nodes to sample it at least
+// as densely as the 17 points do (spacing <= 0.028 of the interval against
+// 1/16); a non-finite ratio forces the plain root count.
 // ---------------------------------------------------------------------------
-constexpr double kFloorPlain = 0.05;
-constexpr double kFloorMid = 0.2;
-constexpr int kFloorMidK = 4;
+constexpr double kFloorDip = 0.2;
+constexpr int kFloorDipK = 2;
 
 __device__ __forceinline__ int8_t floor_from_ratio(float r2, int ls_plain) {
-    if (!(r2 >= float(kFloorPlain * kFloorPlain))) return (int8_t)ls_plain;
-    if (r2 < float(kFloorMid * kFloorMid)) return (int8_t)(kFloorMidK < ls_plain ? kFloorMidK : ls_plain);
+    if (!(r2 == r2)) return (int8_t)ls_plain;
+    if (r2 < float(kFloorDip * kFloorDip)) return (int8_t)(kFloorDipK < ls_plain ? kFloorDipK : ls_plain);
     return 0;
 }
 

@@ -123,3 +123,36 @@ def test_seeded_mixed_classes_and_ellipses(ctx):
         bound = np.repeat(np.maximum(1e-6, 2e-7 * Le / e.b), c)
         assert np.all(np.abs(u - uet) <= bound)
         d_k.free()
+
+
+def test_dip_guard_refines_wide_roots_like_plain(ctx):
+    # near-cusps and near-collinear loops in degrees 2/3/5: with one root per
+    # interval (raised to 4 where the speed dips), every segment wider than a
+    # plain root whose node speeds dip is refined, so lengths and samples
+    # stay with the plain build and the truth
+    b = W.bspline_batch(40, seed=91, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(16, 80), cusp_frac=0.2,
+                        hairpin_frac=0.5)
+    counts = 8 * b.intervals_per_curve()
+    tr = O.Truth(b)
+    L, _ = tr.lengths()
+    ut, _ = tr.reverse(counts, O.uniform_positions(counts))
+    with G.ArclengthCalculateGpu(ctx, b, normalize=True) as eng:
+        eng.createArclengthTable()
+        Lp = eng.getAllCurveArclengths()
+        up = eng.arclengthReverse(counts, None, True)
+        d_k = ctx.array(np.zeros(b.n_intervals, np.int8))
+        eng.createArclengthTableSeeded(d_k)
+        d_k.free()
+        floor = eng.seed_floor()
+        Ls = eng.getAllCurveArclengths()
+        us = eng.arclengthReverse(counts, None, True)
+        deep = eng.tables()[2]
+        st = eng.stats()
+    assert st["overflow"] == 0
+    assert np.any(floor == 2) and set(np.unique(floor)) <= {0, 2}
+    # the dips show up as refined roots of the raised intervals
+    assert np.count_nonzero(deep[floor == 2]) >= 0.5 * np.count_nonzero(floor == 2)
+    assert np.max(np.abs(Ls - L) / L) <= 1e-6
+    assert np.max(np.abs(Ls - Lp) / Lp) <= 2e-7
+    assert np.max(np.abs(us - ut)) <= 1e-6
+    assert np.max(np.abs(us - up)) <= 1e-6
