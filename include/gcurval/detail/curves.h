@@ -114,6 +114,9 @@ GCURVAL_HD inline int span_poly(int s, const double* T, const double* Pc, int ps
             for (int e = 0; e <= q; e++) d[j][e] = 0.0;
             d[j][0] = fq[j] * (Pc[(m + 1) * pstride + c] - Pc[m * pstride + c]);
         }
+        // step r raises the degree of each point by one: its inputs have
+        // degree r - 1, so the coefficients above x^r stay zero and are not
+        // touched (the same values as the full loops, with half the work)
 #pragma unroll
         for (int r = 1; r <= q; r++) {
 #pragma unroll
@@ -121,11 +124,12 @@ GCURVAL_HD inline int span_poly(int s, const double* T, const double* Pc, int ps
                 // d_j <- (1 - a0 - a1 x) d_{j-1} + (a0 + a1 x) d_j
                 double nw[q + 1];
 #pragma unroll
-                for (int e = 0; e <= q; e++) nw[e] = (1.0 - a0[r][j]) * d[j - 1][e] + a0[r][j] * d[j][e];
+                for (int e = 0; e < r; e++) nw[e] = (1.0 - a0[r][j]) * d[j - 1][e] + a0[r][j] * d[j][e];
+                nw[r] = 0.0;
 #pragma unroll
-                for (int e = 0; e < q; e++) nw[e + 1] += a1[r][j] * (d[j][e] - d[j - 1][e]);
+                for (int e = 0; e < r; e++) nw[e + 1] += a1[r][j] * (d[j][e] - d[j - 1][e]);
 #pragma unroll
-                for (int e = 0; e <= q; e++) d[j][e] = nw[e];
+                for (int e = 0; e <= r; e++) d[j][e] = nw[e];
             }
         }
 #pragma unroll

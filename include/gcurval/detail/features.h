@@ -71,22 +71,29 @@ __device__ __forceinline__ void features_common(const float sp[5], const float d
 template <int P, int D>
 __device__ void features_bspline(const float* c, double h, int n_int_curve, const double* ctrl, int pstride, int s,
                                  float tol0, float* f, float* r2_out) {
+    // dC/dx at x = j / 16, j = 0..16: the min / max of |dC/dx|^2 over all
+    // 17 (the safety-floor quantity) and the vectors at j = 0, 4, 8, 12, 16
     float sp[5], dv[5][3];
+    float mn2 = 3.0e38f, mx2 = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        const float x = 0.25f * q;
-        float s2 = 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; d++) dv[q][d] = 0.f;
+    for (int j = 0; j <= 16; j++) {
+        const float x = j * (1.0f / 16.0f);
+        float s2 = 0.f, vd[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int d = 0; d < D; d++) {
             float v = c[d * P + P - 1];
 #pragma unroll
-            for (int e = P - 2; e >= 0; e--) v = v * x + c[d * P + e];
-            dv[q][d] = v;
-            s2 += v * v;
+            for (int e = P - 2; e >= 0; e--) v = fmaf(v, x, c[d * P + e]);
+            vd[d] = v;
+            s2 = fmaf(v, v, s2);
         }
-        sp[q] = fsqrt(s2);
+        mn2 = fminf(mn2, s2);
+        mx2 = fmaxf(mx2, s2);
+        if ((j & 3) == 0) {
+            sp[j >> 2] = fsqrt(s2);
+#pragma unroll
+            for (int d = 0; d < 3; d++) dv[j >> 2][d] = vd[d];
+        }
     }
     features_common(sp, dv, tol0, f);
     const float Lest = (sp[0] + 4.f * sp[2] + sp[4]) * (1.0f / 6.0f) + 1e-12f;
@@ -121,8 +128,8 @@ __device__ void features_bspline(const float* c, double h, int n_int_curve, cons
     f[12] = cmin;
     f[13] = flog2((float)(h * n_int_curve) + 1e-12f);
     f[14] = P * 0.2f + D * 0.1f;
-    // 17-sample min / max speed (the safety-floor quantity)
-    const float r2 = speed2_ratio<P, D>(c);
+    // 17-sample min / max speed (the safety-floor quantity, as speed2_ratio)
+    const float r2 = mx2 > 0.0f ? mn2 / mx2 : 1.0f;
     f[15] = fsqrt(r2);
     *r2_out = r2;
 }
