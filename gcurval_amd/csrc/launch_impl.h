@@ -212,8 +212,19 @@ hipError_t Launch<Dt, NGL>::seeded_bspline(const LaunchCfg& c, int p, int D, con
 #undef GCV_SEEDED_CASE
 }
 
+// FP64 evaluator of the final Newton step: the polish evaluator for FP32
+// builds (FP64 builds take no such step)
+template <typename Dt>
+using EllipsePolish = std::conditional_t<(sizeof(Dt) < sizeof(double)), EllipsePolishEval, EllipseEval<double>>;
+template <typename Dt>
+static EllipsePolish<Dt> make_polish(const double* a, const double* b, int normalize) {
+    EllipsePolish<Dt> e;
+    static_cast<EllipseEval<double>&>(e) = EllipseEval<double>{a, b, normalize};
+    return e;
+}
+
 template <typename Dt, int NGL, bool kSeeded>
-static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const EllipseEval<double>& ev64,
+static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const EllipsePolish<Dt>& ev64,
                             const SampleArgs& A, const int32_t* blk_curve) {
     // resident blocks per CU, queried once (shared by every context and host
     // thread; a race only repeats the query)
@@ -222,7 +233,7 @@ static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const
     if (per_cu == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, kSeeded>, kSampleBlock, 0) !=
+                &n, k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipsePolish<Dt>, kSeeded>, kSampleBlock, 0) !=
                 hipSuccess || n < 1)
             n = 1;
         per_cu = n;
@@ -239,7 +250,7 @@ static void run_sample_runs(const LaunchCfg& c, const EllipseEval<Dt>& ev, const
     int64_t n_blocks = (int64_t)c.n_cu * per_cu * kRunRounds;
     if (n_blocks * wpb > n_tiles) n_blocks = (n_tiles + wpb - 1) / wpb;
     const int64_t tpw = (n_tiles + n_blocks * wpb - 1) / (n_blocks * wpb);
-    hipLaunchKernelGGL((k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, kSeeded>),
+    hipLaunchKernelGGL((k_sample_runs<Dt, NGL, EllipseEval<Dt>, EllipsePolish<Dt>, kSeeded>),
                        dim3((unsigned)n_blocks), dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve, tpw);
 }
 
@@ -248,14 +259,14 @@ hipError_t Launch<Dt, NGL>::sample_ellipse(const LaunchCfg& c, const double* a, 
                                            const SampleArgs& A, const int32_t* blk_curve) {
     if (A.n_total <= 0) return hipSuccess;
     EllipseEval<Dt> ev{a, b, normalize};
-    EllipseEval<double> ev64{a, b, normalize};
+    const EllipsePolish<Dt> ev64 = make_polish<Dt>(a, b, normalize);
     const int64_t blocks = (A.n_total + kSampleBlock - 1) / kSampleBlock;
     prof_begin(c.owner, "k_sample_ellipse");
     // the common modes (uniform or explicit positions, plain or seeded roots)
     // are compiled in, so the kernel has no branch between its loads
     const bool pos = A.pos != nullptr, seeded = A.root_k != nullptr;
 #define GCV_ELLIPSE_SAMPLE(P, S)                                                                          \
-    hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipseEval<double>, StaticMode<P, S>>),       \
+    hipLaunchKernelGGL((k_sample<Dt, NGL, EllipseEval<Dt>, EllipsePolish<Dt>, StaticMode<P, S>>),         \
                        dim3((unsigned)blocks), dim3(kSampleBlock), 0, c.stream, ev, ev64, A, blk_curve)
     if (pos && seeded) {
         GCV_ELLIPSE_SAMPLE(true, true);

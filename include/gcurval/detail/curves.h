@@ -217,6 +217,35 @@ __device__ __forceinline__ double sin2_2pi(double x, double shift) {
     return h * v;
 }
 
+// sin^2(pi y) as sin2_2pi(double) with V of degree 7 (the Chebyshev expansion
+// truncated after T_7, 1.9e-12 relative), for the final FP64 Newton step of
+// FP32 builds, whose speeds need about 1e-10 relative accuracy (their
+// quadrature carries the FP32 table's 1e-7).
+__device__ __forceinline__ double sin2_2pi_polish(double x, double shift) {
+    const double y = fma(2.0, x, shift);
+    const double e = y - rint(y);
+    const double h = e * e;
+    double v = -0.1238973981824048;
+    v = fma(v, h, 0.8503903438737084);
+    v = fma(v, h, -3.9503437263639913);
+    v = fma(v, h, 13.21296251345923);
+    v = fma(v, h, -30.12231017436645);
+    v = fma(v, h, 42.72840827584517);
+    v = fma(v, h, -32.46969700744408);
+    v = fma(v, h, 9.869604401081775);
+    return h * v;
+}
+
+// sqrt(v) for v >= 0 to about 2^-49 relative (0 below 2^-1000): v_rsq_f64
+// (about 2^-24) refined by one Newton correction g += (v - g^2) / (2 g) with
+// the residual from an FMA; for the final FP64 Newton step of FP32 builds.
+__device__ __forceinline__ double sqrt_polish(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    const double g = v * y;
+    const double g1 = fma(fma(-g, g, v), 0.5 * y, g);
+    return v >= 0x1p-1000 ? g1 : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Ellipse evaluator. With A = 2 pi a', B = 2 pi b' and a', b' the optionally
 // normalised semi-axes, speed(x)^2 = A^2 sin^2(2 pi x) + B^2 cos^2(2 pi x)
@@ -281,6 +310,15 @@ struct EllipseEval {
     }
     // interval-local x -> curve parameter t
     __device__ __forceinline__ double to_u(int, double x) const { return kTwoPi * x; }
+};
+
+// FP64 speed of the final Newton step of FP32 ellipse builds: the degree-7
+// squared sine and the one-correction square root (speed to about 1e-12
+// relative, 15 fewer FP64 instructions per sample than the full evaluator).
+struct EllipsePolishEval : EllipseEval<double> {
+    __device__ __forceinline__ double speed(const Ctx& c, double x) const {
+        return sqrt_polish(fma(double(c.D), sin2_2pi_polish(x, double(c.shift)), double(c.base)));
+    }
 };
 
 // ---------------------------------------------------------------------------

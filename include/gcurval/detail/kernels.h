@@ -1902,7 +1902,8 @@ __global__ __launch_bounds__(256) void k_sample_runs(Ev ev, Ev64 ev64, SampleArg
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             }
             if (g >= next && g < S.s1 && g <= gl) {
-                // target as target_of: j / (N - 1) or j / N, relative to the length
+                // target as target_of (the same FP64 operations, so samples
+                // at root boundaries pick the same root as the per-sample path)
                 const int64_t j = g - S.s0;
                 const double pos = S.den_pos ? (double)j * S.rN : 0.0;
                 const double sv = A.is_relative ? pos * S.cl : pos;

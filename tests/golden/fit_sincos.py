@@ -13,7 +13,8 @@ FP32: V of degree 5 by iteratively reweighted least squares toward minimax
 relative error on a Chebyshev grid in h, then rounded to float.
 FP64: V of degree 9, the Chebyshev expansion of V on h in [0, 1/4] truncated
 after T_9 and converted to powers of h in 40-digit arithmetic (the first
-omitted Chebyshev coefficient is 6.7e-17 of min V).
+omitted Chebyshev coefficient is 6.7e-17 of min V); sin2_2pi_polish, the
+final Newton step's form, truncates after T_7 (1.9e-12).
 Run: python tests/golden/fit_sincos.py
 """
 import mpmath as mp
@@ -88,5 +89,7 @@ def max_rel_error(coefs, dtype):
 if __name__ == "__main__":
     v32 = fit_fp32()
     v64 = cheb_fp64()
+    v64p = cheb_fp64(7)
     print("FP32 V", v32, "max rel err", max_rel_error(v32, np.float32))
     print("FP64 V", v64, "max rel err", max_rel_error(v64, np.float64), "truncation", first_omitted())
+    print("FP64 polish V", v64p, "max rel err", max_rel_error(v64p, np.float64), "truncation", first_omitted(7))

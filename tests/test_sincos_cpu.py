@@ -19,9 +19,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "..", "include", "gcurval", "detail", "curves.h")
 
 
-def _constants(kind):
+def _constants(kind, name="sin2_2pi"):
     text = open(SRC).read()
-    start = text.index(f"__device__ __forceinline__ {kind} sin2_2pi({kind} x")
+    start = text.index(f"__device__ __forceinline__ {kind} {name}({kind} x")
     body = text[start:text.index("\n}\n", start)]
     vals = [float(v) for v in re.findall(r"-?\d+\.\d+(?:e-?\d+)?", body)]
     # fma(2.0, x, shift) comes first; the rest are V's coefficients, highest first
@@ -79,10 +79,22 @@ def test_fp32_polynomial_within_3e7():
     assert _max_rel(_eval(X32, np.float32, -0.5), _truth(X32, mp.cos)) <= 3e-7
 
 
+def test_polish_polynomial_within_3e12():
+    # the degree-7 form of the final FP64 Newton step of FP32 builds
+    c = _constants("double", "sin2_2pi_polish")
+    assert len(c) == 8
+    assert _max_rel(_eval(X64, np.float64, 0.0, c), _truth(X64, mp.sin)) <= 3e-12
+    assert _max_rel(_eval(X64, np.float64, -0.5, c), _truth(X64, mp.cos)) <= 3e-12
+    # and a truncation one degree lower would not meet that bar
+    assert _max_rel(_eval(X64, np.float64, 0.0, c[1:]), _truth(X64, mp.sin)) > 3e-12
+
+
 def test_constants_are_the_derivation():
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
     import fit_sincos as F
     np.testing.assert_array_equal(np.array(_constants("double")[::-1]), np.array(F.cheb_fp64()))
+    np.testing.assert_array_equal(np.array(_constants("double", "sin2_2pi_polish")[::-1]),
+                                  np.array(F.cheb_fp64(7)))
 
 
 def test_planted_coefficient_error_is_caught():
