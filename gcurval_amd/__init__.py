@@ -547,9 +547,9 @@ class ArclengthCalculateGpu:
     def stats(self):
         """Build statistics of the last build: quadratures, refined roots,
         deepest task stack, overflow flag (must be 0), refinement tasks,
-        leaves that still failed the error test at the depth cap, seeded
-        tiles re-run with the full task stack, 64-root wave tiles, and
-        refinement passes over popped tasks."""
+        leaves that still failed the error test at the depth cap, tiles
+        re-run with the full task stack (the counts include both passes),
+        64-root wave tiles, and refinement passes over popped tasks."""
         v = [C.c_uint64() for _ in range(9)]
         self.ctx._check(lib().gcv_table_stats(self.h, *[C.byref(x) for x in v]))
         keys = ("quadratures", "deep_roots", "max_stack", "overflow", "tasks", "depth_capped", "reruns", "tiles",

@@ -1034,6 +1034,10 @@ static Dt eps_of(const gcv_table* t) {
     return (Dt)((Dt)t->o.tolerance / (Dt)(1 << t->log2_nsub));
 }
 
+// Task-stack entries per wave of the uniform and whole-interval subdivision
+// launches (tiles that outgrow it are re-run with the full stack).
+constexpr int kSmallStack = 256;
+
 // Uniform build: every interval starts from 2^ls roots (the plain method when
 // ls = log2 n_subdivided); optional per-interval quadrature counts.
 template <typename Dt, int NGL>
@@ -1045,18 +1049,30 @@ static hipError_t build_uniform(gcv_table* t, int ls, int32_t* cost) {
     A.log2_nsub = ls;
     A.eps0 = eps_of<Dt>(t);
     A.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, ls, sizeof(Dt) == 4);
-    A.stack_cap = 64 * (A.max_depth + 2);
+    // a small task stack (more resident waves: the full stack of 64 (depth +
+    // 2) entries holds 4 waves per SIMD in LDS); the rare tile that outgrows
+    // it is re-run with the full stack (c1 build 0.39 -> 0.32 ms, c3
+    // subdivision 1.22 -> 1.07 ms)
+    A.stack_cap = std::min(kSmallStack, 64 * (A.max_depth + 2));
     A.int_cost = cost;
+    A.rerun = t->rerun_list;
+    A.rerun_count = t->stats + dev::kStatWords;
+    A.rerun_self = 1;
+    A.ls_plain = t->log2_nsub;
+    A.base_depth = t->o.max_depth;
     if (t->kind == KIND_ELLIPSE) {
         e = Launch<Dt, NGL>::subdivide_ellipse(L, t->a, t->b, t->o.normalize, A);
     } else {
         const BSplineTableDev T = bspline_dev(t);
+        int ci = 0;
         for (const ClassRange& r : t->classes) {
             dev::SubdivArgs<Dt> Ar = A;
             if (t->class_list) {
                 Ar.int_list = t->class_list + r.offset;
                 Ar.n_slots = r.count;
             }
+            Ar.rerun = t->rerun_list + r.offset;
+            Ar.rerun_count = t->stats + dev::kStatWords + (int64_t)(ci++ % dev::kQueueClasses) * dev::kQueueStride;
             e = Launch<Dt, NGL>::subdivide_bspline(L, r.p, r.D, T, Ar);
             if (e != hipSuccess) return e;
         }
@@ -1148,7 +1164,11 @@ static hipError_t build_seeded(gcv_table* t, const int8_t* kk, int32_t* cost, bo
         S.whole.log2_nsub = 6;
         S.whole.eps0 = eps_of<Dt>(t);
         S.whole.max_depth = dev::effective_depth(t->o.max_depth, t->log2_nsub, 6, f32);
-        S.whole.stack_cap = 64 * (S.whole.max_depth + 2);
+        // small stack; outgrown tiles join the bins' re-run list
+        S.whole.stack_cap = std::min(kSmallStack, 64 * (S.whole.max_depth + 2));
+        S.whole.rerun = t->rerun_list + off;
+        S.whole.rerun_count = q;
+        S.whole.rerun_self = 0;
         S.whole.int_cost = cost;
         S.whole.root_off = t->root_off;
         S.rerun = subdiv_args<Dt>(t);

@@ -67,6 +67,20 @@ static hipError_t run_subdivide_as(const LaunchCfg& c, const Ev& ev, const Subdi
                        A);
     e = hipGetLastError();
     prof_end(c.owner, c.stream);
+    if (e != hipSuccess || !A.rerun || !A.rerun_self) return e;
+    // tiles that outgrew the small stack, again with the full one (the list
+    // is usually empty and the waves return at once; its length is read on
+    // the device)
+    SubdivArgs<Dt> F = A;
+    F.stack_cap = 64 * (A.max_depth + 2);
+    const size_t flds = (size_t)subdiv_lds_per_wave<Dt>(F.stack_cap);
+    const int64_t fblocks = std::min<int64_t>((int64_t)c.n_cu, A.n_slots);
+    prof_begin(c.owner, "k_subdivide_rerun", c.stream);
+    hipLaunchKernelGGL((k_subdivide_rerun<Dt, NGL, Ev, false>), dim3((unsigned)std::max<int64_t>(fblocks, 1)), dim3(64),
+                       flds, c.stream, ev, F, (const int32_t*)A.rerun, (const unsigned long long*)A.rerun_count,
+                       (const int8_t*)nullptr, A.ls_plain, A.base_depth);
+    e = hipGetLastError();
+    prof_end(c.owner, c.stream);
     return e;
 }
 

@@ -67,7 +67,16 @@ struct SubdivArgs {
     int32_t* int_cost;          // optional: quadratures spent on each interval
     int stack_cap;              // task-stack entries per wave (LDS)
     unsigned long long* stats;
+    // optional: a tile whose refinement outgrows stack_cap lists its
+    // intervals here (count in *rerun_count) for a re-run with the full stack
+    // of 64 (max_depth + 2) entries; null: the overflow flag is raised instead
+    int32_t* rerun;
+    unsigned long long* rerun_count;
+    int rerun_self;             // the launcher runs the re-run kernel itself (uniform builds)
+    int ls_plain;               // log2 roots of the plain build and its depth cap, for the re-run
+    int base_depth;
 };
+
 
 // Task-stack key: float builds (depth <= 20) fit root | level | index in 32
 // bits; double builds use 64 bits.
@@ -170,6 +179,20 @@ __device__ __forceinline__ int clamp_k_seed(int k) { return k < 0 ? 0 : (k > 6 ?
 __device__ __forceinline__ int64_t readfirstlane64(int64_t v) {
     return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)v >> 32)) << 32) |
                      (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
+// List the intervals of a tile whose refinement outgrew the stack (one lane
+// each: the last root of every interval of the tile) for the full-stack
+// re-run; v: the lane's slot holds an interval.
+__device__ __forceinline__ void list_for_rerun(bool v, int i, int ls, int32_t* rerun,
+                                               unsigned long long* rerun_count) {
+    const int lane = lane_id();
+    const bool mine = v && ((lane & ((1 << ls) - 1)) == (1 << ls) - 1);
+    const uint64_t m = ballot(mine);
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(rerun_count, (unsigned long long)popc64(m));
+    base = (unsigned long long)readfirstlane64((int64_t)__shfl((long long)base, 0, 64));
+    if (mine) rerun[base + mbcnt(m)] = i;
 }
 
 // A wave's refinement-task stack in LDS: keys and parent lengths.
@@ -476,10 +499,18 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
     for (int64_t tile = gwave; tile < n_tiles; tile += nwaves) {
         const Stage nxt = fetch(i_nxt, v_nxt);
         i_nxt = interval_of(tile + 2 * nwaves, v_nxt);
-        // the stack holds 64 (max_depth + 2) entries here, so it cannot overflow
+        // a tile that outgrows a small stack is listed for the full-stack
+        // re-run (the statistics count the work of both passes); with the
+        // full stack of 64 (max_depth + 2) entries it cannot overflow
         if (subdivide_tile<Dt, NGL, Ev, kWhole>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, ls, A.max_depth, W,
-                                                scan, wc))
-            wc.overflow = 1;
+                                                scan, wc)) {
+            if (A.rerun) {
+                list_for_rerun(cur.v, cur.i, ls, A.rerun, A.rerun_count);
+                wc.n_rerun++;
+            } else {
+                wc.overflow = 1;
+            }
+        }
         cur = nxt;
     }
     wc.flush(A.stats);
@@ -562,12 +593,7 @@ __device__ __forceinline__ void subdivide_bin(const Ev& ev, const SubdivArgs<Dt>
         if (subdivide_tile<Dt, NGL, Ev, false, true>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, K, max_depth, W,
                                                      scan, wc, B.ls_plain)) {
             // list the tile's intervals (one lane each) for the full-stack re-run
-            const bool mine = cur.v && ((lane & ((1 << K) - 1)) == (1 << K) - 1);
-            const uint64_t m = ballot(mine);
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(B.rerun_count, (unsigned long long)popc64(m));
-            base = (unsigned long long)readfirstlane64((int64_t)__shfl((long long)base, 0, 64));
-            if (mine) B.rerun[base + mbcnt(m)] = cur.i;
+            list_for_rerun(cur.v, cur.i, K, B.rerun, B.rerun_count);
             wc.n_rerun++;
         }
         cur = nxt;
@@ -603,9 +629,11 @@ struct SeededLaunch {
     const int8_t* root_k;
 };
 
-// The intervals listed by k_subdivide_bins, one wave tile each (root count
-// 2^root_k[i], the lanes past it idle), with the full task stack in LDS.
-template <typename Dt, int NGL, class Ev>
+// The intervals listed by k_subdivide_bins or k_subdivide, one wave tile
+// each (root count 2^root_k[i], or 2^A.log2_nsub without root_k; the lanes
+// past it idle), with the full task stack in LDS. kGuard: seeded builds
+// (dip_forces), as the kernels whose tiles are re-run.
+template <typename Dt, int NGL, class Ev, bool kGuard = true>
 __global__ __launch_bounds__(64) void k_subdivide_rerun(Ev ev, SubdivArgs<Dt> A, const int32_t* list,
                                                         const unsigned long long* count, const int8_t* root_k,
                                                         int ls_plain, int base_depth) {
@@ -617,13 +645,14 @@ __global__ __launch_bounds__(64) void k_subdivide_rerun(Ev ev, SubdivArgs<Dt> A,
     WaveCounters wc;
     for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
         const int i = load_s(list + j);
-        const int k = load_s(root_k + i);
+        const int k = root_k ? (int)load_s(root_k + i) : A.log2_nsub;
+        const int64_t rb = A.root_off ? load_s(A.root_off + i) : (int64_t)i << k;
         const typename Ev::Ctx ctx = ev.load(i);
         const double sc = ev.scale(i);
         const LdsSegScan scan(W.scan, k);
-        if (subdivide_tile<Dt, NGL, Ev, false, true>(ev, A, ctx, i, lane < (1 << k), load_s(A.root_off + i), sc, k,
-                                                     effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc,
-                                                     ls_plain))
+        if (subdivide_tile<Dt, NGL, Ev, false, kGuard>(ev, A, ctx, i, lane < (1 << k), rb, sc, k,
+                                                       effective_depth(base_depth, ls_plain, k, kF32), W, scan, wc,
+                                                       ls_plain))
             wc.overflow = 1;  // unreachable: the stack holds 64 (depth + 2) entries
     }
     wc.flush(A.stats);

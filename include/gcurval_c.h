@@ -185,10 +185,10 @@ int gcv_table_device_views(gcv_table* t, const double** int_cum, const double** 
 /* Build statistics of the last build: quadratures evaluated, refined roots,
  * maximum task-stack depth, stack overflow flag (must be 0), refinement tasks,
  * leaves that still failed the error test when the depth cap stopped them,
- * seeded-build tiles whose refinement outgrew the small task stack of the
- * bins kernel (their intervals are re-run with the full stack), 64-root wave
- * tiles, and refinement passes (wave passes over popped tasks). Any pointer
- * may be NULL. */
+ * tiles whose refinement outgrew the small task stack of the first pass
+ * (256 entries; their intervals are re-run with the full stack, and the
+ * counts include both passes' work), 64-root wave tiles, and refinement
+ * passes (wave passes over popped tasks). Any pointer may be NULL. */
 int gcv_table_stats(gcv_table* t, uint64_t* quadratures, uint64_t* deep_roots, uint64_t* max_stack,
                     uint64_t* overflow, uint64_t* tasks, uint64_t* depth_capped, uint64_t* reruns,
                     uint64_t* tiles, uint64_t* passes);

@@ -16,6 +16,15 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+def assert_quads(cost, st):
+    """Per-interval quadrature counts add up to the build's: exactly without
+    re-runs; a tile re-run with the full task stack also counted its first pass."""
+    if st["reruns"] == 0:
+        assert int(cost.sum()) == st["quadratures"]
+    else:
+        assert int(cost.sum()) < st["quadratures"]
+
+
 
 def _params(seed):
     rng = np.random.default_rng(seed)
@@ -146,7 +155,7 @@ def test_learned_build_matches_truth(ctx, mixed):
             keff = np.maximum(k_used, eng.seed_floor()).astype(np.int64)
             cost = d_cost.download()
             assert np.all(cost >= 3 * (1 << keff))
-            assert int(cost.sum()) == eng.stats()["quadratures"]
+            assert_quads(cost, eng.stats())
         d_cost.free()
 
 

@@ -350,8 +350,35 @@ def test_tolerance_below_float_floor_saturates_depth(ctx):
         st = eng.stats()
         assert st["overflow"] == 0
         assert st["max_stack"] <= 64 * (10 + 1)
+        # the first pass runs a small task stack (256 entries): a tile that
+        # outgrows it is re-run with the full one, which alone records a
+        # deeper stack
+        assert (st["max_stack"] > 256) == (st["reruns"] > 0)
         Lt, _ = O.Truth(b).lengths()
         assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["ellipse", "bspline"])
+def test_rerun_of_deep_tiles_with_the_full_stack(ctx, kind):
+    # FP64 below its rounding floor: every root refines to the depth cap, so
+    # each tile's task stack grows to 64 (depth + 1) entries, past the first
+    # pass's 256; the tiles are re-run with the full stack (one re-run list
+    # per class for the mixed B-spline batch)
+    if kind == "ellipse":
+        b = W.ellipse_batch(6, seed=82)
+    else:
+        b = W.bspline_batch(4, seed=83, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(5, 8))
+    with build(ctx, b, precision=G.GCV_FP64, tolerance=1e-300, max_depth=6, normalize=True) as eng:
+        st = eng.stats()
+        assert st["overflow"] == 0
+        assert 1 <= st["reruns"] <= b.n_intervals
+        assert 256 < st["max_stack"] <= 64 * (6 + 1)
+        Lt, _ = O.Truth(b).lengths()
+        assert np.max(np.abs(eng.getAllCurveArclengths() - Lt) / Lt) <= 1e-9
+        counts = 16 * np.asarray(b.intervals_per_curve()) if kind == "bspline" else np.full(b.n_curve, 64)
+        u = eng.arclengthReverse(counts, None, True)
+        ut, _ = O.Truth(b).reverse(counts, O.uniform_positions(counts))
+        assert np.max(np.abs(u - ut)) <= 1e-9
 
 
 def test_max_depth_one_refines_once(ctx):

@@ -15,6 +15,15 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+def assert_quads(cost, st):
+    """Per-interval quadrature counts add up to the build's: exactly without
+    re-runs; a tile re-run with the full task stack also counted its first pass."""
+    if st["reruns"] == 0:
+        assert int(cost.sum()) == st["quadratures"]
+    else:
+        assert int(cost.sum()) < st["quadratures"]
+
+
 
 @pytest.fixture(scope="module")
 def cubic():
@@ -51,7 +60,7 @@ def test_uniform_k_builds(ctx, cubic, k):
         assert np.max(np.abs(Lg - L) / L) <= 1e-6
         cost = d_cost.download()
         st = eng.stats()
-        assert int(cost.sum()) == st["quadratures"]
+        assert_quads(cost, st)
         assert np.all(cost >= 3 * (1 << k))
         assert st["overflow"] == 0
         check_samples(eng, b, counts, ut, segt, L, cum)
@@ -73,7 +82,7 @@ def test_seeded_random_k_matches_truth(ctx, cubic):
         cost = d_cost.download()
         keff = np.maximum(k, eng.seed_floor()).astype(np.int64)
         assert np.all(cost >= 3 * (1 << keff))
-        assert int(cost.sum()) == eng.stats()["quadratures"]
+        assert_quads(cost, eng.stats())
         _, rp, _, _, _ = eng.tables()
         assert rp.size == int(np.sum(1 << keff))
         # dense roots: interval i's block starts at the sum of 2^k_j over j < i
