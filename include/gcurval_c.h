@@ -302,13 +302,14 @@ typedef struct gcv_tune {
     double best_build_ms;
     double default_build_ms; /* 4 waves per block, full grid, one stream */
     int n_tried;
-    int bin_streams;         /* learned builds: streams the per-bin subdivision launches are spread over */
+    int bin_streams;         /* learned builds: streams the per-class passes are spread over */
 } gcv_tune;
 int gcv_table_autotune(gcv_table* t, gcv_mlp* m, int reps, gcv_tune* out);
 /* Set the launch shape directly (0 waves per block: the default). bin_streams:
-   1 = the learned build's per-bin launches on the context stream, 2 or 3 =
-   spread over that many streams and joined before the build returns, 0 = the
-   default (3 for B-spline batches of several degree/dimension classes, else 1). */
+   1 = a build's per-class passes (coefficients, subdivision launches) on the
+   context stream, 2 or 3 = spread over that many streams and joined before
+   the build returns, 0 = the default (2 for B-spline batches of several
+   degree/dimension classes, else 1). */
 int gcv_table_set_launch(gcv_table* t, int sub_waves_per_block, int sub_grid_pct, int bin_streams);
 /* Cumulative-scan kernel of B-spline builds (no counterpart in the reference,
    which scans per curve, kernel/generic.inl.cuh:117-137): 0 = by curve length
