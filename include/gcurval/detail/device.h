@@ -40,6 +40,16 @@ template <> struct GL<5> {
                                     0.23692688505618908, 0.23692688505618908};
 };
 
+// 8 nodes: the cross-check rule of GL-5 seeded builds (below)
+template <> struct GL<8> {
+    static constexpr double x[8] = {-0.18343464249564978, 0.18343464249564978, -0.525532409916329,
+                                    0.525532409916329,    -0.7966664774136267, 0.7966664774136267,
+                                    -0.9602898564975362,  0.9602898564975362};
+    static constexpr double w[8] = {0.36268378337836177, 0.36268378337836177, 0.31370664587788705,
+                                    0.31370664587788705, 0.22238103445337434, 0.22238103445337434,
+                                    0.10122853629037669, 0.10122853629037669};
+};
+
 // An evaluator with `static constexpr bool kRule = true` supplies its own
 // length rule `length(ctx, x0, x1)` (user plug-ins: the traits'
 // CalculateFunction); the built-in curve families are integrated here.
@@ -117,6 +127,31 @@ template <typename Dt>
 __device__ __forceinline__ bool dip_forces(int m, int ls_plain, Dt mn, Dt mx) {
     const int mid = ls_plain < 4 ? ls_plain : 4;
     return (m < ls_plain && mn < Dt(0.05) * mx) || (m < mid && mn < Dt(0.2) * mx);
+}
+
+// Cross-check of wide segments in seeded builds. The halves-versus-whole test
+// compares two estimates of one rule; on a segment much wider than the
+// distance to the integrand's nearest complex singularity (|C'| small next
+// to |C''| times the width, not necessarily a dip) the two rule errors can
+// agree, so the test passes with both estimates wrong (one dataset interval:
+// 1.3e-6 of its curve's length from two roots). A segment wider than
+// 2^-kCrossBelow of its interval is therefore accepted as a leaf only when
+// L + R also agrees with a higher-order rule over the whole segment (GL-5
+// for GL-3 builds, GL-8 for GL-5 builds) to the same tolerance. Narrower
+// segments keep the plain method's test alone.
+constexpr int kCrossBelow = 3;
+template <int NGL> struct CrossRule;
+template <> struct CrossRule<3> { static constexpr int n = 5; };
+template <> struct CrossRule<5> { static constexpr int n = 8; };
+
+template <int NGL, class Ev, typename Dt>
+__device__ __forceinline__ bool cross_fails(const Ev& ev, const typename Ev::Ctx& c, Dt x0, Dt x1, Dt lr, Dt tol) {
+    if constexpr (HasLengthRule<Ev>::value) {
+        return false;
+    } else {
+        const Dt d = quad<CrossRule<NGL>::n>(ev, c, x0, x1) - lr;
+        return (d < Dt(0) ? -d : d) > tol;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -279,6 +279,18 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
         L = quad_mm<NGL>(ev, ctx, x0, xm, mn, mx);
         R = quad_mm<NGL>(ev, ctx, xm, x1, mn, mx);
         dip = dip_forces(ls, ls_plain, mn, mx);
+        if (ls < kCrossBelow) {
+            // only where the other tests accept the root (the same decision)
+            const bool need = valid && !dip && !(dabs((L + R) - P) > A.eps0);
+            if (ballot(need)) {
+                // bounds through an empty asm: the check's node positions are
+                // formed here per tile, not hoisted into registers held for
+                // the whole kernel
+                Dt cx0 = x0, cx1 = x1;
+                asm volatile("" : "+v"(cx0), "+v"(cx1));
+                if (need) dip = cross_fails<NGL>(ev, ctx, cx0, cx1, L + R, A.eps0);
+            }
+        }
     } else {
         L = quad<NGL>(ev, ctx, x0, xm);
         R = quad<NGL>(ev, ctx, xm, x1);
@@ -334,6 +346,12 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
             }
             const Dt lr = L2 + R2;
             const bool over = dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
+            if constexpr (kGuard) {
+                const bool wide = take && ls + l < kCrossBelow && !over && !dip2;
+                if (ballot(wide)) {
+                    if (wide) dip2 = cross_fails<NGL>(ev, cr, a0, a1, lr, A.eps0 * pow2m(Dt(0), l));
+                }
+            }
             bool f2 = take && l < max_depth && (over || dip2);
             if (take && l >= max_depth && over) wc.n_cap++;  // a leaf that still fails: the depth cap
             const uint64_t tmask = ballot(take);
@@ -1405,7 +1423,7 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
     task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
     Dt P = quad<NGL>(ev, c, x0, x1);
     double acc = 0.0;
-    for (int guard = 0; guard < (1 << 26); guard++) {
+    for (int it = 0; it < (1 << 26); it++) {
         task_bounds<Dt>(k, l, idx, ls, x0, xm, x1);
         Dt L, R;
         bool dip = false;
@@ -1419,7 +1437,11 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
             R = quad<NGL>(ev, c, xm, x1);
         }
         const Dt lr = L + R;
-        if (l < max_depth && (dabs(lr - P) > eps0 * pow2m(Dt(0), l) || dip)) {
+        bool split = dabs(lr - P) > eps0 * pow2m(Dt(0), l) || dip;
+        // the build's cross-check of wide segments, where the other tests accept
+        if (guard && !split && ls + l < kCrossBelow)
+            split = cross_fails<NGL>(ev, c, x0, x1, lr, eps0 * pow2m(Dt(0), l));
+        if (l < max_depth && split) {
             stackR[l + 1] = R;
             l++;
             idx = 2u * idx;

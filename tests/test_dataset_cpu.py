@@ -1,6 +1,7 @@
 """CPU tests of the B-spline dataset loader on the committed fixture (the first
 300 curves of dataset/BSpline/Bundle_0.txt, verbatim) and the whole-dataset
 totals recorded when the fixture was made (tests/golden/make_dataset_fixture.py)."""
+import gzip
 import json
 import os
 
@@ -58,3 +59,24 @@ def test_whole_dataset_totals_recorded():
     meta = json.load(open(os.path.join(HERE, "golden", "dataset_meta.json")))
     # the reference's dataset: 100 files x 1000 curves, cubic, 499,339 knot intervals
     assert meta["n_curve"] == 100000 and meta["n_intervals"] == 499339 and meta["degrees"] == [3]
+
+
+def test_ten_bundle_fixtures(tmp_path):
+    """Bundle_0..9 verbatim (gzip): 10,000 curves, the totals recorded when
+    the fixtures were made, and Bundle_0's first 300 records equal to the
+    300-curve fixture's."""
+    meta = json.load(open(os.path.join(HERE, "golden", "dataset_meta.json")))
+    paths = []
+    for j in range(meta["fixture_bundles"]):
+        with open(os.path.join(HERE, "golden", f"dataset_bundle{j}.txt.gz"), "rb") as f:
+            text = gzip.decompress(f.read())
+        p = tmp_path / f"Bundle_{j}.txt"
+        p.write_bytes(text)
+        paths.append(str(p))
+    b = D.load_bspline_dataset(paths)
+    assert b.n_curve == meta["bundles_n_curve"] == 10000
+    assert b.n_intervals == meta["bundles_n_intervals"] == 49657
+    head = D.load_bspline_dataset([FIX])
+    n = int(head.knot_off[-1])
+    np.testing.assert_array_equal(b.knots[:n], head.knots)
+    np.testing.assert_array_equal(b.ctrl[:int(head.ctrl_off[-1])], head.ctrl)
