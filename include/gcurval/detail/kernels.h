@@ -174,6 +174,29 @@ __device__ __forceinline__ void task_bounds(int kr, int l, uint32_t idx, int ls,
 template <typename Dt>
 __device__ __forceinline__ Dt dabs(Dt v) { return v < Dt(0) ? -v : v; }
 
+// Tolerance floor of segments no wider than a plain root (ls + l >= ls_plain)
+// inside roots wider than the plain method's (ls < ls_plain: seeded and
+// uniform-k builds). Their tolerance per width is 2^(ls_plain - ls) times
+// tighter than the plain method's, and |L + R - P| cannot fall below the
+// rounding of the sums (FP32, one root per interval: below the tolerance only
+// where the speed is under about 0.2 in normalised units), so without a floor
+// the test chases rounding noise down to the depth cap (a float32 simulation
+// on 49,657 dataset intervals from one root each: 298M quadratures, 7.5M
+// with the floor, 9.5M for the plain method; lengths unchanged). Such a
+// segment is accepted once |L + R - P| is within 4 units of rounding of L + R
+// (2^-21 relative in FP32, 2^-50 in FP64). Wider segments keep the
+// tolerance alone: there |L + R - P| is no reliable bound on the error of
+// L + R, which the sampler's inversion inside the leaf carries.
+template <typename Dt>
+__device__ __forceinline__ Dt rounding_floor(int ls, int l, int ls_plain) {
+    return (ls < ls_plain && ls + l >= ls_plain) ? (sizeof(Dt) == 4 ? Dt(0x1p-21) : Dt(0x1p-50)) : Dt(0);
+}
+template <typename Dt>
+__device__ __forceinline__ Dt tol_of(Dt eps, Dt rel, Dt sum) {
+    const Dt f = rel * dabs(sum);
+    return f > eps ? f : eps;
+}
+
 __device__ __forceinline__ int clamp_k_seed(int k) { return k < 0 ? 0 : (k > 6 ? 6 : k); }
 
 __device__ __forceinline__ int64_t readfirstlane64(int64_t v) {
@@ -268,6 +291,14 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
     const int ipw_log2 = 6 - ls;
     const int k = lane & (nsub - 1);
     const TaskStack<Dt>& stk = W.stack;
+    // tolerance of a task at level l (whole-interval tiles hold plain roots:
+    // no floor, and no code for it)
+    auto tol = [&](int l, Dt sum) -> Dt {
+        if constexpr (kWhole)
+            return A.eps0 * pow2m(Dt(0), l);
+        else
+            return tol_of(A.eps0 * pow2m(Dt(0), l), rounding_floor<Dt>(ls, l, ls_plain), sum);
+    };
 
     Dt x0, xm, x1;
     task_bounds<Dt>(k, 0, 0u, ls, x0, xm, x1);
@@ -345,11 +376,12 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
                 R2 = quad<NGL>(ev, cr, am, a1);
             }
             const Dt lr = L2 + R2;
-            const bool over = dabs(lr - par) > A.eps0 * pow2m(Dt(0), l);
+            const Dt tol2 = tol(l, lr);
+            const bool over = dabs(lr - par) > tol2;
             if constexpr (kGuard) {
                 const bool wide = take && ls + l < kCrossBelow && !over && !dip2;
                 if (ballot(wide)) {
-                    if (wide) dip2 = cross_fails<NGL>(ev, cr, a0, a1, lr, A.eps0 * pow2m(Dt(0), l));
+                    if (wide) dip2 = cross_fails<NGL>(ev, cr, a0, a1, lr, tol2);
                 }
             }
             bool f2 = take && l < max_depth && (over || dip2);
@@ -521,7 +553,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ev ev, SubdivArgs<Dt> A) {
         // re-run (the statistics count the work of both passes); with the
         // full stack of 64 (max_depth + 2) entries it cannot overflow
         if (subdivide_tile<Dt, NGL, Ev, kWhole>(ev, A, cur.ctx, cur.i, cur.v, cur.rb, cur.sc, ls, A.max_depth, W,
-                                                scan, wc)) {
+                                                scan, wc, A.ls_plain)) {
             if (A.rerun) {
                 list_for_rerun(cur.v, cur.i, ls, A.rerun, A.rerun_count);
                 wc.n_rerun++;
@@ -1437,10 +1469,10 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
             R = quad<NGL>(ev, c, xm, x1);
         }
         const Dt lr = L + R;
-        bool split = dabs(lr - P) > eps0 * pow2m(Dt(0), l) || dip;
+        const Dt tol = tol_of(eps0 * pow2m(Dt(0), l), rounding_floor<Dt>(ls, l, ls_plain), lr);
+        bool split = dabs(lr - P) > tol || dip;
         // the build's cross-check of wide segments, where the other tests accept
-        if (guard && !split && ls + l < kCrossBelow)
-            split = cross_fails<NGL>(ev, c, x0, x1, lr, eps0 * pow2m(Dt(0), l));
+        if (guard && !split && ls + l < kCrossBelow) split = cross_fails<NGL>(ev, c, x0, x1, lr, tol);
         if (l < max_depth && split) {
             stackR[l + 1] = R;
             l++;
