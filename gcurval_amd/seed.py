@@ -225,10 +225,12 @@ def start_targets(log2_err: np.ndarray) -> np.ndarray:
 
 
 def training_batches(seed: int = 100):
-    """Synthetic batches spanning the classes the predictor serves (about 2.2M
+    """Synthetic batches spanning the classes the predictor serves (about 2.5M
     intervals): cubic 3-D random walks with near-cusps, mixed degrees 2/3/5 in
-    2-D/3-D with near-collinear loops, degrees 1/4, and ellipses. The seeds
-    differ from every benchmark and test batch."""
+    2-D/3-D with near-collinear loops, degrees 1/4, few scattered control
+    points with random knots (the reference dataset's regime, cubic 2-D and
+    mixed), and ellipses. The seeds differ from every benchmark and test
+    batch."""
     from . import workloads as W
     out = []
     for j in range(3):
@@ -239,6 +241,8 @@ def training_batches(seed: int = 100):
                                    n_ctrl_range=(16, 1024), hairpin_frac=0.2))
     out.append(W.bspline_batch(1000, seed=seed + 11, degrees=(1, 4), dims=(2, 3), n_ctrl_range=(8, 256),
                                hairpin_frac=0.1))
+    out.append(W.bspline_scattered(30000, seed=seed + 30, degrees=(3,), dims=(2,)))
+    out.append(W.bspline_scattered(30000, seed=seed + 31, degrees=(2, 3, 5), dims=(2, 3), n_ctrl_range=(4, 32)))
     out.append(W.ellipse_batch(200000, seed=seed + 20))
     return out
 

@@ -145,6 +145,36 @@ def bspline_batch(n_curve: int, seed: int = 413, degrees=(3,), dims=(3,), n_ctrl
     return BSplineBatch(deg, dim, knot_off, ctrl_off, knots, ctrl)
 
 
+def bspline_scattered(n_curve: int, seed: int = 413, degrees=(3,), dims=(2,), n_ctrl_range=(4, 16),
+                      f32_inputs: bool = True) -> BSplineBatch:
+    """Clamped B-splines with few control points scattered uniformly in a box
+    and random interior knots: the regime of the reference's B-spline dataset
+    (dataset/BSpline: cubic, 4..12 control points at unrelated positions,
+    non-uniform knots), where one knot interval spans a large, sharply turning
+    part of its curve. Control-point counts uniform on n_ctrl_range."""
+    rng = np.random.default_rng(seed)
+    lo, hi = n_ctrl_range
+    stride = max(dims)
+    deg = rng.choice(np.asarray(degrees, np.int32), n_curve).astype(np.int32)
+    dim = rng.choice(np.asarray(dims, np.int32), n_curve).astype(np.int32)
+    n_ctrl = np.maximum(rng.integers(lo, hi + 1, n_curve), deg + 1).astype(np.int64)
+    knot_off = np.zeros(n_curve + 1, np.int64)
+    ctrl_off = np.zeros(n_curve + 1, np.int64)
+    np.cumsum(n_ctrl + deg + 1, out=knot_off[1:])
+    np.cumsum(n_ctrl, out=ctrl_off[1:])
+    knots = np.empty(knot_off[-1], np.float64)
+    ctrl = np.zeros((ctrl_off[-1], stride), np.float64)
+    for c in range(n_curve):
+        p, D, n = int(deg[c]), int(dim[c]), int(n_ctrl[c])
+        inner = np.sort(rng.uniform(0.0, 1.0, n - p - 1))
+        knots[knot_off[c]:knot_off[c + 1]] = np.concatenate([np.zeros(p + 1), inner, np.ones(p + 1)])
+        ctrl[ctrl_off[c]:ctrl_off[c + 1], :D] = rng.uniform(0.0, 20.0, (n, D))
+    if f32_inputs:
+        knots = knots.astype(np.float32).astype(np.float64)
+        ctrl = ctrl.astype(np.float32).astype(np.float64)
+    return BSplineBatch(deg, dim, knot_off, ctrl_off, knots, ctrl)
+
+
 def bspline_c2(n_curve: int = 20000, seed: int = 413) -> BSplineBatch:
     """20,000 cubic 3-D curves, 8..2048 control points, 10% near-cusp steps."""
     return bspline_batch(n_curve, seed, degrees=(3,), dims=(3,), n_ctrl_range=(8, 2048), cusp_frac=0.1)
