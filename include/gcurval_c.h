@@ -140,7 +140,10 @@ int gcv_table_build(gcv_table* t);
 
 /* Uniform build with 2^log2_roots roots per knot interval (0..6); the plain
  * build is log2_roots = log2 n_subdivided. Every root is tested with the
- * plain level-0 tolerance (tolerance / n_subdivided), halving per level.
+ * plain level-0 tolerance (tolerance / n_subdivided), halving per level;
+ * in roots wider than the plain ones, a segment no wider than a plain root
+ * is also accepted once the halves-versus-whole difference is within 4
+ * units of rounding of its length (the test cannot resolve less).
  * d_cost (optional, device int32 [n_intervals]) receives the quadratures
  * each interval used. */
 int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_cost);
@@ -148,9 +151,13 @@ int gcv_table_build_uniform(gcv_table* t, int log2_roots, int32_t* d_cost);
 /* Seeded build: knot interval i starts from 2^k_i roots, d_k device int8
  * [n_intervals] with k_i in 0..6 (for example predicted by the MLP); every
  * root still passes the plain adaptive error test (tolerance / n_subdivided
- * at level 0, halving per level), so refinement falls back to the plain
- * method where the seed is too coarse. No counterpart in the reference (its pre-split is the fixed
- * N_SUBDIVIDED of gcurval/config.h:20-24). d_cost as above. */
+ * at level 0, halving per level, with the rounding floor above), so
+ * refinement falls back to the plain method where the seed is too coarse.
+ * Segments wider than a plain root are also refined where their node speeds
+ * dip, and segments wider than 1/8 of their interval must agree with a
+ * higher-order rule (GL-5 for GL-3 builds, GL-8 for GL-5 builds). No
+ * counterpart in the reference (its pre-split is the fixed N_SUBDIVIDED of
+ * gcurval/config.h:20-24). d_cost as above. */
 int gcv_table_build_seeded(gcv_table* t, const int8_t* d_k, int32_t* d_cost);
 
 /* Safety floor of the last seeded build (device int8 [n_intervals]): the
