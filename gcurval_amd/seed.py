@@ -193,13 +193,19 @@ def cost_matrix(ctx: Context, batch, precision=GCV_FP32, with_start: bool = Fals
     return (cost, floor, start) if with_start else (cost, floor)
 
 
-def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "clean") -> np.ndarray:
+REFINE_WEIGHT = int(os.environ.get("GCV_SEED_REFINE_WEIGHT", "6"))
+
+
+def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "weighted") -> np.ndarray:
     """Per-interval targets from the uniform-k quadrature counts.
 
     y[:, 0]: with policy "clean", the smallest k at or above the floor whose
     2^k roots all pass the level-0 test (3 quadratures per root; k = 6 when
     none does), so the table has almost no refined roots and the sampler no
-    tree re-walks; with "cheapest", the k of fewest quadratures.
+    tree re-walks; with "cheapest", the k of fewest quadratures; with
+    "weighted", the k of least cost when a refinement quadrature counts
+    REFINE_WEIGHT level-0 ones (refinement passes run few lanes and the
+    sampler re-walks refined roots).
     y[:, 1] = 1 when the interval needs no refinement at that k.
     """
     ks = np.arange(MAX_K + 1)[:, None]
@@ -209,6 +215,10 @@ def labels_from_costs(cost: np.ndarray, floor: np.ndarray, policy: str = "clean"
         k = np.where(clean.any(0), np.argmax(clean, axis=0), MAX_K)
     elif policy == "cheapest":
         k = np.argmin(np.where(ok, cost, np.iinfo(np.int64).max), axis=0)
+    elif policy == "weighted":
+        base = 3 * (1 << ks)
+        w = base + REFINE_WEIGHT * np.maximum(cost - base, 0)
+        k = np.argmin(np.where(ok, w, np.iinfo(np.int64).max), axis=0)
     else:
         raise ValueError(f"unknown label policy {policy}")
     c = cost[k, np.arange(cost.shape[1])]
@@ -247,7 +257,7 @@ def training_batches(seed: int = 100):
     return out
 
 
-def make_dataset(ctx: Context, batches, mlp: SeedMLP, policy: str = "clean"):
+def make_dataset(ctx: Context, batches, mlp: SeedMLP, policy: str = "weighted"):
     """Feature rows and targets [n][4]: the seed labels, then the start targets
     measured on the uniform-k table of each interval's label k."""
     X, Y = [], []
@@ -302,7 +312,7 @@ def _main():
     ap.add_argument("--train", action="store_true")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--policy", default="clean", choices=("clean", "cheapest"))
+    ap.add_argument("--policy", default="weighted", choices=("clean", "cheapest", "weighted"))
     a = ap.parse_args()
     if not a.train:
         ap.print_help()

@@ -140,6 +140,16 @@ __device__ __forceinline__ bool dip_forces(int m, int ls_plain, Dt mn, Dt mx) {
 // for GL-3 builds, GL-8 for GL-5 builds) to the same tolerance. Narrower
 // segments keep the plain method's test alone.
 constexpr int kCrossBelow = 3;
+// Segments wider than a plain root whose six node speeds vary by 2x or more
+// (min < kCrossRatio max) take the check too: on the 100,000 dataset curves
+// such segments of 1/8 to 1/32 of their interval, near a slow point (speed
+// ratio 0.09-0.31), passed the halves test up to 1e-5 of their curve's
+// length off, and the test costs little there (the segments are few).
+constexpr float kCrossRatio = 0.5f;
+template <typename Dt>
+__device__ __forceinline__ bool cross_wanted(int m, int ls_plain, Dt mn, Dt mx) {
+    return m < kCrossBelow || (m < ls_plain && mn < Dt(kCrossRatio) * mx);
+}
 template <int NGL> struct CrossRule;
 template <> struct CrossRule<3> { static constexpr int n = 5; };
 template <> struct CrossRule<5> { static constexpr int n = 8; };

@@ -310,9 +310,10 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
         L = quad_mm<NGL>(ev, ctx, x0, xm, mn, mx);
         R = quad_mm<NGL>(ev, ctx, xm, x1, mn, mx);
         dip = dip_forces(ls, ls_plain, mn, mx);
-        if (ls < kCrossBelow) {
+        if (ls < ls_plain) {
             // only where the other tests accept the root (the same decision)
-            const bool need = valid && !dip && !(dabs((L + R) - P) > A.eps0);
+            const bool need =
+                valid && !dip && !(dabs((L + R) - P) > A.eps0) && cross_wanted(ls, ls_plain, mn, mx);
             if (ballot(need)) {
                 // bounds through an empty asm: the check's node positions are
                 // formed here per tile, not hoisted into registers held for
@@ -366,11 +367,11 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
             task_bounds<Dt>(r & (nsub - 1), l, idx, ls, a0, am, a1);
             Dt L2, R2;
             bool dip2 = false;
+            Dt mn2 = Dt(3.0e38f), mx2 = Dt(0);
             if constexpr (kGuard) {
-                Dt mn = Dt(3.0e38f), mx = Dt(0);
-                L2 = quad_mm<NGL>(ev, cr, a0, am, mn, mx);
-                R2 = quad_mm<NGL>(ev, cr, am, a1, mn, mx);
-                dip2 = dip_forces(ls + l, ls_plain, mn, mx);
+                L2 = quad_mm<NGL>(ev, cr, a0, am, mn2, mx2);
+                R2 = quad_mm<NGL>(ev, cr, am, a1, mn2, mx2);
+                dip2 = dip_forces(ls + l, ls_plain, mn2, mx2);
             } else {
                 L2 = quad<NGL>(ev, cr, a0, am);
                 R2 = quad<NGL>(ev, cr, am, a1);
@@ -379,7 +380,7 @@ __device__ __forceinline__ bool subdivide_tile(const Ev& ev, const SubdivArgs<Dt
             const Dt tol2 = tol(l, lr);
             const bool over = dabs(lr - par) > tol2;
             if constexpr (kGuard) {
-                const bool wide = take && ls + l < kCrossBelow && !over && !dip2;
+                const bool wide = take && !over && !dip2 && cross_wanted(ls + l, ls_plain, mn2, mx2);
                 if (ballot(wide)) {
                     if (wide) dip2 = cross_fails<NGL>(ev, cr, a0, a1, lr, tol2);
                 }
@@ -1459,8 +1460,8 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
         task_bounds<Dt>(k, l, idx, ls, x0, xm, x1);
         Dt L, R;
         bool dip = false;
+        Dt mn = Dt(3.0e38f), mx = Dt(0);
         if (guard) {
-            Dt mn = Dt(3.0e38f), mx = Dt(0);
             L = quad_mm<NGL>(ev, c, x0, xm, mn, mx);
             R = quad_mm<NGL>(ev, c, xm, x1, mn, mx);
             dip = dip_forces(ls + l, ls_plain, mn, mx);
@@ -1472,7 +1473,8 @@ __device__ void walk_to_leaf(const Ev& ev, const typename Ev::Ctx& c, int k, int
         const Dt tol = tol_of(eps0 * pow2m(Dt(0), l), rounding_floor<Dt>(ls, l, ls_plain), lr);
         bool split = dabs(lr - P) > tol || dip;
         // the build's cross-check of wide segments, where the other tests accept
-        if (guard && !split && ls + l < kCrossBelow) split = cross_fails<NGL>(ev, c, x0, x1, lr, tol);
+        if (guard && !split && cross_wanted(ls + l, ls_plain, mn, mx))
+            split = cross_fails<NGL>(ev, c, x0, x1, lr, tol);
         if (l < max_depth && split) {
             stackR[l + 1] = R;
             l++;

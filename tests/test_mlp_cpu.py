@@ -106,16 +106,23 @@ def test_start_targets():
 
 def test_labels_from_costs():
     # interval 0: k=1 passes cleanly; 1: floor 4 forces k>=4; 2: never clean, k=6 cheapest;
-    # 3: k=1 refined but cheapest (9 < 12), k=2 the first clean seed
-    cost = np.array([[30, 3, 9000, 40], [6, 6, 8000, 9], [12, 12, 7000, 12], [24, 24, 6000, 24],
-                     [48, 48, 5000, 48], [96, 96, 4000, 96], [192, 192, 1000, 192]], np.int64)
-    floor = np.array([0, 4, 0, 0], np.int8)
+    # 3: k=1 refined but cheapest (9 < 12), k=2 the first clean seed; 4: k=4 the first clean
+    # seed, k=2 with one refinement task (14 quadratures) cheapest even with refinement
+    # quadratures weighted 6x (12 + 6 * 2 = 24 < 48)
+    cost = np.array([[30, 3, 9000, 40, 63], [6, 6, 8000, 9, 26], [12, 12, 7000, 12, 14], [24, 24, 6000, 24, 26],
+                     [48, 48, 5000, 48, 48], [96, 96, 4000, 96, 96], [192, 192, 1000, 192, 192]], np.int64)
+    floor = np.array([0, 4, 0, 0, 0], np.int8)
     y = S.labels_from_costs(cost, floor, "clean")
-    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 2])
-    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 1])
+    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 2, 4])
+    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 1, 1])
     y = S.labels_from_costs(cost, floor, "cheapest")
-    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 1])
-    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 0])
+    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 1, 2])
+    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 0, 0])
+    assert S.REFINE_WEIGHT == 6
+    y = S.labels_from_costs(cost, floor, "weighted")
+    # interval 3: k=1 weighs 6 + 6 * 3 = 24 > 12, so the clean k=2 wins
+    np.testing.assert_array_equal(y[:, 0], [1, 4, 6, 2, 2])
+    np.testing.assert_array_equal(y[:, 1], [1, 1, 0, 1, 0])
     with pytest.raises(ValueError):
         S.labels_from_costs(cost, floor, "nope")
 
